@@ -1,0 +1,109 @@
+"""ctypes binding of libfftwpp_amd.so (the C ABI of include/fftwpp_amd.h and
+include/cfftw++.h).  Loading fails loudly when the library has not been built:
+there is no CPU fallback."""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libfftwpp_amd.so")
+
+P = ctypes.c_void_p
+Z = ctypes.c_size_t
+I = ctypes.c_int
+D = ctypes.c_double
+ZP = ctypes.POINTER(ctypes.c_size_t)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    # kernel layer
+    "fftwpp_amd_last_error": (ctypes.c_char_p, []),
+    "fftwpp_amd_dim_create": (I, [ctypes.POINTER(P), Z, Z, Z, I]),
+    "fftwpp_amd_dim_destroy": (I, [P]),
+    "fftwpp_amd_max_m": (Z, [I]),
+    "fftwpp_amd_forward_many": (I, [P, I, P, Z, Z, Z, Z, P, Z, Z, Z, Z, P]),
+    "fftwpp_amd_backward_many": (I, [P, I, P, Z, Z, P, Z, Z, Z, Z, Z, Z, I, D, P]),
+    "fftwpp_amd_conv_rows": (I, [P, I, I, P, P, Z, Z, P, Z, D, P]),
+    "fftwpp_amd_multiply": (I, [I, I, P, P, Z, P]),
+    "fftwpp_amd_scale": (I, [I, P, Z, D, P]),
+    "fftwpp_amd_symmetrize_x": (I, [Z, Z, Z, P, Z, P]),
+    "fftwpp_amd_symmetrize_xy": (I, [Z, Z, Z, Z, Z, P, Z, Z, P]),
+    "fftwpp_amd_malloc": (I, [ctypes.POINTER(P), Z]),
+    "fftwpp_amd_free": (I, [P]),
+    "fftwpp_amd_memcpy": (I, [P, P, Z, P]),
+    "fftwpp_amd_memset": (I, [P, Z, P]),
+    "fftwpp_amd_synchronize": (I, [P]),
+    "fftwpp_amd_is_device_pointer": (I, [P]),
+    "fftwpp_amd_device_count": (I, []),
+    # convolution layer
+    "fftwpp_amd_conv_create": (I, [ctypes.POINTER(P), I, I, ZP, ZP, ZP, I]),
+    "fftwpp_amd_conv_destroy": (I, [P]),
+    "fftwpp_amd_conv_execute": (I, [P, P, P, P]),
+    "fftwpp_amd_conv_adjoint": (I, [P, P, P, P, P, P, P]),
+    "fftwpp_amd_conv_geometry": (I, [P, ZP, ZP, ZP]),
+    # cfftw++.h drop-in
+    "create_doubleAlign": (P, [Z]),
+    "delete_doubleAlign": (None, [P]),
+    "create_complexAlign": (P, [Z]),
+    "delete_complexAlign": (None, [P]),
+    "get_fftwpp_maxthreads": (Z, []),
+    "set_fftwpp_maxthreads": (None, [Z]),
+    "fftwpp_create_conv1d": (P, [Z]),
+    "fftwpp_conv1d_convolve": (None, [P, P, P]),
+    "fftwpp_conv1d_delete": (None, [P]),
+    "fftwpp_create_hconv1d": (P, [Z]),
+    "fftwpp_hconv1d_convolve": (None, [P, P, P]),
+    "fftwpp_hconv1d_delete": (None, [P]),
+    "fftwpp_create_conv2d": (P, [Z, Z]),
+    "fftwpp_conv2d_convolve": (None, [P, P, P]),
+    "fftwpp_conv2d_delete": (None, [P]),
+    "fftwpp_create_hconv2d": (P, [Z, Z]),
+    "fftwpp_hconv2d_convolve": (None, [P, P, P]),
+    "fftwpp_hconv2d_delete": (None, [P]),
+    "fftwpp_create_conv3d": (P, [Z, Z, Z]),
+    "fftwpp_conv3d_convolve": (None, [P, P, P]),
+    "fftwpp_conv3d_delete": (None, [P]),
+    "fftwpp_create_hconv3d": (P, [Z, Z, Z]),
+    "fftwpp_hconv3d_convolve": (None, [P, P, P]),
+    "fftwpp_hconv3d_delete": (None, [P]),
+    "fftwpp_HermitianSymmetrize": (None, [P]),
+    "fftwpp_HermitianSymmetrizeX": (None, [Z, Z, Z, P]),
+    "fftwpp_HermitianSymmetrizeXY": (None, [Z, Z, Z, Z, Z, P]),
+}
+
+F64, F32 = 0, 1
+KIND_COMPLEX, KIND_CENTERED, KIND_HERMITIAN = 0, 1, 2
+MULT_NONE, MULT_BINARY, MULT_REAL_BINARY, MULT_CORRELATION = 0, 1, 2, 3
+
+_lib = None
+
+
+class FFTWPPError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fftwpp_amd error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load the in-tree library (raises when it is missing)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError(f"{LIBPATH} is not built; run `python -m fftwpp_amd.build`")
+        L = ctypes.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().fftwpp_amd_last_error()
+        raise FFTWPPError(rc, msg.decode() if msg else "")
+
+
+def sizes(vals):
+    arr = (ctypes.c_size_t * 3)(*([int(v) for v in vals] + [0] * (3 - len(vals))))
+    return arr

@@ -1,0 +1,473 @@
+// capi.cc -- the drop-in C wrapper (include/cfftw++.h) and the convolution
+// layer of include/fftwpp_amd.h, both built on the C++ classes of
+// include/convolve.h.
+#include "convolve.h"
+#include "cfftw++.h"
+
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+
+using namespace fftwpp;
+
+namespace fftwpp {
+
+// The compositions of wrappers/HybridConvolution.h:12-182: default padded
+// sizes, outer transforms with multNone and C = product of the inner stored
+// lengths, the inner transform carrying the binary multiplier.
+struct HybridConvolution {
+  Application *app;
+  fftPad *fft;
+  Convolution *convolve;
+  HybridConvolution(size_t L, size_t M=0, size_t m=0, multiplier mult=multBinary) {
+    if(M == 0) M=2*L-1;
+    app=new Application(2,1,mult,fftw::maxthreads,false,m);
+    fft=new fftPad(L,M,*app);
+    convolve=new Convolution(fft);
+  }
+  ~HybridConvolution() { delete convolve; delete fft; delete app; }
+};
+
+static size_t hdefault(size_t L) { return 3*utils::ceilquotient(L,2)-2*(L % 2); }
+
+// Centred complex 1D convolution: fftPadCentered + Convolution as in
+// tests/hybridconv.cc:44-46 with -c; default M is the 3/2 rule.
+struct CenteredConvolution {
+  Application *app;
+  fftPadCentered *fft;
+  Convolution *convolve;
+  CenteredConvolution(size_t L, size_t M=0, size_t m=0) {
+    if(M == 0) M=hdefault(L);
+    app=new Application(2,1,multBinary,fftw::maxthreads,false,m);
+    fft=new fftPadCentered(L,M,*app);
+    convolve=new Convolution(fft);
+  }
+  ~CenteredConvolution() { delete convolve; delete fft; delete app; }
+};
+
+struct HybridConvolutionHermitian {
+  Application *app;
+  fftPadHermitian *fft;
+  Convolution *convolve;
+  HybridConvolutionHermitian(size_t L, size_t M=0, size_t m=0) {
+    if(M == 0) M=hdefault(L);
+    app=new Application(2,1,realMultBinary,fftw::maxthreads,false,m);
+    fft=new fftPadHermitian(L,M,*app);
+    convolve=new Convolution(fft);
+  }
+  ~HybridConvolutionHermitian() { delete convolve; delete fft; delete app; }
+};
+
+struct HybridConvolution2 {
+  Application *appx, *appy;
+  fftPad *fftx, *ffty;
+  Convolution2 *convolve2;
+  HybridConvolution2(size_t Lx, size_t Ly, size_t Mx=0, size_t My=0,
+                     size_t mx=0, size_t my=0, multiplier mult=multBinary) {
+    if(Mx == 0) Mx=2*Lx-1;
+    if(My == 0) My=2*Ly-1;
+    appx=new Application(2,1,multNone,fftw::maxthreads,false,mx);
+    fftx=new fftPad(Lx,Mx,*appx,Ly);
+    appy=new Application(2,1,mult,*appx,my);
+    ffty=new fftPad(Ly,My,*appy);
+    convolve2=new Convolution2(fftx,ffty);
+  }
+  ~HybridConvolution2() {
+    delete convolve2; delete ffty; delete appy; delete fftx; delete appx;
+  }
+};
+
+struct HybridConvolutionHermitian2 {
+  Application *appx, *appy;
+  fftPadCentered *fftx;
+  fftPadHermitian *ffty;
+  Convolution2 *convolve2;
+  HybridConvolutionHermitian2(size_t Lx, size_t Ly, size_t Mx=0, size_t My=0,
+                              size_t mx=0, size_t my=0) {
+    if(Mx == 0) Mx=hdefault(Lx);
+    if(My == 0) My=hdefault(Ly);
+    size_t Hy=utils::ceilquotient(Ly,2);
+    appx=new Application(2,1,multNone,fftw::maxthreads,false,mx);
+    fftx=new fftPadCentered(Lx,Mx,*appx,Hy,Hy);
+    appy=new Application(2,1,realMultBinary,*appx,my);
+    ffty=new fftPadHermitian(Ly,My,*appy);
+    convolve2=new Convolution2(fftx,ffty);
+  }
+  ~HybridConvolutionHermitian2() {
+    delete convolve2; delete ffty; delete appy; delete fftx; delete appx;
+  }
+};
+
+struct HybridConvolution3 {
+  Application *appx, *appy, *appz;
+  fftPad *fftx, *ffty, *fftz;
+  Convolution3 *convolve3;
+  HybridConvolution3(size_t Lx, size_t Ly, size_t Lz, size_t Mx=0,
+                     size_t My=0, size_t Mz=0, size_t mx=0, size_t my=0,
+                     size_t mz=0, multiplier mult=multBinary) {
+    if(Mx == 0) Mx=2*Lx-1;
+    if(My == 0) My=2*Ly-1;
+    if(Mz == 0) Mz=2*Lz-1;
+    appx=new Application(2,1,multNone,fftw::maxthreads,false,mx);
+    fftx=new fftPad(Lx,Mx,*appx,Ly*Lz);
+    appy=new Application(2,1,multNone,*appx,my);
+    ffty=new fftPad(Ly,My,*appy,Lz);
+    appz=new Application(2,1,mult,*appy,mz);
+    fftz=new fftPad(Lz,Mz,*appz);
+    convolve3=new Convolution3(fftx,ffty,fftz);
+  }
+  ~HybridConvolution3() {
+    delete convolve3; delete fftz; delete appz; delete ffty; delete appy;
+    delete fftx; delete appx;
+  }
+};
+
+struct HybridConvolutionHermitian3 {
+  Application *appx, *appy, *appz;
+  fftPadCentered *fftx, *ffty;
+  fftPadHermitian *fftz;
+  Convolution3 *convolve3;
+  HybridConvolutionHermitian3(size_t Lx, size_t Ly, size_t Lz, size_t Mx=0,
+                              size_t My=0, size_t Mz=0, size_t mx=0,
+                              size_t my=0, size_t mz=0) {
+    if(Mx == 0) Mx=hdefault(Lx);
+    if(My == 0) My=hdefault(Ly);
+    if(Mz == 0) Mz=hdefault(Lz);
+    size_t Hz=utils::ceilquotient(Lz,2);
+    appx=new Application(2,1,multNone,fftw::maxthreads,false,mx);
+    fftx=new fftPadCentered(Lx,Mx,*appx,Ly*Hz);
+    appy=new Application(2,1,multNone,*appx,my);
+    ffty=new fftPadCentered(Ly,My,*appy,Hz);
+    appz=new Application(2,1,realMultBinary,*appy,mz);
+    fftz=new fftPadHermitian(Lz,Mz,*appz);
+    convolve3=new Convolution3(fftx,ffty,fftz);
+  }
+  ~HybridConvolutionHermitian3() {
+    delete convolve3; delete fftz; delete appz; delete ffty; delete appy;
+    delete fftx; delete appx;
+  }
+};
+
+} // namespace fftwpp
+
+static thread_local char capi_error[512];
+
+extern "C" void fftwpp_amd_internal_set_error(const char *msg);
+
+static void report_error(const char *where, const std::exception &e)
+{
+  snprintf(capi_error,sizeof(capi_error),"%s: %s",where,e.what());
+  fftwpp_amd_internal_set_error(capi_error);
+  std::cerr << capi_error << std::endl;
+}
+
+template<class F>
+static auto guard(const char *where, F f) -> decltype(f())
+{
+  try {
+    return f();
+  } catch(const std::exception &e) {
+    report_error(where,e);
+    return decltype(f())();
+  }
+}
+
+extern "C" {
+namespace fftwpp {
+
+double *create_doubleAlign(size_t n) { return utils::doubleAlign(n); }
+void delete_doubleAlign(double *p) { utils::deleteAlign(p); }
+double *create_complexAlign(size_t n) { return (double *) utils::ComplexAlign(n); }
+void delete_complexAlign(double *p) { utils::deleteAlign(p); }
+
+size_t get_fftwpp_maxthreads(void) { return fftw::maxthreads; }
+void set_fftwpp_maxthreads(size_t nthreads) { fftw::maxthreads=nthreads; }
+
+HybridConvolution *fftwpp_create_conv1d(size_t L)
+{
+  return guard("fftwpp_create_conv1d",[&]() { return new HybridConvolution(L); });
+}
+void fftwpp_conv1d_delete(HybridConvolution *conv) { delete conv; }
+void fftwpp_conv1d_convolve(HybridConvolution *conv, double *a, double *b)
+{
+  guard("fftwpp_conv1d_convolve",[&]() {
+    Complex *F[]={(Complex *) a,(Complex *) b};
+    conv->convolve->convolve(F);
+    return 0;
+  });
+}
+
+HybridConvolutionHermitian *fftwpp_create_hconv1d(size_t L)
+{
+  return guard("fftwpp_create_hconv1d",[&]() { return new HybridConvolutionHermitian(L); });
+}
+void fftwpp_hconv1d_delete(HybridConvolutionHermitian *conv) { delete conv; }
+void fftwpp_hconv1d_convolve(HybridConvolutionHermitian *conv, double *a,
+                             double *b)
+{
+  guard("fftwpp_hconv1d_convolve",[&]() {
+    Complex *F[]={(Complex *) a,(Complex *) b};
+    conv->convolve->convolve(F);
+    return 0;
+  });
+}
+
+HybridConvolution2 *fftwpp_create_conv2d(size_t Lx, size_t Ly)
+{
+  return guard("fftwpp_create_conv2d",[&]() { return new HybridConvolution2(Lx,Ly); });
+}
+void fftwpp_conv2d_delete(HybridConvolution2 *conv) { delete conv; }
+void fftwpp_conv2d_convolve(HybridConvolution2 *conv, double *a, double *b)
+{
+  guard("fftwpp_conv2d_convolve",[&]() {
+    Complex *F[]={(Complex *) a,(Complex *) b};
+    conv->convolve2->convolve(F);
+    return 0;
+  });
+}
+
+HybridConvolutionHermitian2 *fftwpp_create_hconv2d(size_t Lx, size_t Ly)
+{
+  return guard("fftwpp_create_hconv2d",[&]() { return new HybridConvolutionHermitian2(Lx,Ly); });
+}
+void fftwpp_hconv2d_delete(HybridConvolutionHermitian2 *conv) { delete conv; }
+void fftwpp_hconv2d_convolve(HybridConvolutionHermitian2 *conv, double *a,
+                             double *b)
+{
+  guard("fftwpp_hconv2d_convolve",[&]() {
+    Complex *F[]={(Complex *) a,(Complex *) b};
+    conv->convolve2->convolve(F);
+    return 0;
+  });
+}
+
+HybridConvolution3 *fftwpp_create_conv3d(size_t Lx, size_t Ly, size_t Lz)
+{
+  return guard("fftwpp_create_conv3d",[&]() { return new HybridConvolution3(Lx,Ly,Lz); });
+}
+void fftwpp_conv3d_delete(HybridConvolution3 *conv) { delete conv; }
+void fftwpp_conv3d_convolve(HybridConvolution3 *conv, double *a, double *b)
+{
+  guard("fftwpp_conv3d_convolve",[&]() {
+    Complex *F[]={(Complex *) a,(Complex *) b};
+    conv->convolve3->convolve(F);
+    return 0;
+  });
+}
+
+HybridConvolutionHermitian3 *fftwpp_create_hconv3d(size_t Lx, size_t Ly,
+                                                   size_t Lz)
+{
+  return guard("fftwpp_create_hconv3d",[&]() { return new HybridConvolutionHermitian3(Lx,Ly,Lz); });
+}
+void fftwpp_hconv3d_delete(HybridConvolutionHermitian3 *conv) { delete conv; }
+void fftwpp_hconv3d_convolve(HybridConvolutionHermitian3 *conv, double *a,
+                             double *b)
+{
+  guard("fftwpp_hconv3d_convolve",[&]() {
+    Complex *F[]={(Complex *) a,(Complex *) b};
+    conv->convolve3->convolve(F);
+    return 0;
+  });
+}
+
+void fftwpp_HermitianSymmetrize(double *f)
+{
+  guard("fftwpp_HermitianSymmetrize",[&]() { HermitianSymmetrize((Complex *) f); return 0; });
+}
+
+void fftwpp_HermitianSymmetrizeX(size_t Hx, size_t Hy, size_t x0, double *f)
+{
+  guard("fftwpp_HermitianSymmetrizeX",[&]() {
+    HermitianSymmetrizeX(Hx,Hy,x0,(Complex *) f);
+    return 0;
+  });
+}
+
+void fftwpp_HermitianSymmetrizeXY(size_t Hx, size_t Hy, size_t Hz, size_t x0,
+                                  size_t y0, double *f)
+{
+  guard("fftwpp_HermitianSymmetrizeXY",[&]() {
+    HermitianSymmetrizeXY(Hx,Hy,Hz,x0,y0,(Complex *) f);
+    return 0;
+  });
+}
+
+} // namespace fftwpp
+} // extern "C"
+
+// ------------------------------------------------------------------------
+// Convolution layer of fftwpp_amd.h
+// ------------------------------------------------------------------------
+struct fftwpp_amd_conv {
+  int dim, hermitian, dtype;
+  HybridConvolution *c1;
+  HybridConvolutionHermitian *h1;
+  CenteredConvolution *k1;
+  HybridConvolution2 *c2;
+  HybridConvolutionHermitian2 *h2;
+  HybridConvolution3 *c3;
+  HybridConvolutionHermitian3 *h3;
+  size_t words;   // complex words per array
+  fftwpp::DeviceWork work;
+
+  fftBase *dims[3];
+  void exec(void *a, const void *b, void *stream, int mult) {
+    bool norm=true;
+    if(c1) c1->convolve->execDevice(a,b,norm,dtype,stream,mult);
+    else if(h1) h1->convolve->execDevice(a,b,norm,dtype,stream,mult);
+    else if(k1) k1->convolve->execDevice(a,b,norm,dtype,stream,mult);
+    else if(c2) c2->convolve2->execDevice(a,b,norm,dtype,stream,mult);
+    else if(h2) h2->convolve2->execDevice(a,b,norm,dtype,stream,mult);
+    else if(c3) c3->convolve3->execDevice(a,b,norm,dtype,stream,mult);
+    else if(h3) h3->convolve3->execDevice(a,b,norm,dtype,stream,mult);
+  }
+};
+
+static int conv_status(const char *where, const std::exception &e)
+{
+  const fftwpp::Error *fe=dynamic_cast<const fftwpp::Error *>(&e);
+  snprintf(capi_error,sizeof(capi_error),"%s: %s",where,e.what());
+  fftwpp_amd_internal_set_error(capi_error);
+  return fe ? fe->code : FFTWPP_AMD_ERR_ARG;
+}
+
+extern "C" {
+
+int fftwpp_amd_conv_create(fftwpp_amd_conv **h, int dim, int hermitian,
+                           const size_t *L, const size_t *M, const size_t *m,
+                           int dtype)
+{
+  if(!h || !L) return FFTWPP_AMD_ERR_ARG;
+  *h=NULL;
+  if(dim < 1 || dim > 3) return FFTWPP_AMD_ERR_ARG;
+  if(dtype != FFTWPP_AMD_F64 && dtype != FFTWPP_AMD_F32) return FFTWPP_AMD_ERR_ARG;
+  size_t Mz[3]={0,0,0}, mz[3]={0,0,0};
+  for(int d=0; d < dim; ++d) {
+    if(L[d] == 0) return FFTWPP_AMD_ERR_ARG;
+    if(M) Mz[d]=M[d];
+    if(m) mz[d]=m[d];
+  }
+  fftwpp_amd_conv *c=new fftwpp_amd_conv();
+  c->dim=dim; c->hermitian=hermitian; c->dtype=dtype;
+  try {
+    if(hermitian == 2) {
+      if(dim != 1) throw fftwpp::Error(FFTWPP_AMD_ERR_UNSUPPORTED,"centred complex convolutions are one-dimensional");
+      c->k1=new CenteredConvolution(L[0],Mz[0],mz[0]);
+      c->words=L[0];
+      c->dims[0]=c->k1->fft;
+    } else if(dim == 1) {
+      if(hermitian) {
+        c->h1=new HybridConvolutionHermitian(L[0],Mz[0],mz[0]);
+        c->words=(L[0]+1)/2;
+        c->dims[0]=c->h1->fft;
+      } else {
+        c->c1=new HybridConvolution(L[0],Mz[0],mz[0]);
+        c->words=L[0];
+        c->dims[0]=c->c1->fft;
+      }
+    } else if(dim == 2) {
+      if(hermitian) {
+        c->h2=new HybridConvolutionHermitian2(L[0],L[1],Mz[0],Mz[1],mz[0],mz[1]);
+        c->words=L[0]*((L[1]+1)/2);
+        c->dims[0]=c->h2->fftx; c->dims[1]=c->h2->ffty;
+      } else {
+        c->c2=new HybridConvolution2(L[0],L[1],Mz[0],Mz[1],mz[0],mz[1]);
+        c->words=L[0]*L[1];
+        c->dims[0]=c->c2->fftx; c->dims[1]=c->c2->ffty;
+      }
+    } else {
+      if(hermitian) {
+        c->h3=new HybridConvolutionHermitian3(L[0],L[1],L[2],Mz[0],Mz[1],Mz[2],
+                                              mz[0],mz[1],mz[2]);
+        c->words=L[0]*L[1]*((L[2]+1)/2);
+        c->dims[0]=c->h3->fftx; c->dims[1]=c->h3->ffty; c->dims[2]=c->h3->fftz;
+      } else {
+        c->c3=new HybridConvolution3(L[0],L[1],L[2],Mz[0],Mz[1],Mz[2],
+                                     mz[0],mz[1],mz[2]);
+        c->words=L[0]*L[1]*L[2];
+        c->dims[0]=c->c3->fftx; c->dims[1]=c->c3->ffty; c->dims[2]=c->c3->fftz;
+      }
+    }
+  } catch(const std::exception &e) {
+    int rc=conv_status("fftwpp_amd_conv_create",e);
+    fftwpp_amd_conv_destroy(c);
+    return rc;
+  }
+  *h=c;
+  return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_conv_destroy(fftwpp_amd_conv *c)
+{
+  if(!c) return FFTWPP_AMD_OK;
+  delete c->c1; delete c->h1; delete c->k1; delete c->c2; delete c->h2; delete c->c3;
+  delete c->h3;
+  delete c;
+  return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_conv_execute(fftwpp_amd_conv *c, void *a, const void *b,
+                            void *stream)
+{
+  if(!c || !a || !b) return FFTWPP_AMD_ERR_ARG;
+  if(!fftwpp_amd_is_device_pointer(a) || !fftwpp_amd_is_device_pointer(b)) {
+    fftwpp_amd_internal_set_error("fftwpp_amd_conv_execute: a and b must be device pointers");
+    return FFTWPP_AMD_ERR_ARG;
+  }
+  try {
+    c->exec(a,b,stream,-1);
+  } catch(const std::exception &e) {
+    return conv_status("fftwpp_amd_conv_execute",e);
+  }
+  return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_conv_adjoint(fftwpp_amd_conv *c, const void *w, const void *a,
+                            const void *b, void *grad_a, void *grad_b,
+                            void *stream)
+{
+  if(!c || !w) return FFTWPP_AMD_ERR_ARG;
+  if(c->hermitian == 1) {
+    fftwpp_amd_internal_set_error("fftwpp_amd_conv_adjoint: Hermitian adjoint not implemented");
+    return FFTWPP_AMD_ERR_UNSUPPORTED;
+  }
+  size_t bytes=c->words*(c->dtype == FFTWPP_AMD_F32 ? 8 : 16);
+  const void *ptrs[5]={w,a,b,grad_a,grad_b};
+  for(const void *p : ptrs)
+    if(p && !fftwpp_amd_is_device_pointer(p)) {
+      fftwpp_amd_internal_set_error("fftwpp_amd_conv_adjoint: arguments must be device pointers");
+      return FFTWPP_AMD_ERR_ARG;
+    }
+  try {
+    if(grad_a) {
+      if(!b) return FFTWPP_AMD_ERR_ARG;
+      int rc=fftwpp_amd_memcpy(grad_a,w,bytes,stream);
+      if(rc) return rc;
+      c->exec(grad_a,b,stream,FFTWPP_AMD_MULT_CORRELATION);
+    }
+    if(grad_b) {
+      if(!a) return FFTWPP_AMD_ERR_ARG;
+      int rc=fftwpp_amd_memcpy(grad_b,w,bytes,stream);
+      if(rc) return rc;
+      c->exec(grad_b,a,stream,FFTWPP_AMD_MULT_CORRELATION);
+    }
+  } catch(const std::exception &e) {
+    return conv_status("fftwpp_amd_conv_adjoint",e);
+  }
+  return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_conv_geometry(const fftwpp_amd_conv *c, size_t *N, size_t *m,
+                             size_t *q)
+{
+  if(!c) return FFTWPP_AMD_ERR_ARG;
+  for(int d=0; d < c->dim; ++d) {
+    if(N) N[d]=c->dims[d]->m*c->dims[d]->q;
+    if(m) m[d]=c->dims[d]->m;
+    if(q) q[d]=c->dims[d]->q;
+  }
+  return FFTWPP_AMD_OK;
+}
+
+} // extern "C"

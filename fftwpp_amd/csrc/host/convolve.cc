@@ -1,0 +1,814 @@
+// convolve.cc -- host side of the drop-in C++ API (include/convolve.h).
+// Pure host C++ (compiled with g++): every device action goes through the
+// C ABI of include/fftwpp_amd.h.
+#include "convolve.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <sstream>
+
+namespace fftwpp {
+
+const double twopi=2.0*3.14159265358979323846264338327950288;
+bool showOptTimes=false;
+bool showRoutines=false;
+
+namespace fftw {
+size_t maxthreads=1;
+}
+
+static void check(int rc, const char *what)
+{
+  if(rc != FFTWPP_AMD_OK) {
+    std::ostringstream s;
+    s << what << ": " << fftwpp_amd_last_error();
+    throw Error(rc,s.str());
+  }
+}
+
+namespace utils {
+
+size_t ceilpow2(size_t n)
+{
+  --n;
+  n |= n >> 1; n |= n >> 2; n |= n >> 4; n |= n >> 8; n |= n >> 16; n |= n >> 32;
+  return ++n;
+}
+
+Complex *ComplexAlign(size_t size)
+{
+  if(size == 0) return NULL;
+  void *p=NULL;
+  if(posix_memalign(&p,32,size*sizeof(Complex)) != 0) throw Error(FFTWPP_AMD_ERR_ARG,"out of host memory");
+  Complex *v=(Complex *) p;
+  for(size_t i=0; i < size; ++i) v[i]=Complex(0.0,0.0);
+  return v;
+}
+
+Complex **ComplexAlign(size_t n, size_t size)
+{
+  if(n == 0 || size == 0) return NULL;
+  Complex **v=new Complex*[n];
+  size_t Size=2*ceilquotient(size,2);
+  Complex *B=ComplexAlign((n-1)*Size+size);
+  for(size_t i=0; i < n; ++i) v[i]=B+i*Size;
+  return v;
+}
+
+double *doubleAlign(size_t size)
+{
+  if(size == 0) return NULL;
+  void *p=NULL;
+  if(posix_memalign(&p,32,size*sizeof(double)) != 0) throw Error(FFTWPP_AMD_ERR_ARG,"out of host memory");
+  return (double *) p;
+}
+
+void deleteAlign(void *p) { free(p); }
+
+} // namespace utils
+
+// Smallest N >= m of the form 2^a 3^b 5^c 7^d (convolve.cc:114-124).
+size_t nextfftsize(size_t m)
+{
+  size_t N=utils::ceilpow2(m);
+  if(m == N) return m;
+  for(size_t ni=1; ni < N; ni *= 7)
+    for(size_t nj=ni; nj < N; nj *= 5)
+      for(size_t nk=nj; nk < N; nk *= 3)
+        N=std::min(N,nk*utils::ceilpow2(utils::ceilquotient(m,nk)));
+  return N;
+}
+
+void Indices::copy(Indices *indices, size_t size0)
+{
+  size=indices ? indices->size : size0;
+  if(size > maxsize) {
+    if(maxsize > 0) delete [] index;
+    index=new size_t[size];
+    maxsize=size;
+  }
+  if(indices)
+    for(size_t d=1; d < size; ++d) index[d]=indices->index[d];
+}
+
+Indices::~Indices()
+{
+  if(maxsize > 0) delete [] index;
+}
+
+// ---- multipliers on device arrays (convolve.cc:26-110)
+void multNone(Complex **, size_t, Indices *, size_t) {}
+
+void multBinary(Complex **F, size_t n, Indices *, size_t)
+{
+  check(fftwpp_amd_multiply(FFTWPP_AMD_F64,FFTWPP_AMD_MULT_BINARY,F[0],F[1],n,NULL),"multBinary");
+}
+
+void realMultBinary(Complex **F, size_t n, Indices *, size_t)
+{
+  // n counts doubles, as in the reference
+  check(fftwpp_amd_multiply(FFTWPP_AMD_F64,FFTWPP_AMD_MULT_REAL_BINARY,F[0],F[1],n,NULL),"realMultBinary");
+}
+
+void multcorrelation(Complex **F, size_t n, Indices *, size_t)
+{
+  check(fftwpp_amd_multiply(FFTWPP_AMD_F64,FFTWPP_AMD_MULT_CORRELATION,F[0],F[1],n,NULL),"multcorrelation");
+}
+
+static int multiplierCode(multiplier *mult)
+{
+  if(mult == multNone) return FFTWPP_AMD_MULT_NONE;
+  if(mult == multBinary) return FFTWPP_AMD_MULT_BINARY;
+  if(mult == realMultBinary) return FFTWPP_AMD_MULT_REAL_BINARY;
+  if(mult == multcorrelation) return FFTWPP_AMD_MULT_CORRELATION;
+  return -1;
+}
+
+void Application::check()
+{
+  if(m == 1) {
+    std::cerr << std::endl << "WARNING: m=1 changed to m=0 to force optimization." << std::endl;
+    m=0;
+  }
+}
+
+// ------------------------------------------------------------------------
+// fftBase: parameter algebra (convolve.cc:403-410, 480-509; convolve.h:297-451)
+// ------------------------------------------------------------------------
+void fftBase::parameters(size_t L, size_t M, size_t m, bool centered,
+                         size_t &p, size_t &n, size_t &q)
+{
+  p=utils::ceilquotient(L,m);
+  size_t P=(centered && p == 2*(p/2)) || p == 2 ? (p/2) : p;
+  n=utils::ceilquotient(M,P*m);
+  q=P*n;
+}
+
+void fftBase::checkParameters()
+{
+  if(L > M) {
+    std::ostringstream s;
+    s << "L=" << L << " is greater than M=" << M << ".";
+    throw Error(FFTWPP_AMD_ERR_ARG,s.str());
+  }
+  if(S < C) throw Error(FFTWPP_AMD_ERR_ARG,"stride S cannot be less than count C");
+}
+
+void fftBase::common()
+{
+  parameters(L,M,m,centered,p,n,q);
+  if(q*m < M) {
+    std::ostringstream s;
+    s << "Invalid parameters: q=" << q << " m=" << m << " M=" << M;
+    throw Error(FFTWPP_AMD_ERR_ARG,s.str());
+  }
+  Cm=C*m;
+  Sm=S*m;
+  M=m*q;
+  overwrite=false;
+}
+
+void fftBase::invalid()
+{
+  std::ostringstream s;
+  s << "Invalid parameters: m=" << m << " p=" << p << " q=" << q << " n=" << n
+    << "  D=" << D << " S=" << S;
+  throw Error(FFTWPP_AMD_ERR_ARG,s.str());
+}
+
+fftBase::fftBase(size_t L, size_t M, Application &app, size_t C, size_t S,
+                 bool centered) :
+  ThreadBase(app.Threads()), L(L), M(M), C(C), S(S == 0 ? C : S), m(0), p(0),
+  q(0), n(0), R(0), dr(0), D(0), D0(0), Cm(0), Sm(0), l(0), b(0),
+  inplace(false), app(app), centered(centered), overwrite(false), dev(NULL)
+{
+  checkParameters();
+}
+
+fftBase::fftBase(size_t L, size_t M, Application &app, size_t C, size_t S,
+                 size_t m, size_t D, bool inplace, bool centered) :
+  ThreadBase(app.Threads()), L(L), M(M), C(C), S(S == 0 ? C : S), m(m), p(0),
+  q(0), n(0), R(0), dr(0), D(D), D0(0), Cm(0), Sm(0), l(0), b(0),
+  inplace(inplace), app(app), centered(centered), overwrite(false), dev(NULL)
+{
+  checkParameters();
+  this->app.D=D;
+}
+
+fftBase::~fftBase()
+{
+  fftwpp_amd_dim_destroy(dev);
+}
+
+size_t fftBase::index(size_t r, size_t i)
+{
+  if(q == 1) return i;
+  size_t P=utils::ceilquotient(p,2);
+  if(overwrite && centered && r == 2) {
+    if(i < m) i=(i-1) % m;
+    if(p > 2) i=(i+m) % (P*m);
+  }
+  size_t s=i % m;
+  size_t u;
+  if(D > 1 && ((centered && p % 2 == 0) || p <= 2)) {
+    u=(i/m) % P;
+    size_t offset=r == 0 && i >= P*m && D0 % 2 == 1 ? 1 : 0;
+    size_t incr=(i+P*m*offset)/(2*P*m);
+    r += incr;
+    if(i/(P*m)-2*incr+offset == 1) {
+      if((!centered && p == 2) || (r > 0 && u == 0))
+        s=s > 0 ? s-1 : m-1;
+      if(r == 0)
+        r=n/2;
+      else {
+        r=n-r;
+        u=u > 0 ? u-1 : P-1;
+      }
+    }
+  } else {
+    u=(i/m) % p;
+    r += i/(p*m);
+  }
+  return q*s+n*u+r;
+}
+
+void fftBase::initDevice()
+{
+  fftwpp_amd_dim_destroy(dev);
+  dev=NULL;
+  check(fftwpp_amd_dim_create(&dev,L,m,q,kind()),"device twiddles");
+}
+
+// Residues of block r and, for each output k < noutputs(r), the slot*m + s
+// of padded index index(r,k) = q*s + residue.
+void fftBase::blockResidues(size_t r, std::vector<size_t> &res,
+                            std::vector<uint32_t> &map)
+{
+  size_t nk=noutputs(r);
+  res.clear();
+  map.resize(nk);
+  for(size_t k=0; k < nk; ++k) {
+    size_t i=index(r,k);
+    size_t rr=i % q, s=i/q;
+    size_t slot=0;
+    while(slot < res.size() && res[slot] != rr) ++slot;
+    if(slot == res.size()) res.push_back(rr);
+    map[k]=(uint32_t) (slot*m+s);
+  }
+}
+
+void fftBase::forward(Complex *f, Complex *F, size_t r, Complex *)
+{
+  if(kind() == FFTWPP_AMD_KIND_HERMITIAN || overwrite)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"direct forward() of Hermitian or overwrite transforms");
+  if(!fftwpp_amd_is_device_pointer(f) || !fftwpp_amd_is_device_pointer(F))
+    throw Error(FFTWPP_AMD_ERR_ARG,"forward() takes device pointers");
+  std::vector<size_t> res;
+  std::vector<uint32_t> map;
+  blockResidues(r,res,map);
+  // the reference layouts of D=1 blocks place residue n*u+r at F+u*m*S
+  for(size_t k=0; k < map.size(); k += m) {
+    size_t slot=map[k]/m;
+    for(size_t s=0; s < m; ++s)
+      if(map[k+s] != slot*m+s)
+        throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"forward() of conjugate-pair (D>1) blocks");
+    check(fftwpp_amd_forward_many(dev,FFTWPP_AMD_F64,f,S,C,1,0,F+k*S,S,0,
+                                  res[slot],1,NULL),"forward");
+  }
+}
+
+void fftBase::backward(Complex *F, Complex *f, size_t r, Complex *)
+{
+  if(kind() == FFTWPP_AMD_KIND_HERMITIAN || overwrite)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"direct backward() of Hermitian or overwrite transforms");
+  if(!fftwpp_amd_is_device_pointer(f) || !fftwpp_amd_is_device_pointer(F))
+    throw Error(FFTWPP_AMD_ERR_ARG,"backward() takes device pointers");
+  std::vector<size_t> res;
+  std::vector<uint32_t> map;
+  blockResidues(r,res,map);
+  bool first=true;
+  for(size_t k=0; k < map.size(); k += m) {
+    size_t slot=map[k]/m;
+    for(size_t s=0; s < m; ++s)
+      if(map[k+s] != slot*m+s)
+        throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"backward() of conjugate-pair (D>1) blocks");
+    check(fftwpp_amd_backward_many(dev,FFTWPP_AMD_F64,F+k*S,S,0,f,S,C,1,0,
+                                   res[slot],1,r > 0 || !first,1.0,NULL),"backward");
+    first=false;
+  }
+}
+
+// ------------------------------------------------------------------------
+// GPU parameter choice (replaces the CPU timing optimiser, convolve.cc:177-470)
+// ------------------------------------------------------------------------
+struct Choice { size_t m, D; bool inplace; };
+
+static size_t maxm() { return fftwpp_amd_max_m(FFTWPP_AMD_F64); }
+
+// kind: 0 complex, 1 centred, 2 Hermitian
+static Choice choose(size_t L, size_t M, Application &app, size_t C, size_t S,
+                     bool Explicit, int kind)
+{
+  Choice c;
+  c.inplace=app.I >= 0 ? app.I != 0 : false;
+  bool centered=kind != 0;
+  if(Explicit) {
+    c.m=(app.m >= 1 && app.m >= M) ? app.m : nextfftsize(M);
+    c.D=1;
+    return c;
+  }
+  if(app.m >= 1) {
+    c.m=app.m;
+    size_t p, n, q;
+    fftBase::parameters(L,M,c.m,centered,p,n,q);
+    if(app.D > 0) c.D=app.D;
+    else c.D=(kind == 2 && q > 1) ? 2 : 1;
+    return c;
+  }
+  size_t best_m=0, best_N=0;
+  size_t lim=maxm();
+  // candidate sub-FFT sizes: every split of L into p blocks of smooth m
+  for(size_t pp=1; pp <= 64; ++pp) {
+    size_t m=nextfftsize(utils::ceilquotient(L,pp));
+    if(m > lim) continue;
+    size_t p, n, q;
+    fftBase::parameters(L,M,m,centered,p,n,q);
+    bool ok;
+    if(kind == 0) ok=fftPad::valid(m,p,q,n,1,S);
+    else if(kind == 1) ok=fftPadCentered::validCentered(m,p,q,n,1,S);
+    else ok=fftPadHermitian::validHermitian(m,p,q,n,q > 1 ? 2 : 1,C);
+    if(!ok) continue;
+    size_t N=m*q;
+    // prefer the smallest padded length, then the larger sub-FFT
+    if(best_m == 0 || N < best_N || (N == best_N && m > best_m)) {
+      best_m=m;
+      best_N=N;
+    }
+  }
+  if(best_m == 0) {
+    // explicit padding as the last resort (convolve.cc:431-434)
+    c.m=nextfftsize(M);
+    c.D=1;
+    return c;
+  }
+  c.m=best_m;
+  size_t p, n, q;
+  fftBase::parameters(L,M,c.m,centered,p,n,q);
+  c.D=(kind == 2 && q > 1) ? 2 : 1;
+  return c;
+}
+
+static void report(fftBase *fft, const char *name)
+{
+  if(!fft->app.verbose) return;
+  size_t mpL=fft->m*fft->p-fft->L;
+  std::cout << std::endl << "Optimal padding: ";
+  if(fft->p == fft->q) std::cout << "Explicit" << std::endl;
+  else if(mpL > 0) std::cout << "Hybrid" << std::endl;
+  else std::cout << "Implicit" << std::endl;
+  std::cout << "m=" << fft->m << std::endl << "p=" << fft->p << std::endl
+            << "q=" << fft->q << std::endl << "C=" << fft->C << std::endl
+            << "S=" << fft->S << std::endl << "D=" << fft->D << std::endl
+            << "I=" << fft->inplace << std::endl << "Padding: " << mpL
+            << std::endl;
+  if(showRoutines)
+    std::cout << "Forwards Routine: " << name << "::device" << std::endl
+              << "Backwards Routine: " << name << "::device" << std::endl;
+}
+
+// ------------------------------------------------------------------------
+// fftPad / fftPadCentered / fftPadHermitian
+// ------------------------------------------------------------------------
+bool fftPad::valid(size_t, size_t, size_t q, size_t n, size_t D, size_t S)
+{
+  if(q == 1) return D == 1;
+  return D == 1 || (S == 1 && ((D < n && D % 2 == 0) || D == n));
+}
+
+fftPad::fftPad(size_t L, size_t M, Application &app, size_t C, size_t S,
+               bool Explicit) : fftBase(L,M,app,C,S)
+{
+  Choice c=choose(L,M,app,C,this->S,Explicit,0);
+  m=c.m; D=c.D; inplace=c.inplace;
+  if(Explicit) this->M=m;
+  parameters(L,this->M,m,centered,p,n,q);
+  init();
+}
+
+fftPad::fftPad(size_t L, size_t M, Application &app, size_t C, size_t S,
+               size_t m, size_t D, bool inplace) :
+  fftBase(L,M,app,C,S,m,D,inplace)
+{
+  parameters(L,M,m,centered,p,n,q);
+  if(q > 1 && !valid(m,p,q,n,D,this->S)) invalid();
+  init();
+}
+
+fftPad::fftPad(size_t L, size_t M, Application &app, size_t C, size_t S,
+               centeredConstructor) : fftBase(L,M,app,C,S,true) {}
+
+fftPad::fftPad(size_t L, size_t M, Application &app, size_t C, size_t S,
+               size_t m, size_t D, bool inplace, centeredConstructor) :
+  fftBase(L,M,app,C,S,m,D,inplace,true) {}
+
+// Bookkeeping of fftPad::init (convolve.cc:511-719).
+void fftPad::init()
+{
+  common();
+  if(q == 1) {
+    dr=D=D0=R=1;
+    l=M;
+    b=S*l;
+  } else {
+    size_t P;
+    if(p == 2) P=1;
+    else if(centered) P=p/2;
+    else P=p;
+    l=m*P;
+    b=S*l;
+    overwrite=inplace && L == p*m && n == (centered ? 3 : p+1) && D == 1 &&
+      app.A >= app.B;
+    if(!centered && p > 1) overwrite=false;
+    dr=Dr();
+    R=residueBlocks();
+    D0=n % D;
+    if(D0 == 0) D0=D;
+  }
+  initDevice();
+  report(this,centered ? "fftPadCentered" : "fftPad");
+}
+
+fftPadCentered::fftPadCentered(size_t L, size_t M, Application &app,
+                               size_t C, size_t S, bool Explicit) :
+  fftPad(L,M,app,C,S,noOptimize)
+{
+  Choice c=choose(L,M,app,C,this->S,Explicit,1);
+  m=c.m; D=c.D; inplace=c.inplace;
+  if(Explicit) this->M=m;
+  parameters(L,this->M,m,centered,p,n,q);
+  init();
+}
+
+fftPadCentered::fftPadCentered(size_t L, size_t M, Application &app,
+                               size_t C, size_t S, size_t m, size_t D,
+                               bool inplace) :
+  fftPad(L,M,app,C,S,m,D,inplace,noOptimize)
+{
+  parameters(L,M,m,centered,p,n,q);
+  if(q > 1 && !validCentered(m,p,q,n,D,this->S)) invalid();
+  init();
+}
+
+fftPadHermitian::fftPadHermitian(size_t L, size_t M, Application &app,
+                                 size_t C, bool Explicit) :
+  fftBase(L,M,app,C,C,true), e(0), B(0)
+{
+  Choice c=choose(L,M,app,C,C,Explicit,2);
+  m=c.m; D=c.D; inplace=c.inplace;
+  if(Explicit) this->M=m;
+  parameters(L,this->M,m,centered,p,n,q);
+  init();
+}
+
+fftPadHermitian::fftPadHermitian(size_t L, size_t M, Application &app,
+                                 size_t C, size_t m, size_t D, bool inplace) :
+  fftBase(L,M,app,C,C,m,D,inplace,true), e(0), B(0)
+{
+  parameters(L,M,m,centered,p,n,q);
+  if(q > 1 && !validHermitian(m,p,q,n,D,C)) invalid();
+  init();
+}
+
+// Bookkeeping of fftPadHermitian::init (convolve.cc:4309-4421).
+void fftPadHermitian::init()
+{
+  common();
+  S=C;
+  e=m/2+1;
+  if(q == 1) {
+    B=b=C*e;
+    dr=D0=R=1;
+  } else {
+    dr=Dr();
+    size_t p2=p/2;
+    b=utils::align(utils::ceilquotient(p2*Cm,2));
+    B=utils::align(p2*C*e);
+    if(inplace) b=B;
+    R=residueBlocks();
+    D0=n % D;
+    if(D0 == 0) D0=D;
+  }
+  l=blocksize(0);
+  initDevice();
+  report(this,"fftPadHermitian");
+}
+
+// ------------------------------------------------------------------------
+// Device work buffers
+// ------------------------------------------------------------------------
+void *DeviceWork::get(size_t slot, size_t bytes)
+{
+  if(slot >= bufs.size()) {
+    bufs.resize(slot+1,NULL);
+    sizes.resize(slot+1,0);
+  }
+  if(sizes[slot] < bytes) {
+    fftwpp_amd_free(bufs[slot]);
+    bufs[slot]=NULL;
+    sizes[slot]=0;
+    check(fftwpp_amd_malloc(&bufs[slot],bytes),"device work buffer");
+    sizes[slot]=bytes;
+  }
+  return bufs[slot];
+}
+
+DeviceWork::~DeviceWork()
+{
+  for(void *p : bufs) fftwpp_amd_free(p);
+}
+
+static size_t eltsize(int dtype) { return dtype == FFTWPP_AMD_F32 ? 8 : 16; }
+
+// Stage host arrays through device buffers; run `exec` on device pointers.
+template<class Exec>
+static void staged(DeviceWork &work, Complex **f, size_t offset, size_t n,
+                   size_t nin, Exec exec)
+{
+  bool dev0=fftwpp_amd_is_device_pointer(f[0]);
+  bool dev1=fftwpp_amd_is_device_pointer(f[1]);
+  if(dev0 && dev1) {
+    exec(f[0]+offset,f[1]+offset);
+    check(fftwpp_amd_synchronize(NULL),"synchronize");
+    return;
+  }
+  size_t bytes=n*sizeof(Complex);
+  Complex *d0=(Complex *) work.get(100,bytes);
+  Complex *d1=(Complex *) work.get(101,bytes);
+  check(fftwpp_amd_memcpy(d0,f[0]+offset,bytes,NULL),"copy in");
+  if(nin > 1) check(fftwpp_amd_memcpy(d1,f[1]+offset,bytes,NULL),"copy in");
+  exec(d0,d1);
+  check(fftwpp_amd_memcpy(f[0]+offset,d0,bytes,NULL),"copy out");
+  check(fftwpp_amd_synchronize(NULL),"synchronize");
+}
+
+static void requireBinary(size_t A, size_t B, int code)
+{
+  if(code < 0)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"user-defined host multipliers are not supported; use multBinary, realMultBinary or multcorrelation");
+  if(A != 2 || B != 1)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"only binary convolutions (A=2, B=1) run on the device");
+}
+
+// ------------------------------------------------------------------------
+// Convolution (1D)
+// ------------------------------------------------------------------------
+Convolution::Convolution(fftBase *fft, Complex **, Complex *, Complex *) :
+  ThreadBase(fft->Threads()), fft(fft), L(fft->L), A(fft->app.A),
+  B(fft->app.B), mult(fft->app.mult)
+{
+  scale=1.0/normalization();
+  multCode=multiplierCode(mult);
+  indices.copy(NULL,0);
+  indices.fft=fft;
+}
+
+Convolution::~Convolution() {}
+
+size_t Convolution::arraySize() const
+{
+  return fft->S*(fft->inputLength()-1)+fft->C;
+}
+
+void Convolution::convolveDevice(Complex *f0, const Complex *f1,
+                                 bool normalize, void *stream)
+{
+  execDevice(f0,f1,normalize,FFTWPP_AMD_F64,stream);
+}
+
+void Convolution::execDevice(void *f0, const void *f1, bool normalize,
+                             int dtype, void *stream, int mult)
+{
+  int code=mult >= 0 ? mult : multCode;
+  requireBinary(A,B,code);
+  if(fft->C != 1)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"1D Convolution with C > 1");
+  double s=normalize ? scale : 1.0;
+  size_t n=fft->inputLength();
+  if(fft->m <= fftwpp_amd_max_m(dtype)) {
+    check(fftwpp_amd_conv_rows(fft->dev,dtype,code,f0,f1,1,n,f0,n,s,
+                               stream),"conv_rows");
+    return;
+  }
+  if(fft->kind() == FFTWPP_AMD_KIND_HERMITIAN)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"Hermitian sub-FFT size beyond the LDS limit");
+  // unfused: all residues forward, multiply, backward
+  size_t E=eltsize(dtype);
+  size_t words=fft->q*fft->m;
+  void *T0=work.get(0,words*E);
+  void *T1=work.get(1,words*E);
+  check(fftwpp_amd_forward_many(fft->dev,dtype,f0,1,1,1,0,T0,1,0,0,fft->q,stream),"forward");
+  check(fftwpp_amd_forward_many(fft->dev,dtype,f1,1,1,1,0,T1,1,0,0,fft->q,stream),"forward");
+  check(fftwpp_amd_multiply(dtype,code,T0,T1,code == FFTWPP_AMD_MULT_REAL_BINARY ? 2*words : words,stream),"multiply");
+  check(fftwpp_amd_backward_many(fft->dev,dtype,T0,1,0,f0,1,1,1,0,0,fft->q,0,s,stream),"backward");
+}
+
+void Convolution::run(Complex **f, size_t offset, bool normalize)
+{
+  staged(work,f,offset,arraySize(),2,[&](Complex *a, Complex *b) {
+    execDevice(a,b,normalize,FFTWPP_AMD_F64,NULL);
+  });
+}
+
+// ------------------------------------------------------------------------
+// Hermitian symmetrisers (convolve.h:1168-1267) on device or host data
+// ------------------------------------------------------------------------
+template<class Exec>
+static void onDevice(Complex *f, size_t extent, Exec exec)
+{
+  if(fftwpp_amd_is_device_pointer(f)) {
+    exec(f);
+    check(fftwpp_amd_synchronize(NULL),"synchronize");
+    return;
+  }
+  void *d=NULL;
+  size_t bytes=extent*sizeof(Complex);
+  check(fftwpp_amd_malloc(&d,bytes),"symmetrize buffer");
+  int rc=fftwpp_amd_memcpy(d,f,bytes,NULL);
+  if(!rc) { exec((Complex *) d); rc=fftwpp_amd_memcpy(f,d,bytes,NULL); }
+  if(!rc) rc=fftwpp_amd_synchronize(NULL);
+  fftwpp_amd_free(d);
+  check(rc,"symmetrize");
+}
+
+void HermitianSymmetrize(Complex *f)
+{
+  onDevice(f,1,[&](Complex *d) {
+    check(fftwpp_amd_symmetrize_x(1,1,0,d,1,NULL),"HermitianSymmetrize");
+  });
+}
+
+void HermitianSymmetrizeX(size_t Hx, size_t Hy, size_t x0, Complex *f,
+                          size_t Sx, size_t)
+{
+  size_t extent=(x0+Hx-1)*Sx+Hy;
+  onDevice(f,extent,[&](Complex *d) {
+    check(fftwpp_amd_symmetrize_x(Hx,Hy,x0,d,Sx,NULL),"HermitianSymmetrizeX");
+  });
+}
+
+void HermitianSymmetrizeX(size_t Hx, size_t Hy, size_t x0, Complex *f)
+{
+  HermitianSymmetrizeX(Hx,Hy,x0,f,Hy);
+}
+
+void HermitianSymmetrizeXY(size_t Hx, size_t Hy, size_t Hz, size_t x0,
+                           size_t y0, Complex *f, size_t Sx, size_t Sy,
+                           size_t)
+{
+  size_t extent=(x0+Hx-1)*Sx+(y0+Hy-1)*Sy+Hz;
+  onDevice(f,extent,[&](Complex *d) {
+    check(fftwpp_amd_symmetrize_xy(Hx,Hy,Hz,x0,y0,d,Sx,Sy,NULL),"HermitianSymmetrizeXY");
+  });
+}
+
+void HermitianSymmetrizeXY(size_t Hx, size_t Hy, size_t Hz, size_t x0,
+                           size_t y0, Complex *f)
+{
+  size_t Ly=y0+Hy;
+  HermitianSymmetrizeXY(Hx,Hy,Hz,x0,y0,f,Ly*Hz,Hz);
+}
+
+// ------------------------------------------------------------------------
+// Convolution2 (convolve.h:1269-1548): x outer transform (multNone), inner y
+// rows fused.  All x residues are expanded at once (nloops == 1), so the
+// backward transform writes f[0] directly.
+// ------------------------------------------------------------------------
+Convolution2::Convolution2(fftBase *fftx, fftBase *ffty, Complex **,
+                           Complex *, Complex *) :
+  ThreadBase(fftx->Threads()), fftx(fftx), ffty(ffty), A(fftx->app.A),
+  B(fftx->app.B), mult(ffty->app.mult)
+{
+  Lx=fftx->L;
+  Ly=fftx->C;
+  Sx=fftx->S;
+  ly=ffty->inputLength();
+  scale=1.0/normalization();
+  multCode=multiplierCode(mult);
+  if(ly != Ly) throw Error(FFTWPP_AMD_ERR_ARG,"fftx->C must equal the stored length of the y transform");
+  if(ffty->C != 1) throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner transform with C > 1");
+}
+
+Convolution2::~Convolution2() {}
+
+size_t Convolution2::arraySize() const { return Sx*(Lx-1)+Ly; }
+
+void Convolution2::convolveDevice(Complex *f0, const Complex *f1,
+                                  bool normalize, void *stream)
+{
+  execDevice(f0,f1,normalize,FFTWPP_AMD_F64,stream);
+}
+
+void Convolution2::execDevice(void *f0, const void *f1, bool normalize,
+                              int dtype, void *stream, int mult)
+{
+  int code=mult >= 0 ? mult : multCode;
+  requireBinary(A,B,code);
+  if(ffty->m > fftwpp_amd_max_m(dtype))
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner sub-FFT size beyond the LDS limit");
+  double s=normalize ? scale : 1.0;
+  size_t E=eltsize(dtype);
+  size_t qx=fftx->q, mx=fftx->m;
+  size_t rows=qx*mx;
+  void *F0=work.get(0,rows*ly*E);
+  void *F1=work.get(1,rows*ly*E);
+  check(fftwpp_amd_forward_many(fftx->dev,dtype,f0,Sx,ly,1,0,F0,ly,0,0,qx,stream),"x forward");
+  check(fftwpp_amd_forward_many(fftx->dev,dtype,f1,Sx,ly,1,0,F1,ly,0,0,qx,stream),"x forward");
+  check(fftwpp_amd_conv_rows(ffty->dev,dtype,code,F0,F1,rows,ly,F0,ly,1.0,stream),"y rows");
+  check(fftwpp_amd_backward_many(fftx->dev,dtype,F0,ly,0,f0,Sx,ly,1,0,0,qx,0,s,stream),"x backward");
+}
+
+void Convolution2::run(Complex **f, size_t offset, bool normalize)
+{
+  staged(work,f,offset,arraySize(),2,[&](Complex *a, Complex *b) {
+    execDevice(a,b,normalize,FFTWPP_AMD_F64,NULL);
+  });
+}
+
+// ------------------------------------------------------------------------
+// Convolution3 (convolve.h:1550-1893): x and y outer transforms, z rows fused.
+// ------------------------------------------------------------------------
+Convolution3::Convolution3(fftBase *fftx, fftBase *ffty, fftBase *fftz,
+                           Complex **, Complex *, Complex *, bool) :
+  ThreadBase(fftx->Threads()), fftx(fftx), ffty(ffty), fftz(fftz),
+  A(fftx->app.A), B(fftx->app.B), mult(fftz->app.mult)
+{
+  Lx=fftx->L;
+  Ly=ffty->L;
+  Lz=ffty->C;
+  Sx=fftx->S;
+  Sy=ffty->S;
+  lz=fftz->inputLength();
+  scale=1.0/normalization();
+  multCode=multiplierCode(mult);
+  if(lz != Lz) throw Error(FFTWPP_AMD_ERR_ARG,"ffty->C must equal the stored length of the z transform");
+  if(Sx < Ly*Sy) throw Error(FFTWPP_AMD_ERR_ARG,"Sx cannot be less than Ly*Sy");
+  if(fftx->C != (contiguous() ? Ly*Sy : Lz)) throw Error(FFTWPP_AMD_ERR_ARG,"fftx->C is invalid");
+  if(fftz->C != 1) throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner transform with C > 1");
+}
+
+Convolution3::~Convolution3() {}
+
+size_t Convolution3::arraySize() const { return Sx*(Lx-1)+Sy*(Ly-1)+lz; }
+
+void Convolution3::convolveDevice(Complex *f0, const Complex *f1,
+                                  bool normalize, void *stream)
+{
+  execDevice(f0,f1,normalize,FFTWPP_AMD_F64,stream);
+}
+
+void Convolution3::execDevice(void *f0, const void *f1, bool normalize,
+                              int dtype, void *stream, int mult)
+{
+  int code=mult >= 0 ? mult : multCode;
+  requireBinary(A,B,code);
+  if(fftz->m > fftwpp_amd_max_m(dtype))
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner sub-FFT size beyond the LDS limit");
+  double s=normalize ? scale : 1.0;
+  size_t E=eltsize(dtype);
+  size_t qx=fftx->q, mx=fftx->m, qy=ffty->q, my=ffty->m;
+  size_t planes=qx*mx;           // x-transformed planes of Ly x lz
+  size_t plane=Ly*lz;
+  size_t yplane=qy*my*lz;        // y-transformed plane
+  void *X0=work.get(0,planes*plane*E);
+  void *X1=work.get(1,planes*plane*E);
+  void *Y0=work.get(2,planes*yplane*E);
+  void *Y1=work.get(3,planes*yplane*E);
+  // x forward: columns (y,z); non-contiguous y strides become batches
+  const void *in[2]={f0,f1};
+  void *X[2]={X0,X1};
+  void *Y[2]={Y0,Y1};
+  for(int a=0; a < 2; ++a) {
+    if(contiguous())
+      check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,plane,1,0,X[a],plane,0,0,qx,stream),"x forward");
+    else
+      check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,lz,Ly,Sy,X[a],plane,lz,0,qx,stream),"x forward");
+    check(fftwpp_amd_forward_many(ffty->dev,dtype,X[a],lz,lz,planes,plane,Y[a],lz,yplane,0,qy,stream),"y forward");
+  }
+  check(fftwpp_amd_conv_rows(fftz->dev,dtype,code,Y0,Y1,planes*qy*my,lz,Y0,lz,1.0,stream),"z rows");
+  check(fftwpp_amd_backward_many(ffty->dev,dtype,Y0,lz,yplane,X0,lz,lz,planes,plane,0,qy,0,1.0,stream),"y backward");
+  if(contiguous())
+    check(fftwpp_amd_backward_many(fftx->dev,dtype,X0,plane,0,f0,Sx,plane,1,0,0,qx,0,s,stream),"x backward");
+  else
+    check(fftwpp_amd_backward_many(fftx->dev,dtype,X0,plane,lz,f0,Sx,lz,Ly,Sy,0,qx,0,s,stream),"x backward");
+}
+
+void Convolution3::run(Complex **f, size_t offset, bool normalize)
+{
+  staged(work,f,offset,arraySize(),2,[&](Complex *a, Complex *b) {
+    execDevice(a,b,normalize,FFTWPP_AMD_F64,NULL);
+  });
+}
+
+} // namespace fftwpp

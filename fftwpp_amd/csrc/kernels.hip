@@ -1,0 +1,758 @@
+// kernels.hip -- generic HIP kernels of the hybrid implicitly dealiased
+// convolution and the thin C ABI (include/fftwpp_amd.h) through which the
+// host C++ layer reaches them.
+//
+// Kernel families (reference counterparts in convolve.cc):
+//   fwd_many    forward residue transforms of C strided columns
+//               (fftPad/fftPadCentered forward*Many, :836-1466, :2199-4307)
+//   bwd_many    backward residue transforms with residue accumulation
+//               (backward*Many, :1468-1965)
+//   conv_rows   fused forward -> multiply -> backward -> accumulate over all
+//               residues of contiguous rows (Convolution::convolveRaw,
+//               :7513-7575, with multBinary/multcorrelation, :33-110)
+//   hconv_rows  the same for Hermitian rows (fftPadHermitian forward2/
+//               backward2 :4517-5012 with realMultBinary :60-83); two real
+//               residues share one complex sub-FFT
+//   multiply    the multipliers applied to device arrays
+//   symmetrize  HermitianSymmetrizeX/XY (convolve.h:1168-1267)
+//   twiddles    FP64 generation of zeta_N^j (Complex.h:284-296 expi)
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "common.h"
+#include "fft_lds.h"
+#include "../../include/fftwpp_amd.h"
+
+using namespace fftwpp_amd;
+
+namespace {
+
+// ------------------------------------------------------------------------
+// Twiddle generation: zN[j] = e^{2 pi i j/N} in FP64, rounded to FP32 for
+// the single-precision copy.
+// ------------------------------------------------------------------------
+__global__ void twiddle_kernel(int64_t N, cplx<double> *z64, cplx<float> *z32)
+{
+  int64_t j=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  if(j >= N) return;
+  double s, c;
+  // reduce to [0,1/2] octant-free: sincospi(2j/N) with exact integer ratio
+  sincospi(2.0*(double) j/(double) N,&s,&c);
+  z64[j]=mk(c,s);
+  z32[j]=mk((float) c,(float) s);
+}
+
+template<typename T>
+__device__ __forceinline__ int64_t floormod(int64_t a, int64_t m)
+{
+  int64_t r=a % m;
+  return r < 0 ? r+m : r;
+}
+
+// W_sigma = sum_{k = sigma mod m, k in data range} zeta_N^{k r} f(k)
+// for a complex or centred dimension (k = j + k0, j < L).
+template<typename T>
+__device__ __forceinline__ cplx<T> fold(const DimArgs &d, const cplx<T> *f,
+                                        int64_t stride, int64_t sigma,
+                                        int64_t r)
+{
+  cplx<T> acc=mk((T) 0,(T) 0);
+  int64_t j=floormod<T>(sigma-d.k0,d.m);
+  for(; j < d.L; j += d.m) {
+    int64_t k=j+d.k0;
+    cplx<T> v=f[j*stride];
+    acc=acc+zeta<T>(d,k*r)*v;
+  }
+  return acc;
+}
+
+// Hermitian dimension: f holds modes k in [0,H); f(-k)=conj f(k).
+// Returns sum_{k = sigma mod m, |k| < H} (zeta^{k r1} + i zeta^{k r2}) f(k)
+// (the second term only when r2 >= 0).
+template<typename T>
+__device__ __forceinline__ cplx<T> hfold(const DimArgs &d, const cplx<T> *f,
+                                         int64_t stride, int64_t sigma,
+                                         int64_t r1, int64_t r2)
+{
+  cplx<T> acc=mk((T) 0,(T) 0);
+  int64_t H=d.H;
+  // smallest k > -H with k = sigma (mod m)
+  int64_t k=sigma-d.m*((sigma+H-1)/d.m);
+  for(; k < H; k += d.m) {
+    cplx<T> v=k >= 0 ? f[k*stride] : conjg(f[-k*stride]);
+    cplx<T> w=zeta<T>(d,k*r1);
+    if(r2 >= 0) w=w+muli(zeta<T>(d,k*r2),1);
+    acc=acc+w*v;
+  }
+  return acc;
+}
+
+// ------------------------------------------------------------------------
+// fwd_many: F[b][(r-r0)*m+s][c] = sum_k f[b][k][c] zeta_N^{k(qs+r)}
+// ------------------------------------------------------------------------
+template<typename T>
+__global__ void fwd_many_kernel(DimArgs d, const cplx<T> *f, int64_t S,
+                                int64_t C, int64_t bsi, cplx<T> *F,
+                                int64_t SF, int64_t bso, int64_t r0,
+                                int64_t nres, int TC)
+{
+  extern __shared__ char smem_raw[];
+  cplx<T> *A=reinterpret_cast<cplx<T> *>(smem_raw);
+  const int m=(int) d.m;
+  const int ld=m+1;
+  cplx<T> *B=A+(int64_t) TC*ld;
+  const int64_t c0=(int64_t) blockIdx.x*TC;
+  const int64_t b=blockIdx.y;
+  const int nc=(int) min((int64_t) TC,C-c0);
+  const cplx<T> *fb=f+b*bsi+c0;
+  cplx<T> *Fb=F+b*bso+c0;
+  const int tid=threadIdx.x, nt=blockDim.x;
+  for(int64_t rr=0; rr < nres; ++rr) {
+    int64_t r=r0+rr;
+    for(int idx=tid; idx < nc*m; idx += nt) {
+      int sigma=idx/nc, cc=idx-sigma*nc;
+      A[cc*ld+sigma]=fold<T>(d,fb+cc,S,sigma,r);
+    }
+    __syncthreads();
+    cplx<T> *X=lds_fft<T>(A,B,nc,ld,d,1);
+    cplx<T> *Fr=Fb+rr*m*SF;
+    for(int idx=tid; idx < nc*m; idx += nt) {
+      int s=idx/nc, cc=idx-s*nc;
+      Fr[(int64_t) s*SF+cc]=X[cc*ld+s];
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------
+// bwd_many: f[b][j][c] (+)= sum_{r,s} F[b][(r-r0)*m+s][c] zeta_N^{-k(qs+r)},
+// k = j + k0; the total is multiplied by scale after the last residue.
+// ------------------------------------------------------------------------
+template<typename T>
+__global__ void bwd_many_kernel(DimArgs d, const cplx<T> *F, int64_t SF,
+                                int64_t bsi, cplx<T> *f, int64_t S,
+                                int64_t C, int64_t bso, int64_t r0,
+                                int64_t nres, int accumulate, T scale, int TC)
+{
+  extern __shared__ char smem_raw[];
+  cplx<T> *A=reinterpret_cast<cplx<T> *>(smem_raw);
+  const int m=(int) d.m;
+  const int ld=m+1;
+  cplx<T> *B=A+(int64_t) TC*ld;
+  const int64_t c0=(int64_t) blockIdx.x*TC;
+  const int64_t b=blockIdx.y;
+  const int nc=(int) min((int64_t) TC,C-c0);
+  const cplx<T> *Fb=F+b*bsi+c0;
+  cplx<T> *fb=f+b*bso+c0;
+  const int tid=threadIdx.x, nt=blockDim.x;
+  const int L=(int) d.L;
+  for(int64_t rr=0; rr < nres; ++rr) {
+    int64_t r=r0+rr;
+    const cplx<T> *Fr=Fb+rr*m*SF;
+    for(int idx=tid; idx < nc*m; idx += nt) {
+      int s=idx/nc, cc=idx-s*nc;
+      A[cc*ld+s]=Fr[(int64_t) s*SF+cc];
+    }
+    __syncthreads();
+    cplx<T> *X=lds_fft<T>(A,B,nc,ld,d,-1);
+    bool last=rr == nres-1;
+    bool add=accumulate || rr > 0;
+    for(int idx=tid; idx < nc*L; idx += nt) {
+      int j=idx/nc, cc=idx-j*nc;
+      int64_t k=j+d.k0;
+      int sigma=(int) floormod<T>(k,d.m);
+      cplx<T> v=mulc(X[cc*ld+sigma],zeta<T>(d,k*r));
+      cplx<T> *o=fb+(int64_t) j*S+cc;
+      if(add) v=v+*o;
+      if(last) v=scale*v;
+      *o=v;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------
+// conv_rows: fused complex/centred row convolution.  Rows are contiguous
+// (element stride 1); row i of f, g starts at i*rs, of out at i*os.
+// MULT: 1 = multBinary (F0 *= F1), 3 = multcorrelation (F0 *= conj F1).
+// ------------------------------------------------------------------------
+template<typename T, int MULT>
+__global__ void conv_rows_kernel(DimArgs d, const cplx<T> *f,
+                                 const cplx<T> *g, int64_t rows, int64_t rs,
+                                 cplx<T> *out, int64_t os, T scale, int TR,
+                                 int ld)
+{
+  extern __shared__ char smem_raw[];
+  const int m=(int) d.m;
+  const int L=(int) d.L;
+  cplx<T> *A=reinterpret_cast<cplx<T> *>(smem_raw);   // 2*TR sequences
+  cplx<T> *B=A+(int64_t) 2*TR*ld;                     // 2*TR sequences
+  cplx<T> *acc=B+(int64_t) 2*TR*ld;                   // TR rows of L
+  const int64_t row0=(int64_t) blockIdx.x*TR;
+  const int nr=(int) min((int64_t) TR,rows-row0);
+  const int tid=threadIdx.x, nt=blockDim.x;
+  for(int64_t r=0; r < d.q; ++r) {
+    // build the folded, twiddled sequences of f (rows 0..nr-1) and g
+    for(int idx=tid; idx < 2*nr*m; idx += nt) {
+      int seq=idx/m, sigma=idx-seq*m;
+      int a=seq/nr, i=seq-a*nr;
+      const cplx<T> *src=(a == 0 ? f : g)+(row0+i)*rs;
+      A[(a*TR+i)*ld+sigma]=fold<T>(d,src,1,sigma,r);
+    }
+    __syncthreads();
+    // both inputs transformed together: sequences [0,TR) f, [TR,2TR) g
+    cplx<T> *X=lds_fft<T>(A,B,2*TR,ld,d,1);
+    cplx<T> *Y=X == A ? B : A;
+    for(int idx=tid; idx < nr*m; idx += nt) {
+      int i=idx/m, s=idx-i*m;
+      cplx<T> u=X[i*ld+s], v=X[(TR+i)*ld+s];
+      if(MULT != 0) X[i*ld+s]=MULT == 3 ? mulc(u,v) : u*v;
+    }
+    __syncthreads();
+    cplx<T> *Z=lds_fft<T>(X,Y,TR,ld,d,-1);
+    for(int idx=tid; idx < nr*L; idx += nt) {
+      int i=idx/L, j=idx-i*L;
+      int64_t k=j+d.k0;
+      int sigma=(int) floormod<T>(k,d.m);
+      cplx<T> v=mulc(Z[i*ld+sigma],zeta<T>(d,k*r));
+      acc[i*L+j]=r == 0 ? v : acc[i*L+j]+v;
+    }
+    __syncthreads();
+  }
+  for(int idx=tid; idx < nr*L; idx += nt) {
+    int i=idx/L, j=idx-i*L;
+    out[(row0+i)*os+j]=scale*acc[i*L+j];
+  }
+}
+
+// ------------------------------------------------------------------------
+// hconv_rows: Hermitian rows (H stored modes).  Residues r1, r2=r1+1 are
+// packed as W1 + i W2; both transforms are real, so one complex sub-FFT
+// yields u_{r1} + i u_{r2}.  realMultBinary multiplies real parts and
+// imaginary parts separately; the inverse is unpacked by Hermitian symmetry.
+// ------------------------------------------------------------------------
+template<typename T>
+__global__ void hconv_rows_kernel(DimArgs d, const cplx<T> *f,
+                                  const cplx<T> *g, int64_t rows, int64_t rs,
+                                  cplx<T> *out, int64_t os, T scale, int TR,
+                                  int ld)
+{
+  extern __shared__ char smem_raw[];
+  const int m=(int) d.m;
+  const int H=(int) d.H;
+  cplx<T> *A=reinterpret_cast<cplx<T> *>(smem_raw);
+  cplx<T> *B=A+(int64_t) 2*TR*ld;
+  cplx<T> *acc=B+(int64_t) 2*TR*ld;
+  const int64_t row0=(int64_t) blockIdx.x*TR;
+  const int nr=(int) min((int64_t) TR,rows-row0);
+  const int tid=threadIdx.x, nt=blockDim.x;
+  for(int64_t r1=0; r1 < d.q; r1 += 2) {
+    int64_t r2=r1+1 < d.q ? r1+1 : -1;
+    for(int idx=tid; idx < 2*nr*m; idx += nt) {
+      int seq=idx/m, sigma=idx-seq*m;
+      int a=seq/nr, i=seq-a*nr;
+      const cplx<T> *src=(a == 0 ? f : g)+(row0+i)*rs;
+      A[(a*TR+i)*ld+sigma]=hfold<T>(d,src,1,sigma,r1,r2);
+    }
+    __syncthreads();
+    cplx<T> *X=lds_fft<T>(A,B,2*TR,ld,d,1);
+    cplx<T> *Y=X == A ? B : A;
+    for(int idx=tid; idx < nr*m; idx += nt) {
+      int i=idx/m, s=idx-i*m;
+      cplx<T> u=X[i*ld+s], v=X[(TR+i)*ld+s];
+      X[i*ld+s]=mk(u.x*v.x,u.y*v.y);
+    }
+    __syncthreads();
+    cplx<T> *Z=lds_fft<T>(X,Y,TR,ld,d,-1);
+    for(int idx=tid; idx < nr*H; idx += nt) {
+      int i=idx/H, j=idx-i*H;
+      int sigma=j % m;
+      int msig=sigma == 0 ? 0 : m-sigma;
+      cplx<T> z=Z[i*ld+sigma], zc=conjg(Z[i*ld+msig]);
+      cplx<T> y1=(T) 0.5*(z+zc);
+      cplx<T> v=mulc(y1,zeta<T>(d,(int64_t) j*r1));
+      if(r2 >= 0) {
+        cplx<T> y2=muli((T) 0.5*(z-zc),-1);
+        v=v+mulc(y2,zeta<T>(d,(int64_t) j*r2));
+      }
+      acc[i*H+j]=r1 == 0 ? v : acc[i*H+j]+v;
+    }
+    __syncthreads();
+  }
+  for(int idx=tid; idx < nr*H; idx += nt) {
+    int i=idx/H, j=idx-i*H;
+    out[(row0+i)*os+j]=scale*acc[i*H+j];
+  }
+}
+
+// ------------------------------------------------------------------------
+// Elementwise multipliers on device arrays (convolve.cc:26-110).
+// ------------------------------------------------------------------------
+template<typename T>
+__global__ void multiply_kernel(int mult, cplx<T> *F0, const cplx<T> *F1,
+                                int64_t n)
+{
+  int64_t i=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  if(i >= n) return;
+  if(mult == FFTWPP_AMD_MULT_BINARY) F0[i]=F0[i]*F1[i];
+  else if(mult == FFTWPP_AMD_MULT_CORRELATION) F0[i]=mulc(F0[i],F1[i]);
+  else if(mult == FFTWPP_AMD_MULT_REAL_BINARY) {
+    // realMultBinary: n counts real words (doubles or floats)
+    T *a=reinterpret_cast<T *>(F0);
+    const T *b=reinterpret_cast<const T *>(F1);
+    a[i]=a[i]*b[i];
+  }
+}
+
+template<typename T>
+__global__ void scale_kernel(cplx<T> *f, int64_t n, T s)
+{
+  int64_t i=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  if(i < n) f[i]=s*f[i];
+}
+
+// HermitianSymmetrizeX: F=f+x0*Sx; F[-i*Sx]=conj F[i*Sx] (0<i<Hx);
+// Im F[0]=0; zero row 0 when x0 == Hx.
+__global__ void symx_kernel(int64_t Hx, int64_t Hy, int64_t x0,
+                            cplx<double> *f, int64_t Sx)
+{
+  int64_t t=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  cplx<double> *F=f+x0*Sx;
+  if(t == 0) F[0].y=0.0;
+  if(t >= 1 && t < Hx) F[-t*Sx]=conjg(F[t*Sx]);
+  if(x0 == Hx && t < Hy) {
+    // row 0 is the Nyquist row; it is never a source of the mirror above
+    // because the mirror reads rows x0+1..x0+Hx-1 only.
+    f[t]=mk(0.0,0.0);
+  }
+}
+
+// HermitianSymmetrizeXY, stage 1: the x line through the origin.
+__global__ void symxy1_kernel(int64_t Hx, cplx<double> *f, int64_t origin,
+                              int64_t Sx)
+{
+  int64_t t=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  cplx<double> *F=f+origin;
+  if(t == 0) F[0].y=0.0;
+  if(t >= 1 && t < Hx) F[-t*Sx]=conjg(F[t*Sx]);
+}
+
+// stage 2: f[origin-i*Sx-j*Sy] = conj f[origin+i*Sx+j*Sy], -Hx<i<Hx, 0<j<Hy.
+__global__ void symxy2_kernel(int64_t Hx, int64_t Hy, cplx<double> *f,
+                              int64_t origin, int64_t Sx, int64_t Sy)
+{
+  int64_t t=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  int64_t ni=2*Hx-1, nj=Hy-1;
+  if(t >= ni*nj) return;
+  int64_t i=t/nj-(Hx-1);
+  int64_t j=t % nj+1;
+  f[origin-i*Sx-j*Sy]=conjg(f[origin+i*Sx+j*Sy]);
+}
+
+// stage 3: Nyquist zeroing.
+__global__ void symxy3_kernel(int64_t Hx, int64_t Hy, int64_t Hz, int64_t x0,
+                              int64_t y0, cplx<double> *f, int64_t Sx,
+                              int64_t Sy)
+{
+  int64_t t=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  if(x0 == Hx) {
+    int64_t Ly=y0+Hy;
+    if(t < Ly*Hz) f[Sy*(t/Hz)+t % Hz]=mk(0.0,0.0);
+  }
+  if(y0 == Hy) {
+    int64_t Lx=x0+Hx;
+    if(t < Lx*Hz) f[Sx*(t/Hz)+t % Hz]=mk(0.0,0.0);
+  }
+}
+
+// ------------------------------------------------------------------------
+// Host-side helpers of the C ABI
+// ------------------------------------------------------------------------
+thread_local char last_error[512];
+
+int fail(int code, const char *msg)
+{
+  strncpy(last_error,msg,sizeof(last_error)-1);
+  last_error[sizeof(last_error)-1]=0;
+  return code;
+}
+
+int hipcheck(hipError_t e, const char *what)
+{
+  if(e == hipSuccess) return FFTWPP_AMD_OK;
+  char buf[512];
+  snprintf(buf,sizeof(buf),"%s: %s",what,hipGetErrorString(e));
+  return fail(FFTWPP_AMD_ERR_HIP,buf);
+}
+
+} // namespace
+
+struct fftwpp_amd_dim {
+  DimArgs a64, a32;
+  void *z64, *z32;
+  size_t L, m, q, N;
+  int kind;
+};
+
+static int launch_check(const char *what)
+{
+  return hipcheck(hipGetLastError(),what);
+}
+
+static size_t lds_budget() { return 64*1024; }
+
+// Kernels that take more than the default 64 KB of dynamic LDS must raise
+// their limit first (gfx950 has 160 KB per workgroup).
+template<typename K>
+static int allow_lds(K kernel, size_t bytes)
+{
+  if(bytes <= 64*1024) return FFTWPP_AMD_OK;
+  return hipcheck(hipFuncSetAttribute((const void *) kernel,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int) bytes),"hipFuncSetAttribute");
+}
+static size_t lds_max() { return 160*1024; }
+
+template<typename T>
+static int fwd_many_t(const fftwpp_amd_dim *d, const void *f, size_t S,
+                      size_t C, size_t nbatch, size_t bsi, void *F,
+                      size_t SF, size_t bso, size_t r0, size_t nres,
+                      hipStream_t st)
+{
+  const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
+  size_t elt=sizeof(cplx<T>);
+  size_t per=2*(d->m+1)*elt;
+  if(per > lds_max()) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"m too large for the LDS transform");
+  size_t TC=1;
+  while(TC < 16 && TC < C && 2*TC*per <= lds_budget()) TC *= 2;
+  size_t lds=TC*per;
+  dim3 grid((unsigned) ((C+TC-1)/TC),(unsigned) nbatch);
+  int rc=allow_lds(fwd_many_kernel<T>,lds);
+  if(rc) return rc;
+  hipLaunchKernelGGL(fwd_many_kernel<T>,grid,dim3(256),lds,st,a,
+                     (const cplx<T> *) f,(int64_t) S,(int64_t) C,
+                     (int64_t) bsi,(cplx<T> *) F,(int64_t) SF,(int64_t) bso,
+                     (int64_t) r0,(int64_t) nres,(int) TC);
+  return launch_check("fwd_many_kernel");
+}
+
+template<typename T>
+static int bwd_many_t(const fftwpp_amd_dim *d, const void *F, size_t SF,
+                      size_t bsi, void *f, size_t S, size_t C,
+                      size_t nbatch, size_t bso, size_t r0, size_t nres,
+                      int accumulate, double scale, hipStream_t st)
+{
+  const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
+  size_t elt=sizeof(cplx<T>);
+  size_t per=2*(d->m+1)*elt;
+  if(per > lds_max()) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"m too large for the LDS transform");
+  size_t TC=1;
+  while(TC < 16 && TC < C && 2*TC*per <= lds_budget()) TC *= 2;
+  size_t lds=TC*per;
+  dim3 grid((unsigned) ((C+TC-1)/TC),(unsigned) nbatch);
+  int rc=allow_lds(bwd_many_kernel<T>,lds);
+  if(rc) return rc;
+  hipLaunchKernelGGL(bwd_many_kernel<T>,grid,dim3(256),lds,st,a,
+                     (const cplx<T> *) F,(int64_t) SF,(int64_t) bsi,
+                     (cplx<T> *) f,(int64_t) S,(int64_t) C,(int64_t) bso,
+                     (int64_t) r0,(int64_t) nres,accumulate,(T) scale,(int) TC);
+  return launch_check("bwd_many_kernel");
+}
+
+template<typename T>
+static int conv_rows_t(const fftwpp_amd_dim *d, int mult, const void *f,
+                       const void *g, size_t rows, size_t rs, void *out,
+                       size_t os, double scale, hipStream_t st)
+{
+  const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
+  size_t elt=sizeof(cplx<T>);
+  bool herm=d->kind == FFTWPP_AMD_KIND_HERMITIAN;
+  size_t lout=herm ? (d->L+1)/2 : d->L;
+  // pad the leading dimension against bank conflicts when it still fits
+  size_t ld=d->m+1;
+  if((4*ld+lout)*elt > lds_max()) ld=d->m;
+  size_t per=(4*ld+lout)*elt;
+  if(per > lds_max()) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"m too large for the fused row kernel");
+  size_t TR=1;
+  while(TR < 16 && TR < rows && 2*TR*per <= lds_budget()) TR *= 2;
+  size_t lds=TR*per;
+  dim3 grid((unsigned) ((rows+TR-1)/TR));
+  int rc=FFTWPP_AMD_OK;
+  if(herm) rc=allow_lds(hconv_rows_kernel<T>,lds);
+  else if(mult == FFTWPP_AMD_MULT_CORRELATION) rc=allow_lds(conv_rows_kernel<T,3>,lds);
+  else if(mult == FFTWPP_AMD_MULT_NONE) rc=allow_lds(conv_rows_kernel<T,0>,lds);
+  else rc=allow_lds(conv_rows_kernel<T,1>,lds);
+  if(rc) return rc;
+  if(herm)
+    hipLaunchKernelGGL(hconv_rows_kernel<T>,grid,dim3(256),lds,st,a,
+                       (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,
+                       (int64_t) rs,(cplx<T> *) out,(int64_t) os,(T) scale,(int) TR,(int) ld);
+  else if(mult == FFTWPP_AMD_MULT_CORRELATION)
+    hipLaunchKernelGGL((conv_rows_kernel<T,3>),grid,dim3(256),lds,st,a,
+                       (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,
+                       (int64_t) rs,(cplx<T> *) out,(int64_t) os,(T) scale,(int) TR,(int) ld);
+  else if(mult == FFTWPP_AMD_MULT_NONE)
+    hipLaunchKernelGGL((conv_rows_kernel<T,0>),grid,dim3(256),lds,st,a,
+                       (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,
+                       (int64_t) rs,(cplx<T> *) out,(int64_t) os,(T) scale,(int) TR,(int) ld);
+  else
+    hipLaunchKernelGGL((conv_rows_kernel<T,1>),grid,dim3(256),lds,st,a,
+                       (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,
+                       (int64_t) rs,(cplx<T> *) out,(int64_t) os,(T) scale,(int) TR,(int) ld);
+  return launch_check("conv_rows_kernel");
+}
+
+extern "C" {
+
+const char *fftwpp_amd_last_error(void) { return last_error; }
+
+// Used by the host C++ layer to record its own failures in the same slot.
+void fftwpp_amd_internal_set_error(const char *msg) { fail(0,msg); }
+
+int fftwpp_amd_dim_create(fftwpp_amd_dim **out, size_t L, size_t m,
+                          size_t q, int kind)
+{
+  if(!out) return fail(FFTWPP_AMD_ERR_ARG,"null output handle");
+  *out=NULL;
+  if(L == 0 || m == 0 || q == 0)
+    return fail(FFTWPP_AMD_ERR_ARG,"L, m and q must be positive");
+  if(kind < FFTWPP_AMD_KIND_COMPLEX || kind > FFTWPP_AMD_KIND_HERMITIAN)
+    return fail(FFTWPP_AMD_ERR_ARG,"unknown transform kind");
+  size_t N=m*q;
+  if(m > (1u << 20) || N > ((size_t) 1 << 31))
+    return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"padded length too large");
+  fftwpp_amd_dim *d=new fftwpp_amd_dim;
+  memset(d,0,sizeof(*d));
+  d->L=L; d->m=m; d->q=q; d->N=N; d->kind=kind;
+  DimArgs a;
+  memset(&a,0,sizeof(a));
+  a.L=(int64_t) L; a.m=(int64_t) m; a.q=(int64_t) q; a.N=(int64_t) N;
+  a.k0=kind == FFTWPP_AMD_KIND_CENTERED ? -(int64_t) (L/2) : 0;
+  a.H=kind == FFTWPP_AMD_KIND_HERMITIAN ? (int64_t) ((L+1)/2) : (int64_t) L;
+  // radix plan: 8s, then 4, 2, 3, 5, then remaining primes
+  size_t r=m;
+  int ns=0;
+  while(r % 8 == 0 && ns < MAXSTAGES) { a.radix[ns++]=8; r /= 8; }
+  while(r % 4 == 0 && ns < MAXSTAGES) { a.radix[ns++]=4; r /= 4; }
+  while(r % 2 == 0 && ns < MAXSTAGES) { a.radix[ns++]=2; r /= 2; }
+  for(size_t p=3; r > 1 && ns < MAXSTAGES; p += 2)
+    while(r % p == 0 && ns < MAXSTAGES) { a.radix[ns++]=(int) p; r /= p; }
+  if(r != 1) { delete d; return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"too many FFT stages"); }
+  a.nstages=ns;
+  int rc=hipcheck(hipMalloc(&d->z64,sizeof(cplx<double>)*N),"hipMalloc twiddles");
+  if(rc) { delete d; return rc; }
+  rc=hipcheck(hipMalloc(&d->z32,sizeof(cplx<float>)*N),"hipMalloc twiddles");
+  if(rc) { (void) hipFree(d->z64); delete d; return rc; }
+  unsigned nb=(unsigned) ((N+255)/256);
+  twiddle_kernel<<<nb,256>>>((int64_t) N,(cplx<double> *) d->z64,
+                             (cplx<float> *) d->z32);
+  rc=launch_check("twiddle_kernel");
+  if(!rc) rc=hipcheck(hipDeviceSynchronize(),"twiddle sync");
+  if(rc) { (void) hipFree(d->z64); (void) hipFree(d->z32); delete d; return rc; }
+  d->a64=a; d->a64.zN=d->z64;
+  d->a32=a; d->a32.zN=d->z32;
+  *out=d;
+  return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_dim_destroy(fftwpp_amd_dim *d)
+{
+  if(!d) return FFTWPP_AMD_OK;
+  (void) hipFree(d->z64);
+  (void) hipFree(d->z32);
+  delete d;
+  return FFTWPP_AMD_OK;
+}
+
+size_t fftwpp_amd_max_m(int dtype)
+{
+  // one sequence in the fused row kernel needs 4 buffers of m plus L
+  return dtype == FFTWPP_AMD_F32 ? 4096 : 2048;
+}
+
+
+int fftwpp_amd_forward_many(const fftwpp_amd_dim *d, int dtype,
+                            const void *f, size_t S, size_t C,
+                            size_t nbatch, size_t bstride_in, void *F,
+                            size_t SF, size_t bstride_out, size_t r0,
+                            size_t nres, void *stream)
+{
+  if(!d || !f || !F) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  if(d->kind == FFTWPP_AMD_KIND_HERMITIAN)
+    return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"Hermitian dimensions are innermost only");
+  if(r0+nres > d->q || C == 0 || nbatch == 0) return fail(FFTWPP_AMD_ERR_ARG,"bad residue range or shape");
+  if(S < C || SF < C) return fail(FFTWPP_AMD_ERR_ARG,"stride smaller than count");
+  hipStream_t st=(hipStream_t) stream;
+  return dtype == FFTWPP_AMD_F32 ?
+    fwd_many_t<float>(d,f,S,C,nbatch,bstride_in,F,SF,bstride_out,r0,nres,st) :
+    fwd_many_t<double>(d,f,S,C,nbatch,bstride_in,F,SF,bstride_out,r0,nres,st);
+}
+
+
+int fftwpp_amd_backward_many(const fftwpp_amd_dim *d, int dtype,
+                             const void *F, size_t SF, size_t bstride_in,
+                             void *f, size_t S, size_t C, size_t nbatch,
+                             size_t bstride_out, size_t r0, size_t nres,
+                             int accumulate, double scale, void *stream)
+{
+  if(!d || !f || !F) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  if(d->kind == FFTWPP_AMD_KIND_HERMITIAN)
+    return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"Hermitian dimensions are innermost only");
+  if(r0+nres > d->q || nres == 0 || C == 0 || nbatch == 0) return fail(FFTWPP_AMD_ERR_ARG,"bad residue range or shape");
+  if(S < C || SF < C) return fail(FFTWPP_AMD_ERR_ARG,"stride smaller than count");
+  hipStream_t st=(hipStream_t) stream;
+  return dtype == FFTWPP_AMD_F32 ?
+    bwd_many_t<float>(d,F,SF,bstride_in,f,S,C,nbatch,bstride_out,r0,nres,accumulate,scale,st) :
+    bwd_many_t<double>(d,F,SF,bstride_in,f,S,C,nbatch,bstride_out,r0,nres,accumulate,scale,st);
+}
+
+
+int fftwpp_amd_conv_rows(const fftwpp_amd_dim *d, int dtype, int mult,
+                         const void *f, const void *g, size_t rows,
+                         size_t rowstride, void *out, size_t outstride,
+                         double scale, void *stream)
+{
+  if(!d || !f || !g || !out) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  if(rows == 0) return FFTWPP_AMD_OK;
+  bool herm=d->kind == FFTWPP_AMD_KIND_HERMITIAN;
+  if(herm && mult != FFTWPP_AMD_MULT_REAL_BINARY)
+    return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"Hermitian rows take realMultBinary");
+  if(!herm && mult != FFTWPP_AMD_MULT_BINARY && mult != FFTWPP_AMD_MULT_CORRELATION &&
+     mult != FFTWPP_AMD_MULT_NONE)
+    return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"complex rows take multNone, multBinary or multcorrelation");
+  hipStream_t st=(hipStream_t) stream;
+  return dtype == FFTWPP_AMD_F32 ?
+    conv_rows_t<float>(d,mult,f,g,rows,rowstride,out,outstride,scale,st) :
+    conv_rows_t<double>(d,mult,f,g,rows,rowstride,out,outstride,scale,st);
+}
+
+int fftwpp_amd_multiply(int dtype, int mult, void *F0, const void *F1,
+                        size_t n, void *stream)
+{
+  if(mult == FFTWPP_AMD_MULT_NONE || n == 0) return FFTWPP_AMD_OK;
+  if(!F0 || !F1) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  unsigned nb=(unsigned) ((n+255)/256);
+  hipStream_t st=(hipStream_t) stream;
+  if(dtype == FFTWPP_AMD_F32)
+    hipLaunchKernelGGL(multiply_kernel<float>,dim3(nb),dim3(256),0,st,mult,
+                       (cplx<float> *) F0,(const cplx<float> *) F1,(int64_t) n);
+  else
+    hipLaunchKernelGGL(multiply_kernel<double>,dim3(nb),dim3(256),0,st,mult,
+                       (cplx<double> *) F0,(const cplx<double> *) F1,(int64_t) n);
+  return launch_check("multiply_kernel");
+}
+
+int fftwpp_amd_scale(int dtype, void *f, size_t n, double s, void *stream)
+{
+  if(n == 0) return FFTWPP_AMD_OK;
+  unsigned nb=(unsigned) ((n+255)/256);
+  hipStream_t st=(hipStream_t) stream;
+  if(dtype == FFTWPP_AMD_F32)
+    hipLaunchKernelGGL(scale_kernel<float>,dim3(nb),dim3(256),0,st,
+                       (cplx<float> *) f,(int64_t) n,(float) s);
+  else
+    hipLaunchKernelGGL(scale_kernel<double>,dim3(nb),dim3(256),0,st,
+                       (cplx<double> *) f,(int64_t) n,s);
+  return launch_check("scale_kernel");
+}
+
+int fftwpp_amd_symmetrize_x(size_t Hx, size_t Hy, size_t x0, void *f,
+                            size_t Sx, void *stream)
+{
+  if(!f || Hx == 0) return fail(FFTWPP_AMD_ERR_ARG,"bad argument");
+  if(x0+1 < Hx) return fail(FFTWPP_AMD_ERR_ARG,"x0 must be at least Hx-1");
+  size_t n=Hx > Hy ? Hx : Hy;
+  unsigned nb=(unsigned) ((n+255)/256);
+  hipLaunchKernelGGL(symx_kernel,dim3(nb),dim3(256),0,(hipStream_t) stream,
+                     (int64_t) Hx,(int64_t) Hy,(int64_t) x0,(cplx<double> *) f,
+                     (int64_t) Sx);
+  return launch_check("symx_kernel");
+}
+
+int fftwpp_amd_symmetrize_xy(size_t Hx, size_t Hy, size_t Hz, size_t x0,
+                             size_t y0, void *f, size_t Sx, size_t Sy,
+                             void *stream)
+{
+  if(!f || Hx == 0 || Hy == 0) return fail(FFTWPP_AMD_ERR_ARG,"bad argument");
+  if(x0+1 < Hx || y0+1 < Hy) return fail(FFTWPP_AMD_ERR_ARG,"origin too small");
+  hipStream_t st=(hipStream_t) stream;
+  int64_t origin=(int64_t) (x0*Sx+y0*Sy);
+  hipLaunchKernelGGL(symxy1_kernel,dim3((unsigned) ((Hx+255)/256)),dim3(256),0,st,
+                     (int64_t) Hx,(cplx<double> *) f,origin,(int64_t) Sx);
+  int rc=launch_check("symxy1_kernel");
+  if(rc) return rc;
+  size_t n2=(2*Hx-1)*(Hy-1);
+  if(n2 > 0) {
+    hipLaunchKernelGGL(symxy2_kernel,dim3((unsigned) ((n2+255)/256)),dim3(256),0,st,
+                       (int64_t) Hx,(int64_t) Hy,(cplx<double> *) f,origin,
+                       (int64_t) Sx,(int64_t) Sy);
+    rc=launch_check("symxy2_kernel");
+    if(rc) return rc;
+  }
+  size_t n3=((y0+Hy) > (x0+Hx) ? (y0+Hy) : (x0+Hx))*Hz;
+  if(x0 == Hx || y0 == Hy) {
+    hipLaunchKernelGGL(symxy3_kernel,dim3((unsigned) ((n3+255)/256)),dim3(256),0,st,
+                       (int64_t) Hx,(int64_t) Hy,(int64_t) Hz,(int64_t) x0,
+                       (int64_t) y0,(cplx<double> *) f,(int64_t) Sx,(int64_t) Sy);
+    rc=launch_check("symxy3_kernel");
+  }
+  return rc;
+}
+
+// ---- memory and stream plumbing for host code that includes no HIP header
+int fftwpp_amd_malloc(void **p, size_t bytes)
+{
+  if(!p) return fail(FFTWPP_AMD_ERR_ARG,"null pointer");
+  *p=NULL;
+  if(bytes == 0) return FFTWPP_AMD_OK;
+  return hipcheck(hipMalloc(p,bytes),"hipMalloc");
+}
+
+int fftwpp_amd_free(void *p)
+{
+  if(!p) return FFTWPP_AMD_OK;
+  return hipcheck(hipFree(p),"hipFree");
+}
+
+int fftwpp_amd_memcpy(void *dst, const void *src, size_t bytes, void *stream)
+{
+  if(bytes == 0) return FFTWPP_AMD_OK;
+  return hipcheck(hipMemcpyAsync(dst,src,bytes,hipMemcpyDefault,(hipStream_t) stream),"hipMemcpyAsync");
+}
+
+int fftwpp_amd_memset(void *dst, size_t bytes, void *stream)
+{
+  if(bytes == 0) return FFTWPP_AMD_OK;
+  return hipcheck(hipMemsetAsync(dst,0,bytes,(hipStream_t) stream),"hipMemsetAsync");
+}
+
+int fftwpp_amd_synchronize(void *stream)
+{
+  return hipcheck(hipStreamSynchronize((hipStream_t) stream),"hipStreamSynchronize");
+}
+
+int fftwpp_amd_is_device_pointer(const void *p)
+{
+  if(!p) return 0;
+  hipPointerAttribute_t attr;
+  hipError_t e=hipPointerGetAttributes(&attr,p);
+  if(e != hipSuccess) {
+    (void) hipGetLastError();
+    return 0;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+int fftwpp_amd_device_count(void)
+{
+  int n=0;
+  if(hipGetDeviceCount(&n) != hipSuccess) {
+    (void) hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+} // extern "C"

@@ -1,0 +1,125 @@
+"""PyTorch face of the convolution layer: dealiased convolutions of device
+tensors (complex128 or complex64) on the current HIP stream, and a
+torch.autograd.Function whose backward is the hand-written adjoint
+(dealiased correlations on the same padded residues).
+
+PyTorch is plumbing here: tensors provide device memory and the stream; all
+arithmetic runs in libfftwpp_amd.so.
+"""
+import ctypes
+
+import torch
+
+from ._lib import F32, F64, check, lib, sizes
+
+KINDS = {"complex": 0, "hermitian": 1, "centered": 2}
+
+
+def _dtype_code(dtype):
+    if dtype == torch.complex128:
+        return F64
+    if dtype == torch.complex64:
+        return F32
+    raise TypeError("dtype must be torch.complex128 or torch.complex64")
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class DeviceConvolution:
+    """Plan of a dealiased convolution of device tensors.
+
+    shape: stored shape of each input.  kind="complex": (L,), (Lx,Ly) or
+    (Lx,Ly,Lz) (HybridConvolution{,2,3}); kind="hermitian": (H,), (Lx,Hy) or
+    (Lx,Ly,Hz) with odd full length 2H-1 on the last axis unless
+    ``lengths`` gives the full lengths (HybridConvolutionHermitian{,2,3});
+    kind="centered": (L,) centred 1D.  M and m optionally give the minimum
+    padded sizes and forced sub-FFT sizes per dimension.
+    """
+
+    def __init__(self, shape, kind="complex", dtype=torch.complex128,
+                 M=None, m=None, lengths=None):
+        if isinstance(shape, int):
+            shape = (shape,)
+        self.shape = tuple(int(s) for s in shape)
+        self.kind = kind
+        self.dtype = dtype
+        self.dim = len(self.shape)
+        if lengths is None:
+            lengths = list(self.shape)
+            if kind == "hermitian":
+                lengths[-1] = 2 * self.shape[-1] - 1
+        self.lengths = tuple(int(v) for v in lengths)
+        h = ctypes.c_void_p()
+        check(lib().fftwpp_amd_conv_create(
+            ctypes.byref(h), self.dim, KINDS[kind], sizes(self.lengths),
+            sizes(M) if M is not None else None,
+            sizes(m) if m is not None else None, _dtype_code(dtype)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().fftwpp_amd_conv_destroy(h)
+            self._h = None
+
+    def geometry(self):
+        """(N, m, q) per dimension: padded length, sub-FFT size, residues."""
+        N, m, q = sizes([0] * 3), sizes([0] * 3), sizes([0] * 3)
+        check(lib().fftwpp_amd_conv_geometry(self._h, N, m, q))
+        d = self.dim
+        return tuple(N[:d]), tuple(m[:d]), tuple(q[:d])
+
+    def _check(self, t):
+        if not t.is_cuda:
+            raise ValueError("tensors must live on the GPU")
+        if t.dtype != self.dtype or tuple(t.shape) != self.shape:
+            raise ValueError(f"expected {self.dtype} tensor of shape {self.shape}")
+        if not t.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+
+    def convolve_(self, a, b):
+        """a <- a (*) b in place (normalised)."""
+        self._check(a)
+        self._check(b)
+        check(lib().fftwpp_amd_conv_execute(self._h, a.data_ptr(), b.data_ptr(), _stream()))
+        return a
+
+    def __call__(self, a, b):
+        out = a.clone()
+        return self.convolve_(out, b)
+
+    def adjoint(self, w, a, b, need_a=True, need_b=True):
+        """Cotangents of (a, b) for output cotangent w (PyTorch convention
+        grad = dL/d conj(z)): grad_a = w (x) b, grad_b = w (x) a."""
+        self._check(w)
+        ga = torch.empty_like(w) if need_a else None
+        gb = torch.empty_like(w) if need_b else None
+        check(lib().fftwpp_amd_conv_adjoint(
+            self._h, w.data_ptr(), a.data_ptr(), b.data_ptr(),
+            ga.data_ptr() if ga is not None else None,
+            gb.data_ptr() if gb is not None else None, _stream()))
+        return ga, gb
+
+
+class _ConvolveFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, a, b, plan):
+        a = a.contiguous()
+        b = b.contiguous()
+        ctx.plan = plan
+        ctx.save_for_backward(a, b)
+        return plan(a, b)
+
+    @staticmethod
+    def backward(ctx, w):
+        a, b = ctx.saved_tensors
+        ga, gb = ctx.plan.adjoint(w.contiguous(), a, b,
+                                  ctx.needs_input_grad[0], ctx.needs_input_grad[1])
+        return ga, gb, None
+
+
+def convolve(a, b, plan):
+    """Differentiable dealiased convolution h = a (*) b with ``plan``."""
+    return _ConvolveFn.apply(a, b, plan)

@@ -1,0 +1,170 @@
+/* fftwpp_amd.h -- C ABI of the MI355X (gfx950) hybrid dealiased convolution
+ * library libfftwpp_amd.so.
+ *
+ * Two layers are declared here:
+ *
+ *   1. The kernel layer: padded residue transforms of one dimension, the
+ *      fused row convolutions, the device multipliers and the Hermitian
+ *      symmetrisers.  All pointers are device pointers; `stream` is a
+ *      hipStream_t passed as void* (NULL = the default stream).  The C++
+ *      drop-in classes (convolve.h) are written against this layer only.
+ *   2. The convolution layer: whole 1D/2D/3D convolutions (complex or
+ *      centred Hermitian, FP64 or FP32) on device or host pointers, the
+ *      adjoint (correlation) entry points, and the batched entry points.
+ *      These extend the reference C wrapper (cfftw++.h) with device pointers,
+ *      a precision choice, status codes, batching and adjoints.
+ *
+ * Every function returns 0 (FFTWPP_AMD_OK) on success or a negative status
+ * code; fftwpp_amd_last_error() describes the last failure of the calling
+ * thread.  Nothing here calls exit(), unlike the reference
+ * (convolve.cc:480-501, fftw++.h:251-254).
+ */
+#ifndef FFTWPP_AMD_H
+#define FFTWPP_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FFTWPP_AMD_OK = 0,
+  FFTWPP_AMD_ERR_ARG = -1,          /* invalid argument / parameters     */
+  FFTWPP_AMD_ERR_HIP = -2,          /* HIP runtime failure (OOM, launch) */
+  FFTWPP_AMD_ERR_UNSUPPORTED = -3,  /* valid but not implemented here    */
+  FFTWPP_AMD_ERR_NODEVICE = -4      /* no GPU visible                    */
+};
+
+enum { FFTWPP_AMD_F64 = 0, FFTWPP_AMD_F32 = 1 };
+
+/* fftPad, fftPadCentered, fftPadHermitian (convolve.h:471-803) */
+enum {
+  FFTWPP_AMD_KIND_COMPLEX = 0,
+  FFTWPP_AMD_KIND_CENTERED = 1,
+  FFTWPP_AMD_KIND_HERMITIAN = 2
+};
+
+/* multNone, multBinary, realMultBinary, multcorrelation (convolve.h:81) */
+enum {
+  FFTWPP_AMD_MULT_NONE = 0,
+  FFTWPP_AMD_MULT_BINARY = 1,
+  FFTWPP_AMD_MULT_REAL_BINARY = 2,
+  FFTWPP_AMD_MULT_CORRELATION = 3
+};
+
+const char *fftwpp_amd_last_error(void);
+
+/* ---------------------------------------------------------------------- */
+/* 1. Kernel layer                                                         */
+/* ---------------------------------------------------------------------- */
+
+typedef struct fftwpp_amd_dim fftwpp_amd_dim;
+
+/* One padded dimension: L data values padded to N = m*q (residues q), with
+ * FP64-generated twiddles zeta_N^j on the device.  Replaces the per-fftBase
+ * tables initZetaqm/Zetaqp/ZetaqmS (convolve.cc:126-140,613-636) and the
+ * FFTW plans of fftPad::init (convolve.cc:511-719). */
+int fftwpp_amd_dim_create(fftwpp_amd_dim **d, size_t L, size_t m, size_t q,
+                          int kind);
+int fftwpp_amd_dim_destroy(fftwpp_amd_dim *d);
+
+/* Largest m the fused LDS row kernel accepts for a precision. */
+size_t fftwpp_amd_max_m(int dtype);
+
+/* Forward residues [r0, r0+nres) of nbatch x C columns.
+ * Input element (b, j, c) at f[b*bstride_in + j*S + c], j < L.
+ * Output F[b*bstride_out + ((r-r0)*m + s)*SF + c] = sum_k f_k zeta_N^{k(q s+r)}.
+ * Replaces fftPad::forward*Many / fftPadCentered::forward*Many
+ * (convolve.cc:771-1466, 2199-4307). */
+int fftwpp_amd_forward_many(const fftwpp_amd_dim *d, int dtype,
+                            const void *f, size_t S, size_t C,
+                            size_t nbatch, size_t bstride_in, void *F,
+                            size_t SF, size_t bstride_out, size_t r0,
+                            size_t nres, void *stream);
+
+/* Backward residues [r0, r0+nres): f (+)= sum_r inverse(F_r) conj-twiddled;
+ * `accumulate` adds to the existing output; `scale` multiplies the final
+ * value.  Replaces fftPad::backward*Many (convolve.cc:1468-1965). */
+int fftwpp_amd_backward_many(const fftwpp_amd_dim *d, int dtype,
+                             const void *F, size_t SF, size_t bstride_in,
+                             void *f, size_t S, size_t C, size_t nbatch,
+                             size_t bstride_out, size_t r0, size_t nres,
+                             int accumulate, double scale, void *stream);
+
+/* Fused innermost-dimension convolution of `rows` contiguous rows: every
+ * residue forward for both inputs, the multiplier, backward, accumulate.
+ * Row i of f/g starts at i*rowstride, of out at i*outstride (out may equal
+ * f).  Replaces Convolution::convolveRaw (convolve.cc:7513-7575) for the
+ * device multipliers. */
+int fftwpp_amd_conv_rows(const fftwpp_amd_dim *d, int dtype, int mult,
+                         const void *f, const void *g, size_t rows,
+                         size_t rowstride, void *out, size_t outstride,
+                         double scale, void *stream);
+
+/* The multipliers on device arrays (convolve.cc:26-110): n counts complex
+ * words for multBinary/multcorrelation and real words for realMultBinary. */
+int fftwpp_amd_multiply(int dtype, int mult, void *F0, const void *F1,
+                        size_t n, void *stream);
+
+int fftwpp_amd_scale(int dtype, void *f, size_t n, double s, void *stream);
+
+/* HermitianSymmetrizeX / XY on device data (convolve.h:1177-1256). */
+int fftwpp_amd_symmetrize_x(size_t Hx, size_t Hy, size_t x0, void *f,
+                            size_t Sx, void *stream);
+int fftwpp_amd_symmetrize_xy(size_t Hx, size_t Hy, size_t Hz, size_t x0,
+                             size_t y0, void *f, size_t Sx, size_t Sy,
+                             void *stream);
+
+/* Device memory and stream plumbing (host code includes no HIP header). */
+int fftwpp_amd_malloc(void **p, size_t bytes);
+int fftwpp_amd_free(void *p);
+int fftwpp_amd_memcpy(void *dst, const void *src, size_t bytes, void *stream);
+int fftwpp_amd_memset(void *dst, size_t bytes, void *stream);
+int fftwpp_amd_synchronize(void *stream);
+int fftwpp_amd_is_device_pointer(const void *p);
+int fftwpp_amd_device_count(void);
+
+/* ---------------------------------------------------------------------- */
+/* 2. Convolution layer                                                    */
+/* ---------------------------------------------------------------------- */
+
+typedef struct fftwpp_amd_conv fftwpp_amd_conv;
+
+/* A whole dealiased convolution of `dim` dimensions (1..3).
+ * hermitian = 0: complex non-centred (HybridConvolution{,2,3},
+ *   wrappers/HybridConvolution.h:12-79,113-147), data L[0] x ... contiguous.
+ * hermitian = 2: centred complex, dim = 1 (fftPadCentered + Convolution,
+ *   tests/hybridconv.cc:44-46 with -c); M defaults to the 3/2 rule.
+ * hermitian = 1: centred Hermitian (HybridConvolutionHermitian{,2,3},
+ *   HybridConvolution.h:33-52,80-111,149-182); L[] are the full lengths and
+ *   the last stored dimension holds ceil(L/2) modes.
+ * M[d] = 0 selects the reference default (2L-1, or 3ceil(L/2)-2(L mod 2));
+ * m[d] = 0 lets the library choose the sub-FFT size (autotuned plan).
+ * The result is normalised and written over `a`. */
+int fftwpp_amd_conv_create(fftwpp_amd_conv **h, int dim, int hermitian,
+                           const size_t *L, const size_t *M,
+                           const size_t *m, int dtype);
+int fftwpp_amd_conv_destroy(fftwpp_amd_conv *h);
+
+/* Device pointers: a <- a (*) b on `stream`. */
+int fftwpp_amd_conv_execute(fftwpp_amd_conv *h, void *a, const void *b,
+                            void *stream);
+
+/* Adjoint of the bilinear map (a,b) -> a (*) b: given the cotangent w of the
+ * output, grad_a = w (x) b and grad_b = w (x) a (dealiased correlations,
+ * multcorrelation of convolve.cc:85-110 on the same padded residues).
+ * Pointers are device pointers; either gradient pointer may be NULL. */
+int fftwpp_amd_conv_adjoint(fftwpp_amd_conv *h, const void *w, const void *a,
+                            const void *b, void *grad_a, void *grad_b,
+                            void *stream);
+
+/* Padded sizes N[d] = m*q actually used and the sub-FFT sizes m[d], q[d]. */
+int fftwpp_amd_conv_geometry(const fftwpp_amd_conv *h, size_t *N,
+                             size_t *m, size_t *q);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

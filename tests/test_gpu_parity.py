@@ -1,0 +1,392 @@
+"""GPU parity of the hybrid dealiased convolutions against the CPU oracle.
+
+Bar: relative L2 error <= 1e-12 in FP64 (tests/tests.py:223-224 uses 1e-12);
+<= 1e-5 in FP32 against the FP64 oracle.  Every call goes through the C ABI of
+libfftwpp_amd.so (ctypes), with torch providing device memory."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-5
+
+
+def rnd(rng, *shape):
+    return rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)
+
+
+def run(gpu, kind, f, g, M=None, m=None, dtype=None, lengths=None):
+    import torch
+    from fftwpp_amd.torch_conv import DeviceConvolution
+    dtype = dtype or torch.complex128
+    plan = DeviceConvolution(f.shape, kind=kind, dtype=dtype, M=M, m=m, lengths=lengths)
+    a = torch.tensor(np.ascontiguousarray(f), dtype=dtype, device=gpu)
+    b = torch.tensor(np.ascontiguousarray(g), dtype=dtype, device=gpu)
+    plan.convolve_(a, b)
+    torch.cuda.synchronize()
+    return a.cpu().numpy().astype(np.complex128), plan
+
+
+# ---------------------------------------------------------------- 1D complex
+
+def complex_grid():
+    # tests/tests.py:561-573
+    L = 8
+    for M in [16, 20]:
+        for m in [M, L + 1, L, 4, 2]:
+            yield L, M, m
+
+
+@pytest.mark.parametrize("L,M,m", list(complex_grid()))
+def test_conv1d_reference_grid(gpu, oracle, L, M, m):
+    f, g = oracle.ramp1(L, 0), oracle.ramp1(L, 1)
+    h, plan = run(gpu, "complex", f, g, M=[M], m=[m])
+    N, mm, q = plan.geometry()
+    assert mm[0] == m and N[0] == m * oracle.parameters(L, M, m, False)[2]
+    assert oracle.rel_l2(h, oracle.direct_conv(f, g)) < TOL64
+
+
+def test_conv1d_closed_form(gpu, oracle):
+    # tests/hybridconv.cc:50-69 accuracy case, L=2^6..2^11 (tests/accuracy.py)
+    iF, iG = complex(math.sqrt(3), math.sqrt(7)), complex(math.sqrt(5), math.sqrt(11))
+    for e in range(6, 12):
+        L = 2 ** e
+        j = np.arange(L)
+        z = np.exp(1j * j)
+        h, _ = run(gpu, "complex", iF * z, iG * z)
+        assert oracle.rel_l2(h, iF * iG * (j + 1) * z) < TOL64, L
+
+
+def test_conv1d_workload(gpu, oracle):
+    # L=1024 -> M=2048, uniform[-1,1] from the workload seed
+    f, g = oracle.inputs_1d(1024)
+    h, plan = run(gpu, "complex", f, g, M=[2048])
+    assert plan.geometry()[0][0] >= 2048
+    assert oracle.rel_l2(h, oracle.direct_conv(f, g)) < TOL64
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 100, 1000, 3000, 4099])
+def test_conv1d_ragged_lengths(gpu, oracle, L):
+    rng = np.random.default_rng(L)
+    f, g = rnd(rng, L), rnd(rng, L)
+    h, _ = run(gpu, "complex", f, g)
+    assert oracle.rel_l2(h, oracle.explicit_conv(f, g, 2 * L)) < TOL64
+
+
+def test_conv1d_hybrid_port_agrees(gpu, oracle):
+    rng = np.random.default_rng(11)
+    f, g = rnd(rng, 700), rnd(rng, 700)
+    for m in [700, 350, 175, 1400]:
+        _, _, q = oracle.parameters(700, 1399, m, False)
+        h, _ = run(gpu, "complex", f, g, M=[1399], m=[m])
+        assert oracle.rel_l2(h, oracle.hybrid_conv1(f, g, m, q)) < TOL64, m
+
+
+# ---------------------------------------------------------------- 1D centred
+
+def centered_grid():
+    # tests/tests.py:575-593
+    for L in [8, 7]:
+        L2 = (L + 1) // 2
+        for M in [3 * L2 - 2 * (L % 2), 3 * L2 - 2 * (L % 2) + 1, 2 * L, 5 * L2]:
+            for m in [-(-L // 4), L2, -(-(L2 + L) // 2), M]:
+                yield L, M, m
+
+
+@pytest.mark.parametrize("L,M,m", list(centered_grid()))
+def test_centered_reference_grid(gpu, oracle, L, M, m):
+    from fftwpp_amd import FFTWPPError
+    f, g = oracle.ramp1(L, 0), oracle.ramp1(L, 1)
+    p, n, q = oracle.parameters(L, M, m, True)
+    valid = q == 1 or p % 2 == 0  # fftPadCentered::Opt::valid, convolve.h:605-607
+    if not valid:
+        with pytest.raises(FFTWPPError):
+            run(gpu, "centered", f, g, M=[M], m=[m])
+        return
+    h, _ = run(gpu, "centered", f, g, M=[M], m=[m])
+    assert oracle.rel_l2(h, oracle.direct_convC(f, g)) < TOL64
+
+
+# ---------------------------------------------------------------- 1D Hermitian
+
+def hermitian_grid():
+    # tests/tests.py:595-613
+    for L in [8, 7]:
+        L2 = (L + 1) // 2
+        for M in [3 * L2 - 2 * (L % 2), 2 * L, 5 * L2]:
+            for m in [M, L2, -(-L // 4)]:
+                yield L, M, m
+
+
+@pytest.mark.parametrize("L,M,m", list(hermitian_grid()))
+def test_hconv1d_reference_grid(gpu, oracle, L, M, m):
+    from fftwpp_amd import FFTWPPError
+    H = (L + 1) // 2
+    f, g = oracle.rampH1(H, 0), oracle.rampH1(H, 1)
+    p, n, q = oracle.parameters(L, M, m, True)
+    D = 1 if q == 1 else 2
+    valid = (D == 1 and q == 1) or (D == 2 and p % 2 == 0)  # convolve.h:721-723
+    if not valid:
+        with pytest.raises(FFTWPPError):
+            run(gpu, "hermitian", f, g, M=[M], m=[m], lengths=[L])
+        return
+    h, _ = run(gpu, "hermitian", f, g, M=[M], m=[m], lengths=[L])
+    assert oracle.rel_l2(h, oracle.direct_convh(f, g)) < TOL64
+
+
+@pytest.mark.parametrize("L", [15, 64, 1023, 2047, 4095])
+def test_hconv1d_sizes(gpu, oracle, L):
+    rng = np.random.default_rng(L)
+    H = (L + 1) // 2
+    f, g = rnd(rng, H), rnd(rng, H)
+    f[0] = f[0].real
+    g[0] = g[0].real
+    h, plan = run(gpu, "hermitian", f, g, lengths=[L])
+    M = 3 * H - 2 * (L % 2)
+    assert oracle.rel_l2(h, oracle.explicit_hconv(f, g, M)) < TOL64
+
+
+# ---------------------------------------------------------------- 2D / 3D
+
+def test_conv2d_small_and_medium(gpu, oracle):
+    f, g = oracle.ramp2(8, 8, 0).astype(complex), oracle.ramp2(8, 8, 1).astype(complex)
+    h, _ = run(gpu, "complex", f, g)
+    assert oracle.rel_l2(h, oracle.direct_conv2(f, g)) < TOL64
+    rng = np.random.default_rng(21)
+    f, g = rnd(rng, 37, 29), rnd(rng, 37, 29)
+    h, _ = run(gpu, "complex", f, g)
+    assert oracle.rel_l2(h, oracle.explicit_conv(f, g, (73, 57))) < TOL64
+    f, g = rnd(rng, 200, 120), rnd(rng, 200, 120)
+    for mx, my in [(None, None), (100, 60), (50, 240)]:
+        m = None if mx is None else [mx, my]
+        h, _ = run(gpu, "complex", f, g, m=m)
+        assert oracle.rel_l2(h, oracle.explicit_conv(f, g, (399, 239))) < TOL64
+
+
+def test_conv3d_small_and_medium(gpu, oracle):
+    f = oracle.ramp3(4, 5, 6, 0).astype(complex)
+    g = oracle.ramp3(4, 5, 6, 1).astype(complex)
+    h, _ = run(gpu, "complex", f, g)
+    assert oracle.rel_l2(h, oracle.direct_conv3(f, g)) < TOL64
+    rng = np.random.default_rng(31)
+    f, g = rnd(rng, 16, 12, 20), rnd(rng, 16, 12, 20)
+    h, _ = run(gpu, "complex", f, g)
+    assert oracle.rel_l2(h, oracle.explicit_conv(f, g, (31, 23, 39))) < TOL64
+
+
+@pytest.mark.parametrize("Lx,Ly", [(7, 7), (8, 8), (7, 8), (8, 7), (9, 5)])
+def test_hconv2d_reference_cases(gpu, oracle, Lx, Ly):
+    Hy = (Ly + 1) // 2
+    f, g = oracle.ramph2(Lx, Hy, 0), oracle.ramph2(Lx, Hy, 1)
+    h, _ = run(gpu, "hermitian", f, g, lengths=[Lx, Ly])
+    assert oracle.rel_l2(h, oracle.direct_convh2(f, g)) < TOL64
+
+
+def test_hconv2d_spectral_medium(gpu, oracle):
+    # the pseudospectral workload shape at 1/16 size: kx in [-127,127], ky in [0,127]
+    f, g = oracle.inputs_h2(255, 128)
+    h, plan = run(gpu, "hermitian", f, g, lengths=[255, 255])
+    M = 3 * 128 - 2
+    assert oracle.rel_l2(h, oracle.explicit_hconv(f, g, (M, M))) < TOL64
+
+
+@pytest.mark.parametrize("shape", [(7, 7, 7), (8, 8, 8), (5, 6, 7)])
+def test_hconv3d_reference_cases(gpu, oracle, shape):
+    Lx, Ly, Lz = shape
+    Hz = (Lz + 1) // 2
+    f, g = oracle.ramph3(Lx, Ly, Hz, 0), oracle.ramph3(Lx, Ly, Hz, 1)
+    h, _ = run(gpu, "hermitian", f, g, lengths=list(shape))
+    assert oracle.rel_l2(h, oracle.direct_convh3(f, g)) < TOL64
+
+
+# ---------------------------------------------------------------- FP32 path
+
+def test_fp32_3d_error_vs_fp64(gpu, oracle):
+    import torch
+    rng = np.random.default_rng(41)
+    f, g = rnd(rng, 32, 32, 32), rnd(rng, 32, 32, 32)
+    ref = oracle.explicit_conv(f, g, 64)
+    h32, _ = run(gpu, "complex", f, g, dtype=torch.complex64)
+    err = oracle.rel_l2(h32, ref)
+    print(f"fp32 3D 32^3 rel L2 vs FP64 oracle: {err:.3e}", flush=True)
+    assert err < TOL32
+
+
+def test_fp32_hermitian_2d(gpu, oracle):
+    import torch
+    f, g = oracle.inputs_h2(127, 64)
+    ref = oracle.explicit_hconv(f, g, (190, 190))
+    h32, _ = run(gpu, "hermitian", f, g, dtype=torch.complex64, lengths=[127, 127])
+    assert oracle.rel_l2(h32, ref) < TOL32
+
+
+# ---------------------------------------------------------------- host pointers
+
+def test_reference_python_wrapper_host_arrays(gpu, oracle):
+    """The reference's Python interface (wrappers/fftwpp.py) on numpy arrays."""
+    from fftwpp_amd import fftwpp
+    f = fftwpp.complex_align([8])
+    g = fftwpp.complex_align([8])
+    f[:] = oracle.ramp1(8, 0)
+    g[:] = oracle.ramp1(8, 1)
+    want = oracle.direct_conv(f.copy(), g.copy())
+    gcopy = g.copy()
+    fftwpp.Convolution(f.shape).convolve(f, g)
+    assert oracle.rel_l2(f, want) < TOL64
+    assert np.array_equal(g, gcopy)
+    f = fftwpp.complex_align([4, 5, 6])
+    g = fftwpp.complex_align([4, 5, 6])
+    f[:] = oracle.ramp3(4, 5, 6, 0)
+    g[:] = oracle.ramp3(4, 5, 6, 1)
+    want = oracle.direct_conv3(f.copy(), g.copy())
+    fftwpp.Convolution(f.shape).convolve(f, g)
+    assert oracle.rel_l2(f, want) < TOL64
+    # HConvolution symmetrises first (wrappers/fftwpp.py:329-351)
+    f = np.ascontiguousarray(oracle.ramph2(7, 4, 0))
+    g = np.ascontiguousarray(oracle.ramph2(7, 4, 1))
+    want = oracle.direct_convh2(f.copy(), g.copy())
+    fftwpp.HConvolution(f.shape).convolve(f, g)
+    assert oracle.rel_l2(f, want) < TOL64
+
+
+def test_symmetrize_device_matches_oracle(gpu, oracle):
+    import torch
+    from fftwpp_amd import _lib
+    rng = np.random.default_rng(51)
+    for Lx, Ly, Hz in [(7, 7, 4), (8, 6, 3)]:
+        f = np.ascontiguousarray(rnd(rng, Lx, Ly, Hz))
+        want = oracle.symmetrize_xy(f.copy())
+        t = torch.tensor(f, device=gpu)
+        _lib.lib().fftwpp_HermitianSymmetrizeXY((Lx + 1) // 2, (Ly + 1) // 2, Hz, Lx // 2,
+                                                Ly // 2, t.data_ptr())
+        torch.cuda.synchronize()
+        assert np.array_equal(t.cpu().numpy(), want)
+
+
+# ---------------------------------------------------------------- kernel layer
+
+def test_forward_residues_definition(gpu, oracle):
+    """fftwpp_amd_forward_many residue r equals the padded DFT at q*s+r
+    (the transform identity of tests/hybrid.cc:60-110)."""
+    import ctypes
+    import torch
+    from fftwpp_amd import _lib
+    L_ = _lib.lib()
+    rng = np.random.default_rng(61)
+    for L, m, q, kind in [(8, 8, 2, 0), (12, 4, 6, 0), (7, 4, 3, 1), (30, 10, 6, 1)]:
+        C = 5
+        f = rnd(rng, L, C)
+        d = ctypes.c_void_p()
+        _lib.check(L_.fftwpp_amd_dim_create(ctypes.byref(d), L, m, q, kind))
+        fd = torch.tensor(f, device=gpu)
+        Fd = torch.zeros((q * m, C), dtype=torch.complex128, device=gpu)
+        _lib.check(L_.fftwpp_amd_forward_many(d, 0, fd.data_ptr(), C, C, 1, 0,
+                                              Fd.data_ptr(), C, 0, 0, q, None))
+        torch.cuda.synchronize()
+        F = Fd.cpu().numpy()
+        k = np.arange(L) - (L // 2 if kind == 1 else 0)
+        N = m * q
+        for r in range(q):
+            ell = q * np.arange(m) + r
+            want = np.exp(2j * np.pi * np.outer(ell, k) / N) @ f
+            assert oracle.rel_l2(F[r * m:(r + 1) * m], want) < 1e-13
+        # backward o forward = N * identity (tests/hybrid.cc:112-167)
+        gd = torch.zeros_like(fd)
+        _lib.check(L_.fftwpp_amd_backward_many(d, 0, Fd.data_ptr(), C, 0, gd.data_ptr(), C, C,
+                                               1, 0, 0, q, 0, 1.0 / N, None))
+        torch.cuda.synchronize()
+        assert oracle.rel_l2(gd.cpu().numpy(), f) < 1e-13
+        L_.fftwpp_amd_dim_destroy(d)
+
+
+# ---------------------------------------------------------------- adjoint
+
+def test_adjoint_matches_direct_correlation(gpu, oracle):
+    import torch
+    from fftwpp_amd.torch_conv import DeviceConvolution
+    rng = np.random.default_rng(71)
+    for shape in [(33,), (9, 11), (5, 6, 7)]:
+        a, b, w = rnd(rng, *shape), rnd(rng, *shape), rnd(rng, *shape)
+        plan = DeviceConvolution(shape)
+        T = lambda x: torch.tensor(x, device=gpu)
+        ga, gb = plan.adjoint(T(w), T(a), T(b))
+        torch.cuda.synchronize()
+        # grad_a[i] = sum_k w[k] conj(b[k-i]) over the stored ranges
+        want_a = _correlate(w, b)
+        want_b = _correlate(w, a)
+        assert oracle.rel_l2(ga.cpu().numpy(), want_a) < TOL64
+        assert oracle.rel_l2(gb.cpu().numpy(), want_b) < TOL64
+
+
+def _correlate(w, b):
+    import scipy.signal as ss
+    # c[i] = sum_j w[i+j] conj(b[j]) for i, j, i+j inside the box
+    full = ss.correlate(w, b, mode="full")  # full[n] = sum_j w[n-(len-1)+j] conj(b[j])
+    idx = tuple(slice(s - 1, 2 * s - 1) for s in w.shape)
+    return full[idx]
+
+
+def test_autograd_finite_differences(gpu, oracle):
+    import torch
+    from fftwpp_amd.torch_conv import DeviceConvolution, convolve
+    rng = np.random.default_rng(81)
+    shape = (6, 7)
+    plan = DeviceConvolution(shape)
+    a0, b0, t0 = rnd(rng, *shape), rnd(rng, *shape), rnd(rng, *shape)
+    T = lambda x: torch.tensor(x, device=gpu)
+    a = T(a0).requires_grad_(True)
+    b = T(b0).requires_grad_(True)
+    loss = (convolve(a, b, plan) - T(t0)).abs().pow(2).sum()
+    loss.backward()
+
+    def L(av, bv):
+        h = oracle.explicit_conv(av, bv, (11, 13))
+        return float(np.sum(np.abs(h - t0) ** 2))
+
+    eps = 1e-6
+    ga = a.grad.cpu().numpy()
+    for idx in [(0, 0), (3, 4), (5, 6)]:
+        for unit in [1.0, 1j]:
+            ap, am = a0.copy(), a0.copy()
+            ap[idx] += eps * unit
+            am[idx] -= eps * unit
+            fd = (L(ap, b0) - L(am, b0)) / (2 * eps)
+            got = ga[idx].real if unit == 1.0 else ga[idx].imag
+            assert abs(fd - got) < 1e-5 * max(1.0, abs(fd)), (idx, unit, fd, got)
+
+
+# ---------------------------------------------------------------- full size
+
+def test_conv3d_256_full_size_sampled_direct(gpu, oracle):
+    """256^3 -> 512^3 in FP64 and FP32: FP64 checked against direct sums at
+    sampled outputs, FP32 against FP64 over the whole array, plus symmetry
+    f (*) g == g (*) f."""
+    import torch
+    from fftwpp_amd.torch_conv import DeviceConvolution
+    f, g = oracle.inputs_3d(256)
+    fd = torch.tensor(f, device=gpu)
+    gd = torch.tensor(g, device=gpu)
+    plan = DeviceConvolution((256, 256, 256))
+    h = plan(fd, gd)
+    h2 = plan(gd, fd)
+    torch.cuda.synchronize()
+    H = h.cpu().numpy()
+    assert oracle.rel_l2(h2.cpu().numpy(), H) < TOL64
+    rng = np.random.default_rng(91)
+    pts = [(255, 255, 255), (0, 0, 0), (128, 3, 250)] + \
+        [tuple(rng.integers(0, 256, 3)) for _ in range(5)]
+    num = den = 0.0
+    for (i, j, k) in pts:
+        direct = np.sum(f[:i + 1, :j + 1, :k + 1] * g[i::-1, j::-1, k::-1])
+        num += abs(H[i, j, k] - direct) ** 2
+        den += abs(direct) ** 2
+    assert math.sqrt(num / den) < TOL64
+    p32 = DeviceConvolution((256, 256, 256), dtype=torch.complex64)
+    h32 = p32(fd.to(torch.complex64), gd.to(torch.complex64))
+    torch.cuda.synchronize()
+    err = oracle.rel_l2(h32.cpu().numpy(), H)
+    print(f"fp32 256^3 rel L2 vs FP64: {err:.3e}", flush=True)
+    assert err < TOL32
