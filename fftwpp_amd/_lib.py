@@ -34,12 +34,20 @@ SIGNATURES = {
     "fftwpp_amd_synchronize": (I, [P]),
     "fftwpp_amd_is_device_pointer": (I, [P]),
     "fftwpp_amd_device_count": (I, []),
+    "fftwpp_amd_event_create": (I, [ctypes.POINTER(P)]),
+    "fftwpp_amd_event_record": (I, [P, P]),
+    "fftwpp_amd_event_elapsed": (I, [ctypes.POINTER(ctypes.c_float), P, P]),
+    "fftwpp_amd_event_destroy": (I, [P]),
     # convolution layer
     "fftwpp_amd_conv_create": (I, [ctypes.POINTER(P), I, I, ZP, ZP, ZP, I]),
     "fftwpp_amd_conv_destroy": (I, [P]),
     "fftwpp_amd_conv_execute": (I, [P, P, P, P]),
     "fftwpp_amd_conv_adjoint": (I, [P, P, P, P, P, P, P]),
     "fftwpp_amd_conv_geometry": (I, [P, ZP, ZP, ZP]),
+    "fftwpp_amd_conv_execute_out": (I, [P, P, P, P, P]),
+    "fftwpp_amd_conv_execute_timed": (I, [P, P, P, P, P, ctypes.POINTER(ctypes.c_float), I,
+                                          ctypes.POINTER(I)]),
+    "fftwpp_amd_conv_stage_name": (ctypes.c_char_p, [P, I]),
     # cfftw++.h drop-in
     "create_doubleAlign": (P, [Z]),
     "delete_doubleAlign": (None, [P]),

@@ -1,0 +1,92 @@
+// fft256.h -- register-resident 256-point FFT for the sub-FFT size m=256.
+//
+// Sixteen cooperating lanes hold one sequence; lane j keeps x[j + 16k] in
+// v[k], k < 16 (the same natural layout before and after the transform).
+// 256 = 16 x 16 Stockham: a radix-16 DFT in registers, one transpose through
+// LDS (rows padded to 17 entries), a twiddle by w_256^{t j} and a second
+// radix-16 DFT.  Twiddles are read from the dimension's FP64-generated table
+// zN (N = 256 q), w_256^n = zN[n q].
+#pragma once
+#include "common.h"
+
+namespace fftwpp_amd {
+
+template<typename T> struct K16;
+template<> struct K16<double> {
+  static constexpr double c1=0.92387953251128675612818318939678829;  // cos(pi/8)
+  static constexpr double s1=0.38268343236508977172845998403039887;  // sin(pi/8)
+  static constexpr double r2=0.70710678118654752440084436210484904;
+};
+template<> struct K16<float> {
+  static constexpr float c1=0.92387953251128675612818318939678829f;
+  static constexpr float s1=0.38268343236508977172845998403039887f;
+  static constexpr float r2=0.70710678118654752440084436210484904f;
+};
+
+// In-place DFT4 of (a,b,c,d) with kernel e^{S 2 pi i ut/4}.
+template<typename T, int S>
+__device__ __forceinline__ void dft4(cplx<T> &a, cplx<T> &b, cplx<T> &c,
+                                     cplx<T> &d)
+{
+  cplx<T> s02=a+c, d02=a-c, s13=b+d, d13=muli(b-d,S);
+  a=s02+s13; c=s02-s13; b=d02+d13; d=d02-d13;
+}
+
+// v <- DFT16(v) with kernel e^{S 2 pi i ut/16}; t = t2 + 4 t1, u = u1 + 4 u2.
+template<typename T, int S>
+__device__ __forceinline__ void dft16(cplx<T> *v)
+{
+  // DFT4 over t1 for each t2: a[t2][u1] stored back at v[t2 + 4 u1]
+#pragma unroll
+  for(int t2=0; t2 < 4; ++t2)
+    dft4<T,S>(v[t2],v[t2+4],v[t2+8],v[t2+12]);
+  // twiddles w16^{u1 t2}
+  const T c1=K16<T>::c1, s1=K16<T>::s1, r2=K16<T>::r2;
+  const T s=(T) S;
+  auto tw=[&](cplx<T> &x, T wr, T wi) { x=x*mk(wr,wi); };
+  tw(v[1+4*1],c1,s*s1);   // t2=1,u1=1: w^1
+  tw(v[1+4*2],r2,s*r2);   // w^2
+  tw(v[1+4*3],s1,s*c1);   // w^3
+  tw(v[2+4*1],r2,s*r2);   // t2=2,u1=1: w^2
+  v[2+4*2]=muli(v[2+4*2],S);  // w^4
+  tw(v[2+4*3],-r2,s*r2);  // w^6
+  tw(v[3+4*1],s1,s*c1);   // t2=3,u1=1: w^3
+  tw(v[3+4*2],-r2,s*r2);  // w^6
+  tw(v[3+4*3],-c1,-s*s1); // w^9
+  // DFT4 over t2 for each u1: inputs v[t2 + 4 u1], output index u1 + 4 u2
+  cplx<T> y[16];
+#pragma unroll
+  for(int u1=0; u1 < 4; ++u1) {
+    cplx<T> a=v[4*u1], b=v[4*u1+1], c=v[4*u1+2], d=v[4*u1+3];
+    dft4<T,S>(a,b,c,d);
+    y[u1]=a; y[u1+4]=b; y[u1+8]=c; y[u1+12]=d;
+  }
+#pragma unroll
+  for(int k=0; k < 16; ++k) v[k]=y[k];
+}
+
+// 256-point transform with kernel e^{S 2 pi i jk/256} of the sequence held by
+// the 16 lanes j of a group; `row` is the group's LDS area (16*17 entries).
+// Every thread of the block must call this (it contains block barriers).
+template<typename T, int S>
+__device__ __forceinline__ void fft256(cplx<T> *v, cplx<T> *row, int j,
+                                       const cplx<T> *zN, int64_t q)
+{
+  dft16<T,S>(v);
+  // stage-1 output element e = 16 j + u  (stored at j*17 + u)
+#pragma unroll
+  for(int u=0; u < 16; ++u) row[j*17+u]=v[u];
+  __syncthreads();
+  // stage-2 input element j + 16 t (stored at t*17 + j)
+#pragma unroll
+  for(int t=0; t < 16; ++t) v[t]=row[t*17+j];
+  __syncthreads();
+#pragma unroll
+  for(int t=1; t < 16; ++t) {
+    cplx<T> w=zN[(t*j)*(int) q];
+    v[t]=S > 0 ? v[t]*w : mulc(v[t],w);
+  }
+  dft16<T,S>(v);
+}
+
+} // namespace fftwpp_amd

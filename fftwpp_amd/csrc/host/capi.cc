@@ -312,15 +312,25 @@ struct fftwpp_amd_conv {
   fftwpp::DeviceWork work;
 
   fftBase *dims[3];
-  void exec(void *a, const void *b, void *stream, int mult) {
+  fftwpp::Timeline timeline;
+  void setTimeline(fftwpp::Timeline *t) {
+    if(c1) c1->convolve->timeline=t;
+    if(h1) h1->convolve->timeline=t;
+    if(k1) k1->convolve->timeline=t;
+    if(c2) c2->convolve2->timeline=t;
+    if(h2) h2->convolve2->timeline=t;
+    if(c3) c3->convolve3->timeline=t;
+    if(h3) h3->convolve3->timeline=t;
+  }
+  void exec(void *out, const void *a, const void *b, void *stream, int mult) {
     bool norm=true;
-    if(c1) c1->convolve->execDevice(a,b,norm,dtype,stream,mult);
-    else if(h1) h1->convolve->execDevice(a,b,norm,dtype,stream,mult);
-    else if(k1) k1->convolve->execDevice(a,b,norm,dtype,stream,mult);
-    else if(c2) c2->convolve2->execDevice(a,b,norm,dtype,stream,mult);
-    else if(h2) h2->convolve2->execDevice(a,b,norm,dtype,stream,mult);
-    else if(c3) c3->convolve3->execDevice(a,b,norm,dtype,stream,mult);
-    else if(h3) h3->convolve3->execDevice(a,b,norm,dtype,stream,mult);
+    if(c1) c1->convolve->execDevice(out,a,b,norm,dtype,stream,mult);
+    else if(h1) h1->convolve->execDevice(out,a,b,norm,dtype,stream,mult);
+    else if(k1) k1->convolve->execDevice(out,a,b,norm,dtype,stream,mult);
+    else if(c2) c2->convolve2->execDevice(out,a,b,norm,dtype,stream,mult);
+    else if(h2) h2->convolve2->execDevice(out,a,b,norm,dtype,stream,mult);
+    else if(c3) c3->convolve3->execDevice(out,a,b,norm,dtype,stream,mult);
+    else if(h3) h3->convolve3->execDevice(out,a,b,norm,dtype,stream,mult);
   }
 };
 
@@ -416,11 +426,60 @@ int fftwpp_amd_conv_execute(fftwpp_amd_conv *c, void *a, const void *b,
     return FFTWPP_AMD_ERR_ARG;
   }
   try {
-    c->exec(a,b,stream,-1);
+    c->exec(a,a,b,stream,-1);
   } catch(const std::exception &e) {
     return conv_status("fftwpp_amd_conv_execute",e);
   }
   return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_conv_execute_out(fftwpp_amd_conv *c, void *out, const void *a,
+                                const void *b, void *stream)
+{
+  if(!c || !out || !a || !b) return FFTWPP_AMD_ERR_ARG;
+  if(!fftwpp_amd_is_device_pointer(out) || !fftwpp_amd_is_device_pointer(a) ||
+     !fftwpp_amd_is_device_pointer(b)) {
+    fftwpp_amd_internal_set_error("fftwpp_amd_conv_execute_out: arguments must be device pointers");
+    return FFTWPP_AMD_ERR_ARG;
+  }
+  try {
+    c->exec(out,a,b,stream,-1);
+  } catch(const std::exception &e) {
+    return conv_status("fftwpp_amd_conv_execute_out",e);
+  }
+  return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_conv_execute_timed(fftwpp_amd_conv *c, void *out, const void *a,
+                                  const void *b, void *stream,
+                                  float *stage_ms, int max_stages,
+                                  int *nstages)
+{
+  if(!c || !out || !a || !b) return FFTWPP_AMD_ERR_ARG;
+  if(!fftwpp_amd_is_device_pointer(out) || !fftwpp_amd_is_device_pointer(a) ||
+     !fftwpp_amd_is_device_pointer(b)) {
+    fftwpp_amd_internal_set_error("fftwpp_amd_conv_execute_timed: a and b must be device pointers");
+    return FFTWPP_AMD_ERR_ARG;
+  }
+  try {
+    c->setTimeline(&c->timeline);
+    c->exec(out,a,b,stream,-1);
+    c->setTimeline(NULL);
+    int n=(int) c->timeline.stages();
+    if(nstages) *nstages=n;
+    for(int i=0; i < n && i < max_stages; ++i)
+      if(stage_ms) stage_ms[i]=(float) c->timeline.stage(i);
+  } catch(const std::exception &e) {
+    c->setTimeline(NULL);
+    return conv_status("fftwpp_amd_conv_execute_timed",e);
+  }
+  return FFTWPP_AMD_OK;
+}
+
+const char *fftwpp_amd_conv_stage_name(const fftwpp_amd_conv *c, int i)
+{
+  if(!c || i < 0 || i >= (int) c->timeline.names.size()) return "";
+  return c->timeline.names[i].c_str();
 }
 
 int fftwpp_amd_conv_adjoint(fftwpp_amd_conv *c, const void *w, const void *a,
@@ -444,13 +503,13 @@ int fftwpp_amd_conv_adjoint(fftwpp_amd_conv *c, const void *w, const void *a,
       if(!b) return FFTWPP_AMD_ERR_ARG;
       int rc=fftwpp_amd_memcpy(grad_a,w,bytes,stream);
       if(rc) return rc;
-      c->exec(grad_a,b,stream,FFTWPP_AMD_MULT_CORRELATION);
+      c->exec(grad_a,grad_a,b,stream,FFTWPP_AMD_MULT_CORRELATION);
     }
     if(grad_b) {
       if(!a) return FFTWPP_AMD_ERR_ARG;
       int rc=fftwpp_amd_memcpy(grad_b,w,bytes,stream);
       if(rc) return rc;
-      c->exec(grad_b,a,stream,FFTWPP_AMD_MULT_CORRELATION);
+      c->exec(grad_b,grad_b,a,stream,FFTWPP_AMD_MULT_CORRELATION);
     }
   } catch(const std::exception &e) {
     return conv_status("fftwpp_amd_conv_adjoint",e);

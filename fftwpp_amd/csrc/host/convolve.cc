@@ -506,6 +506,45 @@ void fftPadHermitian::init()
 }
 
 // ------------------------------------------------------------------------
+// Timeline
+// ------------------------------------------------------------------------
+void Timeline::record()
+{
+  if(used == events.size()) {
+    void *e=NULL;
+    check(fftwpp_amd_event_create(&e),"event");
+    events.push_back(e);
+  }
+  check(fftwpp_amd_event_record(events[used++],stream),"event record");
+}
+
+void Timeline::begin(void *s)
+{
+  stream=s;
+  used=0;
+  names.clear();
+  record();
+}
+
+void Timeline::mark(const char *name)
+{
+  names.push_back(name);
+  record();
+}
+
+double Timeline::stage(size_t i)
+{
+  float ms=0.0f;
+  check(fftwpp_amd_event_elapsed(&ms,events[i],events[i+1]),"event elapsed");
+  return ms;
+}
+
+Timeline::~Timeline()
+{
+  for(void *e : events) fftwpp_amd_event_destroy(e);
+}
+
+// ------------------------------------------------------------------------
 // Device work buffers
 // ------------------------------------------------------------------------
 void *DeviceWork::get(size_t slot, size_t bytes)
@@ -584,10 +623,10 @@ size_t Convolution::arraySize() const
 void Convolution::convolveDevice(Complex *f0, const Complex *f1,
                                  bool normalize, void *stream)
 {
-  execDevice(f0,f1,normalize,FFTWPP_AMD_F64,stream);
+  execDevice(f0,f0,f1,normalize,FFTWPP_AMD_F64,stream);
 }
 
-void Convolution::execDevice(void *f0, const void *f1, bool normalize,
+void Convolution::execDevice(void *out, const void *f0, const void *f1, bool normalize,
                              int dtype, void *stream, int mult)
 {
   int code=mult >= 0 ? mult : multCode;
@@ -596,9 +635,11 @@ void Convolution::execDevice(void *f0, const void *f1, bool normalize,
     throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"1D Convolution with C > 1");
   double s=normalize ? scale : 1.0;
   size_t n=fft->inputLength();
+  if(timeline) timeline->begin(stream);
   if(fft->m <= fftwpp_amd_max_m(dtype)) {
-    check(fftwpp_amd_conv_rows(fft->dev,dtype,code,f0,f1,1,n,f0,n,s,
+    check(fftwpp_amd_conv_rows(fft->dev,dtype,code,f0,f1,1,n,out,n,s,
                                stream),"conv_rows");
+    if(timeline) timeline->mark("rows");
     return;
   }
   if(fft->kind() == FFTWPP_AMD_KIND_HERMITIAN)
@@ -611,13 +652,14 @@ void Convolution::execDevice(void *f0, const void *f1, bool normalize,
   check(fftwpp_amd_forward_many(fft->dev,dtype,f0,1,1,1,0,T0,1,0,0,fft->q,stream),"forward");
   check(fftwpp_amd_forward_many(fft->dev,dtype,f1,1,1,1,0,T1,1,0,0,fft->q,stream),"forward");
   check(fftwpp_amd_multiply(dtype,code,T0,T1,code == FFTWPP_AMD_MULT_REAL_BINARY ? 2*words : words,stream),"multiply");
-  check(fftwpp_amd_backward_many(fft->dev,dtype,T0,1,0,f0,1,1,1,0,0,fft->q,0,s,stream),"backward");
+  check(fftwpp_amd_backward_many(fft->dev,dtype,T0,1,0,out,1,1,1,0,0,fft->q,0,s,stream),"backward");
+  if(timeline) timeline->mark("unfused");
 }
 
 void Convolution::run(Complex **f, size_t offset, bool normalize)
 {
   staged(work,f,offset,arraySize(),2,[&](Complex *a, Complex *b) {
-    execDevice(a,b,normalize,FFTWPP_AMD_F64,NULL);
+    execDevice(a,a,b,normalize,FFTWPP_AMD_F64,NULL);
   });
 }
 
@@ -707,10 +749,10 @@ size_t Convolution2::arraySize() const { return Sx*(Lx-1)+Ly; }
 void Convolution2::convolveDevice(Complex *f0, const Complex *f1,
                                   bool normalize, void *stream)
 {
-  execDevice(f0,f1,normalize,FFTWPP_AMD_F64,stream);
+  execDevice(f0,f0,f1,normalize,FFTWPP_AMD_F64,stream);
 }
 
-void Convolution2::execDevice(void *f0, const void *f1, bool normalize,
+void Convolution2::execDevice(void *out, const void *f0, const void *f1, bool normalize,
                               int dtype, void *stream, int mult)
 {
   int code=mult >= 0 ? mult : multCode;
@@ -723,16 +765,21 @@ void Convolution2::execDevice(void *f0, const void *f1, bool normalize,
   size_t rows=qx*mx;
   void *F0=work.get(0,rows*ly*E);
   void *F1=work.get(1,rows*ly*E);
+  if(timeline) timeline->begin(stream);
   check(fftwpp_amd_forward_many(fftx->dev,dtype,f0,Sx,ly,1,0,F0,ly,0,0,qx,stream),"x forward");
+  if(timeline) timeline->mark("x forward f");
   check(fftwpp_amd_forward_many(fftx->dev,dtype,f1,Sx,ly,1,0,F1,ly,0,0,qx,stream),"x forward");
+  if(timeline) timeline->mark("x forward g");
   check(fftwpp_amd_conv_rows(ffty->dev,dtype,code,F0,F1,rows,ly,F0,ly,1.0,stream),"y rows");
-  check(fftwpp_amd_backward_many(fftx->dev,dtype,F0,ly,0,f0,Sx,ly,1,0,0,qx,0,s,stream),"x backward");
+  if(timeline) timeline->mark("y rows");
+  check(fftwpp_amd_backward_many(fftx->dev,dtype,F0,ly,0,out,Sx,ly,1,0,0,qx,0,s,stream),"x backward");
+  if(timeline) timeline->mark("x backward");
 }
 
 void Convolution2::run(Complex **f, size_t offset, bool normalize)
 {
   staged(work,f,offset,arraySize(),2,[&](Complex *a, Complex *b) {
-    execDevice(a,b,normalize,FFTWPP_AMD_F64,NULL);
+    execDevice(a,a,b,normalize,FFTWPP_AMD_F64,NULL);
   });
 }
 
@@ -765,10 +812,10 @@ size_t Convolution3::arraySize() const { return Sx*(Lx-1)+Sy*(Ly-1)+lz; }
 void Convolution3::convolveDevice(Complex *f0, const Complex *f1,
                                   bool normalize, void *stream)
 {
-  execDevice(f0,f1,normalize,FFTWPP_AMD_F64,stream);
+  execDevice(f0,f0,f1,normalize,FFTWPP_AMD_F64,stream);
 }
 
-void Convolution3::execDevice(void *f0, const void *f1, bool normalize,
+void Convolution3::execDevice(void *out, const void *f0, const void *f1, bool normalize,
                               int dtype, void *stream, int mult)
 {
   int code=mult >= 0 ? mult : multCode;
@@ -786,6 +833,7 @@ void Convolution3::execDevice(void *f0, const void *f1, bool normalize,
   void *Y0=work.get(2,planes*yplane*E);
   void *Y1=work.get(3,planes*yplane*E);
   // x forward: columns (y,z); non-contiguous y strides become batches
+  if(timeline) timeline->begin(stream);
   const void *in[2]={f0,f1};
   void *X[2]={X0,X1};
   void *Y[2]={Y0,Y1};
@@ -794,20 +842,25 @@ void Convolution3::execDevice(void *f0, const void *f1, bool normalize,
       check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,plane,1,0,X[a],plane,0,0,qx,stream),"x forward");
     else
       check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,lz,Ly,Sy,X[a],plane,lz,0,qx,stream),"x forward");
+    if(timeline) timeline->mark(a == 0 ? "x forward f" : "x forward g");
     check(fftwpp_amd_forward_many(ffty->dev,dtype,X[a],lz,lz,planes,plane,Y[a],lz,yplane,0,qy,stream),"y forward");
+    if(timeline) timeline->mark(a == 0 ? "y forward f" : "y forward g");
   }
   check(fftwpp_amd_conv_rows(fftz->dev,dtype,code,Y0,Y1,planes*qy*my,lz,Y0,lz,1.0,stream),"z rows");
+  if(timeline) timeline->mark("z rows");
   check(fftwpp_amd_backward_many(ffty->dev,dtype,Y0,lz,yplane,X0,lz,lz,planes,plane,0,qy,0,1.0,stream),"y backward");
+  if(timeline) timeline->mark("y backward");
   if(contiguous())
-    check(fftwpp_amd_backward_many(fftx->dev,dtype,X0,plane,0,f0,Sx,plane,1,0,0,qx,0,s,stream),"x backward");
+    check(fftwpp_amd_backward_many(fftx->dev,dtype,X0,plane,0,out,Sx,plane,1,0,0,qx,0,s,stream),"x backward");
   else
-    check(fftwpp_amd_backward_many(fftx->dev,dtype,X0,plane,lz,f0,Sx,lz,Ly,Sy,0,qx,0,s,stream),"x backward");
+    check(fftwpp_amd_backward_many(fftx->dev,dtype,X0,plane,lz,out,Sx,lz,Ly,Sy,0,qx,0,s,stream),"x backward");
+  if(timeline) timeline->mark("x backward");
 }
 
 void Convolution3::run(Complex **f, size_t offset, bool normalize)
 {
   staged(work,f,offset,arraySize(),2,[&](Complex *a, Complex *b) {
-    execDevice(a,b,normalize,FFTWPP_AMD_F64,NULL);
+    execDevice(a,a,b,normalize,FFTWPP_AMD_F64,NULL);
   });
 }
 

@@ -23,6 +23,7 @@
 
 #include "common.h"
 #include "fft_lds.h"
+#include "kernels256.h"
 #include "../../include/fftwpp_amd.h"
 
 using namespace fftwpp_amd;
@@ -415,6 +416,22 @@ static int allow_lds(K kernel, size_t bytes)
 }
 static size_t lds_max() { return 160*1024; }
 
+// The register-resident m=256 kernels apply to non-centred complex
+// dimensions with p = 1.
+static bool fast256(const fftwpp_amd_dim *d, size_t s1=0, size_t s2=0)
+{
+  // lane offsets (at most 16 strides plus a tile width) must fit in 31 bits
+  const size_t lim=(size_t) 1 << 30;
+  return d->m == 256 && d->kind == FFTWPP_AMD_KIND_COMPLEX && d->L <= 256 &&
+    16*s1+64 < lim && 16*s2+64 < lim;
+}
+
+template<typename T>
+static constexpr int cols256() { return sizeof(T) == 8 ? 8 : 16; }
+
+template<typename T>
+static constexpr int rows256() { return sizeof(T) == 8 ? 4 : 8; }
+
 template<typename T>
 static int fwd_many_t(const fftwpp_amd_dim *d, const void *f, size_t S,
                       size_t C, size_t nbatch, size_t bsi, void *F,
@@ -422,6 +439,15 @@ static int fwd_many_t(const fftwpp_amd_dim *d, const void *f, size_t S,
                       hipStream_t st)
 {
   const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
+  if(fast256(d,S,SF)) {
+    constexpr int TC=cols256<T>();
+    dim3 grid((unsigned) ((C+TC-1)/TC),(unsigned) nbatch);
+    hipLaunchKernelGGL((colfwd256_kernel<T,TC>),grid,dim3(16*TC),0,st,
+                       (const cplx<T> *) f,(int64_t) S,(int64_t) C,(int64_t) bsi,
+                       (cplx<T> *) F,(int64_t) SF,(int64_t) bso,(int) d->L,
+                       (int64_t) r0,(int64_t) nres,(const cplx<T> *) a.zN,(int64_t) d->q);
+    return launch_check("colfwd256_kernel");
+  }
   size_t elt=sizeof(cplx<T>);
   size_t per=2*(d->m+1)*elt;
   if(per > lds_max()) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"m too large for the LDS transform");
@@ -445,6 +471,16 @@ static int bwd_many_t(const fftwpp_amd_dim *d, const void *F, size_t SF,
                       int accumulate, double scale, hipStream_t st)
 {
   const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
+  if(fast256(d,S,SF)) {
+    constexpr int TC=cols256<T>();
+    dim3 grid((unsigned) ((C+TC-1)/TC),(unsigned) nbatch);
+    hipLaunchKernelGGL((colbwd256_kernel<T,TC>),grid,dim3(16*TC),0,st,
+                       (const cplx<T> *) F,(int64_t) SF,(int64_t) bsi,(cplx<T> *) f,
+                       (int64_t) S,(int64_t) C,(int64_t) bso,(int) d->L,(int64_t) r0,
+                       (int64_t) nres,accumulate,(T) scale,(const cplx<T> *) a.zN,
+                       (int64_t) d->q);
+    return launch_check("colbwd256_kernel");
+  }
   size_t elt=sizeof(cplx<T>);
   size_t per=2*(d->m+1)*elt;
   if(per > lds_max()) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"m too large for the LDS transform");
@@ -467,6 +503,24 @@ static int conv_rows_t(const fftwpp_amd_dim *d, int mult, const void *f,
                        size_t os, double scale, hipStream_t st)
 {
   const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
+  if(fast256(d,rs,os)) {
+    constexpr int NR=rows256<T>();
+    dim3 grid((unsigned) ((rows+NR-1)/NR));
+    const cplx<T> *zt=(const cplx<T> *) a.zN;
+    if(mult == FFTWPP_AMD_MULT_CORRELATION)
+      hipLaunchKernelGGL((rowconv256_kernel<T,3,NR>),grid,dim3(16*NR),0,st,
+                         (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
+                         (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt,(int64_t) d->q);
+    else if(mult == FFTWPP_AMD_MULT_NONE)
+      hipLaunchKernelGGL((rowconv256_kernel<T,0,NR>),grid,dim3(16*NR),0,st,
+                         (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
+                         (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt,(int64_t) d->q);
+    else
+      hipLaunchKernelGGL((rowconv256_kernel<T,1,NR>),grid,dim3(16*NR),0,st,
+                         (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
+                         (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt,(int64_t) d->q);
+    return launch_check("rowconv256_kernel");
+  }
   size_t elt=sizeof(cplx<T>);
   bool herm=d->kind == FFTWPP_AMD_KIND_HERMITIAN;
   size_t lout=herm ? (d->L+1)/2 : d->L;
@@ -743,6 +797,31 @@ int fftwpp_amd_is_device_pointer(const void *p)
     return 0;
   }
   return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+int fftwpp_amd_event_create(void **event)
+{
+  if(!event) return fail(FFTWPP_AMD_ERR_ARG,"null event");
+  return hipcheck(hipEventCreate((hipEvent_t *) event),"hipEventCreate");
+}
+
+int fftwpp_amd_event_record(void *event, void *stream)
+{
+  return hipcheck(hipEventRecord((hipEvent_t) event,(hipStream_t) stream),"hipEventRecord");
+}
+
+int fftwpp_amd_event_elapsed(float *ms, void *start, void *stop)
+{
+  if(!ms) return fail(FFTWPP_AMD_ERR_ARG,"null output");
+  int rc=hipcheck(hipEventSynchronize((hipEvent_t) stop),"hipEventSynchronize");
+  if(rc) return rc;
+  return hipcheck(hipEventElapsedTime(ms,(hipEvent_t) start,(hipEvent_t) stop),"hipEventElapsedTime");
+}
+
+int fftwpp_amd_event_destroy(void *event)
+{
+  if(!event) return FFTWPP_AMD_OK;
+  return hipcheck(hipEventDestroy((hipEvent_t) event),"hipEventDestroy");
 }
 
 int fftwpp_amd_device_count(void)

@@ -265,6 +265,23 @@ public:
   int kind() const { return FFTWPP_AMD_KIND_HERMITIAN; }
 };
 
+// Optional per-launch timing of a driver: an event after every launch on the
+// launch stream.  stage(i) is the duration of launch i in milliseconds.
+class Timeline {
+public:
+  std::vector<void *> events;
+  std::vector<std::string> names;
+  size_t used=0;
+  void *stream=NULL;
+  void begin(void *s);
+  void mark(const char *name);
+  double stage(size_t i);
+  size_t stages() const { return names.size(); }
+  ~Timeline();
+private:
+  void record();
+};
+
 // Device staging buffers shared by the convolution drivers.
 class DeviceWork {
 public:
@@ -302,11 +319,14 @@ public:
   // Device-pointer entry (stream = hipStream_t as void*).
   void convolveDevice(Complex *f0, const Complex *f1, bool normalize,
                       void *stream=NULL);
-  // dtype FFTWPP_AMD_F64/F32 on complex128/complex64 device arrays; mult >= 0
-  // overrides the Application's multiplier (the adjoint uses multcorrelation).
-  void execDevice(void *f0, const void *f1, bool normalize, int dtype,
-                  void *stream, int mult=-1);
+  // out <- f0 (*) f1 for dtype FFTWPP_AMD_F64/F32 on complex128/complex64
+  // device arrays (out may equal f0); mult >= 0 overrides the Application's
+  // multiplier (the adjoint uses multcorrelation).
+  void execDevice(void *out, const void *f0, const void *f1, bool normalize,
+                  int dtype, void *stream, int mult=-1);
   size_t arraySize() const;  // complex words per input array
+
+  Timeline *timeline=NULL;  // when set, every launch is timed
 
 protected:
   int multCode;
@@ -348,11 +368,14 @@ public:
 
   void convolveDevice(Complex *f0, const Complex *f1, bool normalize,
                       void *stream=NULL);
-  // dtype FFTWPP_AMD_F64/F32 on complex128/complex64 device arrays; mult >= 0
-  // overrides the Application's multiplier (the adjoint uses multcorrelation).
-  void execDevice(void *f0, const void *f1, bool normalize, int dtype,
-                  void *stream, int mult=-1);
+  // out <- f0 (*) f1 for dtype FFTWPP_AMD_F64/F32 on complex128/complex64
+  // device arrays (out may equal f0); mult >= 0 overrides the Application's
+  // multiplier (the adjoint uses multcorrelation).
+  void execDevice(void *out, const void *f0, const void *f1, bool normalize,
+                  int dtype, void *stream, int mult=-1);
   size_t arraySize() const;
+
+  Timeline *timeline=NULL;  // when set, every launch is timed
 
 protected:
   int multCode;
@@ -390,11 +413,14 @@ public:
 
   void convolveDevice(Complex *f0, const Complex *f1, bool normalize,
                       void *stream=NULL);
-  // dtype FFTWPP_AMD_F64/F32 on complex128/complex64 device arrays; mult >= 0
-  // overrides the Application's multiplier (the adjoint uses multcorrelation).
-  void execDevice(void *f0, const void *f1, bool normalize, int dtype,
-                  void *stream, int mult=-1);
+  // out <- f0 (*) f1 for dtype FFTWPP_AMD_F64/F32 on complex128/complex64
+  // device arrays (out may equal f0); mult >= 0 overrides the Application's
+  // multiplier (the adjoint uses multcorrelation).
+  void execDevice(void *out, const void *f0, const void *f1, bool normalize,
+                  int dtype, void *stream, int mult=-1);
   size_t arraySize() const;
+
+  Timeline *timeline=NULL;  // when set, every launch is timed
 
 protected:
   int multCode;

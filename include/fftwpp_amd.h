@@ -125,6 +125,12 @@ int fftwpp_amd_synchronize(void *stream);
 int fftwpp_amd_is_device_pointer(const void *p);
 int fftwpp_amd_device_count(void);
 
+/* HIP events (timing of launches on a stream). */
+int fftwpp_amd_event_create(void **event);
+int fftwpp_amd_event_record(void *event, void *stream);
+int fftwpp_amd_event_elapsed(float *ms, void *start, void *stop);
+int fftwpp_amd_event_destroy(void *event);
+
 /* ---------------------------------------------------------------------- */
 /* 2. Convolution layer                                                    */
 /* ---------------------------------------------------------------------- */
@@ -158,6 +164,19 @@ int fftwpp_amd_conv_execute(fftwpp_amd_conv *h, void *a, const void *b,
 int fftwpp_amd_conv_adjoint(fftwpp_amd_conv *h, const void *w, const void *a,
                             const void *b, void *grad_a, void *grad_b,
                             void *stream);
+
+/* Out-of-place: out <- a (*) b; a and b are only read (out may equal a). */
+int fftwpp_amd_conv_execute_out(fftwpp_amd_conv *h, void *out, const void *a,
+                                const void *b, void *stream);
+
+/* conv_execute_out with a HIP event recorded after every kernel launch of
+ * the plan: stage_ms[i] is the duration of stage i (waits for completion).
+ * fftwpp_amd_conv_stage_name names stage i of the last timed execution. */
+int fftwpp_amd_conv_execute_timed(fftwpp_amd_conv *h, void *out,
+                                  const void *a, const void *b, void *stream,
+                                  float *stage_ms, int max_stages,
+                                  int *nstages);
+const char *fftwpp_amd_conv_stage_name(const fftwpp_amd_conv *h, int i);
 
 /* Padded sizes N[d] = m*q actually used and the sub-FFT sizes m[d], q[d]. */
 int fftwpp_amd_conv_geometry(const fftwpp_amd_conv *h, size_t *N,

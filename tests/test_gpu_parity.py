@@ -390,3 +390,28 @@ def test_conv3d_256_full_size_sampled_direct(gpu, oracle):
     err = oracle.rel_l2(h32.cpu().numpy(), H)
     print(f"fp32 256^3 rel L2 vs FP64: {err:.3e}", flush=True)
     assert err < TOL32
+
+
+# ---------------------------------------------------------------- m=256 kernels
+
+@pytest.mark.parametrize("L,M", [(256, 511), (200, 399), (256, 700), (131, 600)])
+def test_register_fft256_paths(gpu, oracle, L, M):
+    """Sub-FFT size 256 (register-resident kernels) with L < 256 and q = 2..3,
+    as rows (1D), and as strided columns (2D/3D outer dimensions)."""
+    import torch
+    rng = np.random.default_rng(L + M)
+    _, _, q = oracle.parameters(L, M, 256, False)
+    N = 256 * q
+    f, g = rnd(rng, L), rnd(rng, L)
+    for dtype, tol in [(torch.complex128, TOL64), (torch.complex64, TOL32)]:
+        h, plan = run(gpu, "complex", f, g, M=[M], m=[256], dtype=dtype)
+        assert plan.geometry()[1][0] == 256
+        assert oracle.rel_l2(h, oracle.explicit_conv(f, g, N)) < tol
+    f2, g2 = rnd(rng, L, 40), rnd(rng, L, 40)
+    h2, _ = run(gpu, "complex", f2, g2, M=[M, 79], m=[256, 0])
+    assert oracle.rel_l2(h2, oracle.explicit_conv(f2, g2, (N, 79))) < TOL64
+    f3, g3 = rnd(rng, 6, L, 9), rnd(rng, 6, L, 9)
+    h3, _ = run(gpu, "complex", f3, g3, M=[11, M, 17], m=[0, 256, 0])
+    _, _, q3 = oracle.parameters(6, 11, oracle.parameters(6, 11, 6, False)[0] and 6, False)
+    want = oracle.explicit_conv(f3, g3, (11, N, 17))
+    assert oracle.rel_l2(h3, want) < TOL64
