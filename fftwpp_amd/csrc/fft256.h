@@ -65,25 +65,69 @@ __device__ __forceinline__ void dft16(cplx<T> *v)
   for(int k=0; k < 16; ++k) v[k]=y[k];
 }
 
+// Synchronisation of the lanes that share an LDS row: the whole block
+// (block barrier) or, when the 16 lanes of a sequence sit in one wave, the
+// wave (LDS operations of a wave complete in order; the fences keep the
+// compiler from moving LDS accesses across the exchange).
+template<int WAVE>
+__device__ __forceinline__ void lanes_sync()
+{
+  if(WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE,"wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE,"wavefront");
+  } else
+    __syncthreads();
+}
+
 // 256-point transform with kernel e^{S 2 pi i jk/256} of the sequence held by
-// the 16 lanes j of a group; `row` is the group's LDS area (16*17 entries).
-// Every thread of the block must call this (it contains block barriers).
-template<typename T, int S>
+// the 16 lanes j of a group; `row` is the group's LDS area (16*17 complex
+// words).  Stage twiddles w_256^n come from `tw` with tw[n*ts] = w_256^n.
+// With WAVE=0 every thread of the block must call this (block barriers).
+template<typename T, int S, int WAVE>
 __device__ __forceinline__ void fft256(cplx<T> *v, cplx<T> *row, int j,
-                                       const cplx<T> *zN, int64_t q)
+                                       const cplx<T> *tw, int ts)
 {
   dft16<T,S>(v);
   // stage-1 output element e = 16 j + u  (stored at j*17 + u)
 #pragma unroll
   for(int u=0; u < 16; ++u) row[j*17+u]=v[u];
-  __syncthreads();
+  lanes_sync<WAVE>();
   // stage-2 input element j + 16 t (stored at t*17 + j)
 #pragma unroll
   for(int t=0; t < 16; ++t) v[t]=row[t*17+j];
-  __syncthreads();
+  lanes_sync<WAVE>();
 #pragma unroll
   for(int t=1; t < 16; ++t) {
-    cplx<T> w=zN[(t*j)*(int) q];
+    cplx<T> w=tw[(t*j)*ts];
+    v[t]=S > 0 ? v[t]*w : mulc(v[t],w);
+  }
+  dft16<T,S>(v);
+}
+
+// The same transform exchanging real and imaginary parts in two rounds
+// through a scalar LDS row of 16*17 words: half the LDS per sequence and
+// conflict-free 8-byte accesses for FP64.
+template<typename T, int S, int WAVE>
+__device__ __forceinline__ void fft256s(cplx<T> *v, T *row, int j,
+                                        const cplx<T> *tw, int ts)
+{
+  dft16<T,S>(v);
+#pragma unroll
+  for(int u=0; u < 16; ++u) row[j*17+u]=v[u].x;
+  lanes_sync<WAVE>();
+#pragma unroll
+  for(int t=0; t < 16; ++t) v[t].x=row[t*17+j];
+  lanes_sync<WAVE>();
+#pragma unroll
+  for(int u=0; u < 16; ++u) row[j*17+u]=v[u].y;
+  lanes_sync<WAVE>();
+#pragma unroll
+  for(int t=0; t < 16; ++t) v[t].y=row[t*17+j];
+  lanes_sync<WAVE>();
+#pragma unroll
+  for(int t=1; t < 16; ++t) {
+    cplx<T> w=tw[(t*j)*ts];
     v[t]=S > 0 ? v[t]*w : mulc(v[t],w);
   }
   dft16<T,S>(v);

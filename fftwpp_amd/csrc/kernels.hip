@@ -423,14 +423,14 @@ static bool fast256(const fftwpp_amd_dim *d, size_t s1=0, size_t s2=0)
   // lane offsets (at most 16 strides plus a tile width) must fit in 31 bits
   const size_t lim=(size_t) 1 << 30;
   return d->m == 256 && d->kind == FFTWPP_AMD_KIND_COMPLEX && d->L <= 256 &&
-    16*s1+64 < lim && 16*s2+64 < lim;
+    d->q <= 8 && 16*s1+64 < lim && 16*s2+64 < lim;
 }
 
 template<typename T>
 static constexpr int cols256() { return sizeof(T) == 8 ? 8 : 16; }
 
 template<typename T>
-static constexpr int rows256() { return sizeof(T) == 8 ? 4 : 8; }
+static constexpr int rows256() { return sizeof(T) == 8 ? 8 : 16; }
 
 template<typename T>
 static int fwd_many_t(const fftwpp_amd_dim *d, const void *f, size_t S,
@@ -442,7 +442,10 @@ static int fwd_many_t(const fftwpp_amd_dim *d, const void *f, size_t S,
   if(fast256(d,S,SF)) {
     constexpr int TC=cols256<T>();
     dim3 grid((unsigned) ((C+TC-1)/TC),(unsigned) nbatch);
-    hipLaunchKernelGGL((colfwd256_kernel<T,TC>),grid,dim3(16*TC),0,st,
+    size_t lds=col256_lds<T>(TC,d->q);
+    int rc=allow_lds(colfwd256_kernel<T,TC>,lds);
+    if(rc) return rc;
+    hipLaunchKernelGGL((colfwd256_kernel<T,TC>),grid,dim3(16*TC),lds,st,
                        (const cplx<T> *) f,(int64_t) S,(int64_t) C,(int64_t) bsi,
                        (cplx<T> *) F,(int64_t) SF,(int64_t) bso,(int) d->L,
                        (int64_t) r0,(int64_t) nres,(const cplx<T> *) a.zN,(int64_t) d->q);
@@ -474,7 +477,10 @@ static int bwd_many_t(const fftwpp_amd_dim *d, const void *F, size_t SF,
   if(fast256(d,S,SF)) {
     constexpr int TC=cols256<T>();
     dim3 grid((unsigned) ((C+TC-1)/TC),(unsigned) nbatch);
-    hipLaunchKernelGGL((colbwd256_kernel<T,TC>),grid,dim3(16*TC),0,st,
+    size_t lds=col256_lds<T>(TC,d->q);
+    int rc=allow_lds(colbwd256_kernel<T,TC>,lds);
+    if(rc) return rc;
+    hipLaunchKernelGGL((colbwd256_kernel<T,TC>),grid,dim3(16*TC),lds,st,
                        (const cplx<T> *) F,(int64_t) SF,(int64_t) bsi,(cplx<T> *) f,
                        (int64_t) S,(int64_t) C,(int64_t) bso,(int) d->L,(int64_t) r0,
                        (int64_t) nres,accumulate,(T) scale,(const cplx<T> *) a.zN,
@@ -507,18 +513,24 @@ static int conv_rows_t(const fftwpp_amd_dim *d, int mult, const void *f,
     constexpr int NR=rows256<T>();
     dim3 grid((unsigned) ((rows+NR-1)/NR));
     const cplx<T> *zt=(const cplx<T> *) a.zN;
-    if(mult == FFTWPP_AMD_MULT_CORRELATION)
-      hipLaunchKernelGGL((rowconv256_kernel<T,3,NR>),grid,dim3(16*NR),0,st,
+    size_t lds=256*d->q*sizeof(cplx<T>)+NR*16*17*sizeof(T);
+    int rc;
+    if(mult == FFTWPP_AMD_MULT_CORRELATION) {
+      if((rc=allow_lds(rowconv256_kernel<T,3,NR>,lds))) return rc;
+      hipLaunchKernelGGL((rowconv256_kernel<T,3,NR>),grid,dim3(16*NR),lds,st,
                          (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
                          (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt,(int64_t) d->q);
-    else if(mult == FFTWPP_AMD_MULT_NONE)
-      hipLaunchKernelGGL((rowconv256_kernel<T,0,NR>),grid,dim3(16*NR),0,st,
+    } else if(mult == FFTWPP_AMD_MULT_NONE) {
+      if((rc=allow_lds(rowconv256_kernel<T,0,NR>,lds))) return rc;
+      hipLaunchKernelGGL((rowconv256_kernel<T,0,NR>),grid,dim3(16*NR),lds,st,
                          (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
                          (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt,(int64_t) d->q);
-    else
-      hipLaunchKernelGGL((rowconv256_kernel<T,1,NR>),grid,dim3(16*NR),0,st,
+    } else {
+      if((rc=allow_lds(rowconv256_kernel<T,1,NR>,lds))) return rc;
+      hipLaunchKernelGGL((rowconv256_kernel<T,1,NR>),grid,dim3(16*NR),lds,st,
                          (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
                          (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt,(int64_t) d->q);
+    }
     return launch_check("rowconv256_kernel");
   }
   size_t elt=sizeof(cplx<T>);
