@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run a 512 MiB device-to-device copy first (the known-size "
+                         "reference for HBM traffic counters)")
     return ap.parse_args()
 
 
@@ -150,6 +153,13 @@ def main():
     L = args.size
     L3 = L ** 3
     dev = torch.device("cuda:0")
+    if args.calibrate:
+        src = torch.ones(64 << 20, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     f, g = inputs(L)
 

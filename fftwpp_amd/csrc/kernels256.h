@@ -15,6 +15,13 @@ namespace fftwpp_amd {
 template<typename T>
 __device__ __forceinline__ cplx<T> zero() { return mk((T) 0,(T) 0); }
 
+__device__ __forceinline__ void wsync_rows()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE,"wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE,"wavefront");
+}
+
 // Dynamic LDS bytes of the column kernels for TC columns and q residues.
 template<typename T>
 inline size_t col256_lds(int TC, size_t q)
