@@ -4,8 +4,8 @@
 // v[k], k < 16 (the same natural layout before and after the transform).
 // 256 = 16 x 16 Stockham: a radix-16 DFT in registers, one transpose through
 // LDS (rows padded to 17 entries), a twiddle by w_256^{t j} and a second
-// radix-16 DFT.  Twiddles are read from the dimension's FP64-generated table
-// zN (N = 256 q), w_256^n = zN[n q].
+// radix-16 DFT.  Twiddles are copies of the dimension's FP64-generated table
+// zN (N = 256 q), w_256^n = zN[n q], staged in LDS.
 #pragma once
 #include "common.h"
 
@@ -81,56 +81,61 @@ __device__ __forceinline__ void lanes_sync()
 }
 
 // 256-point transform with kernel e^{S 2 pi i jk/256} of the sequence held by
-// the 16 lanes j of a group; `row` is the group's LDS area (16*17 complex
-// words).  Stage twiddles w_256^n come from `tw` with tw[n*ts] = w_256^n.
-// With WAVE=0 every thread of the block must call this (block barriers).
+// the 16 lanes j of a group, exchanging real and imaginary parts in two
+// rounds through the group's scalar LDS row area `row` (at least 16*17
+// words).  Lane j writes its stage-1 output u to row[17 j + (u ^ h)] and
+// reads stage-2 input t from row[17 t + (j ^ h)]: the swizzle h (0 or 2, the
+// same for the 16 lanes of a group) lets groups whose areas start on the same
+// bank use disjoint banks.  Stage twiddles w_256^{t j} come from the 16 x 16
+// table tw16[16 t + j], so the 16 lanes read consecutive words.  With WAVE=0
+// every thread of the block must call this (block barriers).
 template<typename T, int S, int WAVE>
-__device__ __forceinline__ void fft256(cplx<T> *v, cplx<T> *row, int j,
-                                       const cplx<T> *tw, int ts)
+__device__ __forceinline__ void fft256s(cplx<T> *v, T *row, int j, int h,
+                                        const cplx<T> *tw16)
 {
   dft16<T,S>(v);
-  // stage-1 output element e = 16 j + u  (stored at j*17 + u)
+  // u ^ h = u + h when bit 1 of u is clear, u - h when it is set (h = 0, 2)
+  T *wa=row+17*j+h, *wb=row+17*j-h;
+  const T *ra=row+(j ^ h);
 #pragma unroll
-  for(int u=0; u < 16; ++u) row[j*17+u]=v[u];
+  for(int u=0; u < 16; ++u) ((u & 2) ? wb : wa)[u]=v[u].x;
   lanes_sync<WAVE>();
-  // stage-2 input element j + 16 t (stored at t*17 + j)
 #pragma unroll
-  for(int t=0; t < 16; ++t) v[t]=row[t*17+j];
+  for(int t=0; t < 16; ++t) v[t].x=ra[17*t];
+  lanes_sync<WAVE>();
+#pragma unroll
+  for(int u=0; u < 16; ++u) ((u & 2) ? wb : wa)[u]=v[u].y;
+  lanes_sync<WAVE>();
+#pragma unroll
+  for(int t=0; t < 16; ++t) v[t].y=ra[17*t];
   lanes_sync<WAVE>();
 #pragma unroll
   for(int t=1; t < 16; ++t) {
-    cplx<T> w=tw[(t*j)*ts];
+    cplx<T> w=(tw16+16*t)[j];
     v[t]=S > 0 ? v[t]*w : mulc(v[t],w);
   }
   dft16<T,S>(v);
 }
 
-// The same transform exchanging real and imaginary parts in two rounds
-// through a scalar LDS row of 16*17 words: half the LDS per sequence and
-// conflict-free 8-byte accesses for FP64.
-template<typename T, int S, int WAVE>
-__device__ __forceinline__ void fft256s(cplx<T> *v, T *row, int j,
-                                        const cplx<T> *tw, int ts)
+// The 16 x 16 stage-twiddle table tw16[16 t + j] = w_256^{t j} from the
+// dimension's table zN (N = 256 q, zN[n] = w_N^n): w_256^n = zN[q n].
+// Cooperative fill by `nthreads` threads of the block.
+template<typename T>
+__device__ __forceinline__ void fill_tw16(cplx<T> *tw16, const cplx<T> *zN,
+                                          int q, int tid, int nthreads)
 {
-  dft16<T,S>(v);
-#pragma unroll
-  for(int u=0; u < 16; ++u) row[j*17+u]=v[u].x;
-  lanes_sync<WAVE>();
-#pragma unroll
-  for(int t=0; t < 16; ++t) v[t].x=row[t*17+j];
-  lanes_sync<WAVE>();
-#pragma unroll
-  for(int u=0; u < 16; ++u) row[j*17+u]=v[u].y;
-  lanes_sync<WAVE>();
-#pragma unroll
-  for(int t=0; t < 16; ++t) v[t].y=row[t*17+j];
-  lanes_sync<WAVE>();
-#pragma unroll
-  for(int t=1; t < 16; ++t) {
-    cplx<T> w=tw[(t*j)*ts];
-    v[t]=S > 0 ? v[t]*w : mulc(v[t],w);
-  }
-  dft16<T,S>(v);
+  for(int n=tid; n < 256; n += nthreads)
+    tw16[n]=zN[q*((n >> 4)*(n & 15))];
+}
+
+// Residue twiddles rtab[256 rr + n] = w_N^{(r0 + rr) n}, n < 256, rr < nres.
+template<typename T>
+__device__ __forceinline__ void fill_rtab(cplx<T> *rtab, const cplx<T> *zN,
+                                          int r0, int nres, int tid,
+                                          int nthreads)
+{
+  for(int e=tid; e < 256*nres; e += nthreads)
+    rtab[e]=zN[(r0+(e >> 8))*(e & 255)];
 }
 
 } // namespace fftwpp_amd

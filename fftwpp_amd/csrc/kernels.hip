@@ -427,9 +427,6 @@ static bool fast256(const fftwpp_amd_dim *d, size_t s1=0, size_t s2=0)
 }
 
 template<typename T>
-static constexpr int cols256() { return sizeof(T) == 8 ? 8 : 16; }
-
-template<typename T>
 static constexpr int rows256() { return sizeof(T) == 8 ? 8 : 16; }
 
 template<typename T>
@@ -440,12 +437,12 @@ static int fwd_many_t(const fftwpp_amd_dim *d, const void *f, size_t S,
 {
   const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
   if(fast256(d,S,SF)) {
-    constexpr int TC=cols256<T>();
+    constexpr int TC=Col256<T>::TC;
     dim3 grid((unsigned) ((C+TC-1)/TC),(unsigned) nbatch);
-    size_t lds=col256_lds<T>(TC,d->q);
-    int rc=allow_lds(colfwd256_kernel<T,TC>,lds);
+    size_t lds=col256_lds<T>(nres);
+    int rc=allow_lds(colfwd256_kernel<T>,lds);
     if(rc) return rc;
-    hipLaunchKernelGGL((colfwd256_kernel<T,TC>),grid,dim3(16*TC),lds,st,
+    hipLaunchKernelGGL((colfwd256_kernel<T>),grid,dim3(16*TC),lds,st,
                        (const cplx<T> *) f,(int64_t) S,(int64_t) C,(int64_t) bsi,
                        (cplx<T> *) F,(int64_t) SF,(int64_t) bso,(int) d->L,
                        (int64_t) r0,(int64_t) nres,(const cplx<T> *) a.zN,(int64_t) d->q);
@@ -475,12 +472,12 @@ static int bwd_many_t(const fftwpp_amd_dim *d, const void *F, size_t SF,
 {
   const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
   if(fast256(d,S,SF)) {
-    constexpr int TC=cols256<T>();
+    constexpr int TC=Col256<T>::TC;
     dim3 grid((unsigned) ((C+TC-1)/TC),(unsigned) nbatch);
-    size_t lds=col256_lds<T>(TC,d->q);
-    int rc=allow_lds(colbwd256_kernel<T,TC>,lds);
+    size_t lds=col256_lds<T>(nres);
+    int rc=allow_lds(colbwd256_kernel<T>,lds);
     if(rc) return rc;
-    hipLaunchKernelGGL((colbwd256_kernel<T,TC>),grid,dim3(16*TC),lds,st,
+    hipLaunchKernelGGL((colbwd256_kernel<T>),grid,dim3(16*TC),lds,st,
                        (const cplx<T> *) F,(int64_t) SF,(int64_t) bsi,(cplx<T> *) f,
                        (int64_t) S,(int64_t) C,(int64_t) bso,(int) d->L,(int64_t) r0,
                        (int64_t) nres,accumulate,(T) scale,(const cplx<T> *) a.zN,
@@ -513,7 +510,7 @@ static int conv_rows_t(const fftwpp_amd_dim *d, int mult, const void *f,
     constexpr int NR=rows256<T>();
     dim3 grid((unsigned) ((rows+NR-1)/NR));
     const cplx<T> *zt=(const cplx<T> *) a.zN;
-    size_t lds=256*d->q*sizeof(cplx<T>)+NR*16*17*sizeof(T);
+    size_t lds=row256_lds<T>(NR,d->q);
     int rc;
     if(mult == FFTWPP_AMD_MULT_CORRELATION) {
       if((rc=allow_lds(rowconv256_kernel<T,3,NR>,lds))) return rc;

@@ -22,42 +22,58 @@ __device__ __forceinline__ void wsync_rows()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE,"wavefront");
 }
 
-// Dynamic LDS bytes of the column kernels for TC columns and q residues.
+// Column kernels: lane (c, j) = (threadIdx.x % TC, threadIdx.x / TC), so
+// consecutive lanes read consecutive columns; the 16 lanes j of column c
+// share an exchange area of CS scalar words (block barriers, since they span
+// waves).  TC, CS and the swizzle h(c) make every LDS exchange access
+// bank-conflict free on gfx950 (64-bit writes are served 16 lanes per pass
+// over 32 banks, 64-bit reads 32 lanes per pass over 64 banks, 32-bit
+// accesses 32 lanes per pass over 32 banks): FP64 uses 8 columns, areas 276
+// words apart and h = 2 for columns 4..7; FP32 uses 16 columns, areas 274
+// words apart and no swizzle.
+template<typename T> struct Col256 {
+  static constexpr int TC=sizeof(T) == 8 ? 8 : 16;
+  static constexpr int CS=sizeof(T) == 8 ? 276 : 274;
+  __device__ static int swz(int c) { return sizeof(T) == 8 ? ((c >> 1) & 2) : 0; }
+};
+
+// Dynamic LDS bytes of the column kernels for nres residues: residue
+// twiddles (256 nres complex), stage twiddles (256 complex), exchange areas.
 template<typename T>
-inline size_t col256_lds(int TC, size_t q)
+inline size_t col256_lds(size_t nres)
 {
-  return sizeof(T) == 4 ? 256*q*sizeof(cplx<T>)+TC*16*17*sizeof(T)
-    : TC*16*17*sizeof(cplx<T>);
+  return (256*nres+256)*sizeof(cplx<T>)+Col256<T>::TC*Col256<T>::CS*sizeof(T);
 }
 
-// Columns: lane (c, j) = (threadIdx.x % TC, threadIdx.x / TC) so consecutive
-// lanes read consecutive columns; the 16 lanes j of column c share an LDS row
-// (block barriers, since they span waves).  FP32: the block copies the
-// twiddle table zN (256 q entries) into LDS and exchanges real and imaginary
-// parts separately; FP64: twiddles come from the global table and complex
-// words are exchanged (fewer barriers).  Dynamic LDS: see col256_lds().
-// Column transforms for one block; `tab` holds the twiddles (LDS or global,
-// fixed per instantiation) and `row` is this column's exchange area.
-template<typename T, int TC, bool SPLIT, typename TAB>
+// Dynamic LDS bytes of the row kernel with NR rows per block and q residues.
+template<typename T>
+inline size_t row256_lds(int NR, size_t q)
+{
+  return (256*q+256)*sizeof(cplx<T>)+NR*16*17*sizeof(T);
+}
+
+// Forward residue transforms of one column: for each residue r = r0 + rr,
+// F_r(s) = DFT_256[w_N^{r k} f(k)](s), written at Fb + rr*256*SF.
+template<typename T>
 __device__ __forceinline__ void colfwd_body(const cplx<T> *fb, cplx<T> *Fb,
                                             int64_t S, int64_t SF, bool ok,
                                             int c, int j, int L, int64_t r0,
-                                            int64_t nres, TAB tab, int qi,
-                                            void *row)
+                                            int64_t nres, const cplx<T> *rtab,
+                                            const cplx<T> *tw16, T *row, int h)
 {
   const unsigned lin=(unsigned) (c+j*S);
   const unsigned lout=(unsigned) (c+j*SF);
   for(int64_t rr=0; rr < nres; ++rr) {
-    const int r=(int) (r0+rr);
+    const int64_t r=r0+rr;
+    const cplx<T> *rt=rtab+256*rr;
     cplx<T> v[16];
 #pragma unroll
     for(int k=0; k < 16; ++k) {
       const cplx<T> *bk=fb+(int64_t) 16*k*S;
       cplx<T> x=(ok && j+16*k < L) ? bk[lin] : zero<T>();
-      v[k]=r == 0 ? x : x*(tab+16*k*r)[j*r];
+      v[k]=r == 0 ? x : x*(rt+16*k)[j];
     }
-    if constexpr(SPLIT) fft256s<T,1,0>(v,(T *) row,j,tab,qi);
-    else fft256<T,1,0>(v,(cplx<T> *) row,j,tab,qi);
+    fft256s<T,1,0>(v,row,j,h,tw16);
     cplx<T> *Fr=Fb+rr*256*SF;
     if(ok) {
 #pragma unroll
@@ -66,13 +82,15 @@ __device__ __forceinline__ void colfwd_body(const cplx<T> *fb, cplx<T> *Fb,
   }
 }
 
-template<typename T, int TC, bool SPLIT, typename TAB>
+// Backward residue transforms of one column, accumulated over the residues:
+// f(k) = scale * sum_rr w_N^{-r k} DFT_256^{-1}[F_r](k) (+ f(k) if accumulate).
+template<typename T>
 __device__ __forceinline__ void colbwd_body(const cplx<T> *Fb, cplx<T> *fb,
                                             int64_t S, int64_t SF, bool ok,
                                             int c, int j, int L, int64_t r0,
                                             int64_t nres, int accumulate,
-                                            T scale, TAB tab, int qi,
-                                            void *row)
+                                            T scale, const cplx<T> *rtab,
+                                            const cplx<T> *tw16, T *row, int h)
 {
   const unsigned lin=(unsigned) (c+j*SF);
   const unsigned lout=(unsigned) (c+j*S);
@@ -80,17 +98,17 @@ __device__ __forceinline__ void colbwd_body(const cplx<T> *Fb, cplx<T> *fb,
 #pragma unroll
   for(int k=0; k < 16; ++k) acc[k]=zero<T>();
   for(int64_t rr=0; rr < nres; ++rr) {
-    const int r=(int) (r0+rr);
+    const int64_t r=r0+rr;
+    const cplx<T> *rt=rtab+256*rr;
     const cplx<T> *Fr=Fb+rr*256*SF;
     cplx<T> v[16];
 #pragma unroll
     for(int u=0; u < 16; ++u)
       v[u]=ok ? (Fr+(int64_t) 16*u*SF)[lin] : zero<T>();
-    if constexpr(SPLIT) fft256s<T,-1,0>(v,(T *) row,j,tab,qi);
-    else fft256<T,-1,0>(v,(cplx<T> *) row,j,tab,qi);
+    fft256s<T,-1,0>(v,row,j,h,tw16);
 #pragma unroll
     for(int k=0; k < 16; ++k)
-      acc[k]=acc[k]+(r == 0 ? v[k] : mulc(v[k],(tab+16*k*r)[j*r]));
+      acc[k]=acc[k]+(r == 0 ? v[k] : mulc(v[k],(rt+16*k)[j]));
   }
   if(ok) {
 #pragma unroll
@@ -105,62 +123,57 @@ __device__ __forceinline__ void colbwd_body(const cplx<T> *Fb, cplx<T> *fb,
   }
 }
 
-template<typename T, int TC>
-__global__ void __launch_bounds__(16*TC,sizeof(T) == 4 ? 3 : 2)
+template<typename T>
+__global__ void __launch_bounds__(16*Col256<T>::TC,sizeof(T) == 4 ? 3 : 2)
 colfwd256_kernel(const cplx<T> *f, int64_t S, int64_t C, int64_t bsi,
                  cplx<T> *F, int64_t SF, int64_t bso, int L, int64_t r0,
                  int64_t nres, const cplx<T> *zN, int64_t q)
 {
+  constexpr int TC=Col256<T>::TC;
   extern __shared__ char smem_raw[];
-  const int qi=(int) q;
+  cplx<T> *rtab=reinterpret_cast<cplx<T> *>(smem_raw);
+  cplx<T> *tw16=rtab+256*nres;
+  fill_rtab(rtab,zN,(int) r0,(int) nres,threadIdx.x,16*TC);
+  fill_tw16(tw16,zN,(int) q,threadIdx.x,16*TC);
+  __syncthreads();
   const int c=threadIdx.x % TC, j=threadIdx.x/TC;
+  T *row=reinterpret_cast<T *>(tw16+256)+c*Col256<T>::CS;
   const int64_t c0=(int64_t) blockIdx.x*TC;
   const bool ok=c0+c < C;
   const cplx<T> *fb=f+(int64_t) blockIdx.y*bsi+c0;
   cplx<T> *Fb=F+(int64_t) blockIdx.y*bso+c0;
-  if constexpr(sizeof(T) == 4) {
-    cplx<T> *tab=reinterpret_cast<cplx<T> *>(smem_raw);
-    for(int n=threadIdx.x; n < 256*qi; n += 16*TC) tab[n]=zN[n];
-    __syncthreads();
-    T *row=reinterpret_cast<T *>(tab+256*qi)+c*16*17;
-    colfwd_body<T,TC,true>(fb,Fb,S,SF,ok,c,j,L,r0,nres,(const cplx<T> *) tab,qi,row);
-  } else {
-    cplx<T> *row=reinterpret_cast<cplx<T> *>(smem_raw)+c*16*17;
-    colfwd_body<T,TC,false>(fb,Fb,S,SF,ok,c,j,L,r0,nres,zN,qi,row);
-  }
+  colfwd_body<T>(fb,Fb,S,SF,ok,c,j,L,r0,nres,rtab,tw16,row,Col256<T>::swz(c));
 }
 
-template<typename T, int TC>
-__global__ void __launch_bounds__(16*TC,sizeof(T) == 4 ? 3 : 2)
+template<typename T>
+__global__ void __launch_bounds__(16*Col256<T>::TC,sizeof(T) == 4 ? 3 : 2)
 colbwd256_kernel(const cplx<T> *F, int64_t SF, int64_t bsi, cplx<T> *f,
                  int64_t S, int64_t C, int64_t bso, int L, int64_t r0,
                  int64_t nres, int accumulate, T scale, const cplx<T> *zN,
                  int64_t q)
 {
+  constexpr int TC=Col256<T>::TC;
   extern __shared__ char smem_raw[];
-  const int qi=(int) q;
+  cplx<T> *rtab=reinterpret_cast<cplx<T> *>(smem_raw);
+  cplx<T> *tw16=rtab+256*nres;
+  fill_rtab(rtab,zN,(int) r0,(int) nres,threadIdx.x,16*TC);
+  fill_tw16(tw16,zN,(int) q,threadIdx.x,16*TC);
+  __syncthreads();
   const int c=threadIdx.x % TC, j=threadIdx.x/TC;
+  T *row=reinterpret_cast<T *>(tw16+256)+c*Col256<T>::CS;
   const int64_t c0=(int64_t) blockIdx.x*TC;
   const bool ok=c0+c < C;
   const cplx<T> *Fb=F+(int64_t) blockIdx.y*bsi+c0;
   cplx<T> *fb=f+(int64_t) blockIdx.y*bso+c0;
-  if constexpr(sizeof(T) == 4) {
-    cplx<T> *tab=reinterpret_cast<cplx<T> *>(smem_raw);
-    for(int n=threadIdx.x; n < 256*qi; n += 16*TC) tab[n]=zN[n];
-    __syncthreads();
-    T *row=reinterpret_cast<T *>(tab+256*qi)+c*16*17;
-    colbwd_body<T,TC,true>(Fb,fb,S,SF,ok,c,j,L,r0,nres,accumulate,scale,
-                           (const cplx<T> *) tab,qi,row);
-  } else {
-    cplx<T> *row=reinterpret_cast<cplx<T> *>(smem_raw)+c*16*17;
-    colbwd_body<T,TC,false>(Fb,fb,S,SF,ok,c,j,L,r0,nres,accumulate,scale,zN,qi,row);
-  }
+  colbwd_body<T>(Fb,fb,S,SF,ok,c,j,L,r0,nres,accumulate,scale,rtab,tw16,row,
+                 Col256<T>::swz(c));
 }
 
 // Rows: NR rows per block of 16*NR lanes; lane j of row i = threadIdx.x % 16,
 // so the 16 lanes of a row share a wave and exchange through LDS without
-// block barriers.  The block first copies the twiddle table zN (N = 256 q
-// entries) into LDS.  Dynamic LDS: 256 q complex + NR*16*17 scalar words.
+// block barriers (row areas 16*17 words apart are conflict free).  The block
+// first stages the residue twiddles (256 q) and the stage twiddles (256) in
+// LDS.  Dynamic LDS: row256_lds().
 template<typename T, int MULT, int NR>
 __global__ void __launch_bounds__(16*NR,sizeof(T) == 4 ? 3 : 2)
 rowconv256_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows, int64_t rs,
@@ -168,10 +181,12 @@ rowconv256_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows, int64_t rs,
                   int64_t q)
 {
   extern __shared__ char smem_raw[];
-  cplx<T> *tab=reinterpret_cast<cplx<T> *>(smem_raw);   // 256 q twiddles
   const int qi=(int) q;
-  T *lds=reinterpret_cast<T *>(tab+256*qi);
-  for(int n=threadIdx.x; n < 256*qi; n += 16*NR) tab[n]=zN[n];
+  cplx<T> *rtab=reinterpret_cast<cplx<T> *>(smem_raw);   // 256 q
+  cplx<T> *tw16=rtab+256*qi;                              // 256
+  T *lds=reinterpret_cast<T *>(tw16+256);
+  fill_rtab(rtab,zN,0,qi,threadIdx.x,16*NR);
+  fill_tw16(tw16,zN,qi,threadIdx.x,16*NR);
   __syncthreads();
   const int i=threadIdx.x/16, j=threadIdx.x % 16;
   const int64_t row0=(int64_t) blockIdx.x*NR;
@@ -186,6 +201,7 @@ rowconv256_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows, int64_t rs,
 #pragma unroll
   for(int k=0; k < 16; ++k) acc[k]=zero<T>();
   for(int r=0; r < qi; ++r) {
+    const cplx<T> *rt=rtab+256*r;
     cplx<T> a[16], b[16];
 #pragma unroll
     for(int k=0; k < 16; ++k) {
@@ -193,24 +209,24 @@ rowconv256_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows, int64_t rs,
       cplx<T> x=in ? (fr+16*k)[lin] : zero<T>();
       cplx<T> y=in ? (gr+16*k)[lin] : zero<T>();
       if(r != 0) {
-        cplx<T> w=(tab+16*k*r)[j*r];
+        cplx<T> w=(rt+16*k)[j];
         x=x*w;
         y=y*w;
       }
       a[k]=x;
       b[k]=y;
     }
-    fft256s<T,1,1>(a,row,j,tab,qi);
-    fft256s<T,1,1>(b,row,j,tab,qi);
+    fft256s<T,1,1>(a,row,j,0,tw16);
+    fft256s<T,1,1>(b,row,j,0,tw16);
 #pragma unroll
     for(int k=0; k < 16; ++k) {
       if(MULT == 1) a[k]=a[k]*b[k];
       else if(MULT == 3) a[k]=mulc(a[k],b[k]);
     }
-    fft256s<T,-1,1>(a,row,j,tab,qi);
+    fft256s<T,-1,1>(a,row,j,0,tw16);
 #pragma unroll
     for(int k=0; k < 16; ++k)
-      acc[k]=acc[k]+(r == 0 ? a[k] : mulc(a[k],(tab+16*k*r)[j*r]));
+      acc[k]=acc[k]+(r == 0 ? a[k] : mulc(a[k],(rt+16*k)[j]));
   }
   if(ok) {
 #pragma unroll
