@@ -506,6 +506,31 @@ static int conv_rows_t(const fftwpp_amd_dim *d, int mult, const void *f,
                        size_t os, double scale, hipStream_t st)
 {
   const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
+  // q = 2: both residues of a row in one wave (each row read once)
+  if(fast256(d,rs,os) && d->q == 2) {
+    constexpr int WPB=4;
+    dim3 grid((unsigned) ((rows+2*WPB-1)/(2*WPB)));
+    const cplx<T> *zt=(const cplx<T> *) a.zN;
+    size_t lds=row256_lds<T>(4*WPB,2);
+    int rc;
+    if(mult == FFTWPP_AMD_MULT_CORRELATION) {
+      if((rc=allow_lds(rowconv256p_kernel<T,3,WPB>,lds))) return rc;
+      hipLaunchKernelGGL((rowconv256p_kernel<T,3,WPB>),grid,dim3(64*WPB),lds,st,
+                         (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
+                         (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt);
+    } else if(mult == FFTWPP_AMD_MULT_NONE) {
+      if((rc=allow_lds(rowconv256p_kernel<T,0,WPB>,lds))) return rc;
+      hipLaunchKernelGGL((rowconv256p_kernel<T,0,WPB>),grid,dim3(64*WPB),lds,st,
+                         (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
+                         (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt);
+    } else {
+      if((rc=allow_lds(rowconv256p_kernel<T,1,WPB>,lds))) return rc;
+      hipLaunchKernelGGL((rowconv256p_kernel<T,1,WPB>),grid,dim3(64*WPB),lds,st,
+                         (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
+                         (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt);
+    }
+    return launch_check("rowconv256p_kernel");
+  }
   if(fast256(d,rs,os)) {
     constexpr int NR=rows256<T>();
     dim3 grid((unsigned) ((rows+NR-1)/NR));

@@ -10,6 +10,7 @@
 #pragma once
 #include "fft256.h"
 
+
 namespace fftwpp_amd {
 
 template<typename T>
@@ -63,15 +64,20 @@ __device__ __forceinline__ void colfwd_body(const cplx<T> *fb, cplx<T> *Fb,
 {
   const unsigned lin=(unsigned) (c+j*S);
   const unsigned lout=(unsigned) (c+j*SF);
+  // the column is read once and kept in registers for all residues
+  cplx<T> x0[16];
+#pragma unroll
+  for(int k=0; k < 16; ++k) {
+    const cplx<T> *bk=fb+(int64_t) 16*k*S;
+    x0[k]=(ok && j+16*k < L) ? bk[lin] : zero<T>();
+  }
   for(int64_t rr=0; rr < nres; ++rr) {
     const int64_t r=r0+rr;
     const cplx<T> *rt=rtab+256*rr;
     cplx<T> v[16];
 #pragma unroll
     for(int k=0; k < 16; ++k) {
-      const cplx<T> *bk=fb+(int64_t) 16*k*S;
-      cplx<T> x=(ok && j+16*k < L) ? bk[lin] : zero<T>();
-      v[k]=r == 0 ? x : x*(rt+16*k)[j];
+      v[k]=r == 0 ? x0[k] : x0[k]*(rt+16*k)[j];
     }
     fft256s<T,1,0>(v,row,j,h,tw16);
     cplx<T> *Fr=Fb+rr*256*SF;
@@ -174,8 +180,15 @@ colbwd256_kernel(const cplx<T> *F, int64_t SF, int64_t bsi, cplx<T> *f,
 // block barriers (row areas 16*17 words apart are conflict free).  The block
 // first stages the residue twiddles (256 q) and the stage twiddles (256) in
 // LDS.  Dynamic LDS: row256_lds().
+// KEEP: the rows of f and g are read once and kept in registers for all
+// residues (FP32); otherwise each residue reads them again (FP64, where
+// keeping them would leave one wave per SIMD).
+template<typename T> struct Row256 {
+  static constexpr bool KEEP=sizeof(T) == 4;
+  static constexpr int MINW=2;
+};
 template<typename T, int MULT, int NR>
-__global__ void __launch_bounds__(16*NR,sizeof(T) == 4 ? 3 : 2)
+__global__ void __launch_bounds__(16*NR,Row256<T>::MINW)
 rowconv256_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows, int64_t rs,
                   cplx<T> *out, int64_t os, int L, T scale, const cplx<T> *zN,
                   int64_t q)
@@ -200,14 +213,30 @@ rowconv256_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows, int64_t rs,
   cplx<T> acc[16];
 #pragma unroll
   for(int k=0; k < 16; ++k) acc[k]=zero<T>();
+  constexpr bool KEEP=Row256<T>::KEEP;
+  cplx<T> a0[KEEP ? 16 : 1], b0[KEEP ? 16 : 1];
+  if constexpr(KEEP) {
+#pragma unroll
+    for(int k=0; k < 16; ++k) {
+      bool in=ok && j+16*k < L;
+      a0[k]=in ? (fr+16*k)[lin] : zero<T>();
+      b0[k]=in ? (gr+16*k)[lin] : zero<T>();
+    }
+  }
   for(int r=0; r < qi; ++r) {
     const cplx<T> *rt=rtab+256*r;
     cplx<T> a[16], b[16];
 #pragma unroll
     for(int k=0; k < 16; ++k) {
-      bool in=ok && j+16*k < L;
-      cplx<T> x=in ? (fr+16*k)[lin] : zero<T>();
-      cplx<T> y=in ? (gr+16*k)[lin] : zero<T>();
+      cplx<T> x, y;
+      if constexpr(KEEP) {
+        x=a0[k];
+        y=b0[k];
+      } else {
+        bool in=ok && j+16*k < L;
+        x=in ? (fr+16*k)[lin] : zero<T>();
+        y=in ? (gr+16*k)[lin] : zero<T>();
+      }
       if(r != 0) {
         cplx<T> w=(rt+16*k)[j];
         x=x*w;
@@ -232,6 +261,87 @@ rowconv256_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows, int64_t rs,
 #pragma unroll
     for(int k=0; k < 16; ++k)
       if(j+16*k < L) (orow+16*k)[lo]=scale*acc[k];
+  }
+}
+
+// Sum of a value over the two 16-lane groups of each 32-lane half of the
+// wave (lanes l and l ^ 16): v_permlane16_swap exchanges the odd 16-lane
+// rows of its first operand with the even rows of its second, so with both
+// operands v the two results hold v of the group and v of its partner.
+__device__ __forceinline__ float pair16_sum(float v)
+{
+  auto s=__builtin_amdgcn_permlane16_swap(__float_as_uint(v),__float_as_uint(v),
+                                          false,false);
+  return __uint_as_float(s[0])+__uint_as_float(s[1]);
+}
+__device__ __forceinline__ double pair16_sum(double v)
+{
+  const unsigned long long u=(unsigned long long) __double_as_longlong(v);
+  auto lo=__builtin_amdgcn_permlane16_swap((unsigned) u,(unsigned) u,false,false);
+  auto hi=__builtin_amdgcn_permlane16_swap((unsigned) (u >> 32),(unsigned) (u >> 32),
+                                           false,false);
+  const double a=__longlong_as_double((long long) (((unsigned long long) hi[0] << 32) | lo[0]));
+  const double b=__longlong_as_double((long long) (((unsigned long long) hi[1] << 32) | lo[1]));
+  return a+b;
+}
+
+// Rows with q = 2 (N = 512): a wave holds 2 rows x 2 residues x 16 lanes,
+// lane = j + 16 (2 i + r) for row i of the wave and residue r.  The two
+// residue groups of a row load the same elements in the same instruction, so
+// f and g are read from memory once; each group transforms its residue and
+// the contributions are summed across the group pair in registers.  Group r
+// then stores the outputs j + 16 k with k in [8 r, 8 r + 8).  Dynamic LDS:
+// row256_lds(4 WPB, 2).
+template<typename T, int MULT, int WPB>
+__global__ void __launch_bounds__(64*WPB,2)
+rowconv256p_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows,
+                   int64_t rs, cplx<T> *out, int64_t os, int L, T scale,
+                   const cplx<T> *zN)
+{
+  extern __shared__ char smem_raw[];
+  cplx<T> *rtab=reinterpret_cast<cplx<T> *>(smem_raw);   // 2 x 256
+  cplx<T> *tw16=rtab+512;                                 // 256
+  T *lds=reinterpret_cast<T *>(tw16+256);
+  fill_rtab(rtab,zN,0,2,threadIdx.x,64*WPB);
+  fill_tw16(tw16,zN,2,threadIdx.x,64*WPB);
+  __syncthreads();
+  const int j=threadIdx.x & 15;
+  const int grp=threadIdx.x >> 4;          // 4 groups per wave
+  const int r=grp & 1;
+  const int64_t row=(int64_t) blockIdx.x*(2*WPB)+(grp >> 1);
+  const bool ok=row < rows;
+  const cplx<T> *rt=rtab+256*r;
+  const cplx<T> *fr=f+(ok ? row : 0)*rs;
+  const cplx<T> *gr=g+(ok ? row : 0)*rs;
+  T *xrow=lds+grp*16*17;
+  cplx<T> a[16], b[16];
+#pragma unroll
+  for(int k=0; k < 16; ++k) {
+    bool in=ok && j+16*k < L;
+    cplx<T> x=in ? (fr+16*k)[j] : zero<T>();
+    cplx<T> y=in ? (gr+16*k)[j] : zero<T>();
+    cplx<T> w=(rt+16*k)[j];                 // w_512^{r (j + 16 k)}
+    a[k]=x*w;
+    b[k]=y*w;
+  }
+  fft256s<T,1,1>(a,xrow,j,0,tw16);
+  fft256s<T,1,1>(b,xrow,j,0,tw16);
+#pragma unroll
+  for(int k=0; k < 16; ++k) {
+    if(MULT == 1) a[k]=a[k]*b[k];
+    else if(MULT == 3) a[k]=mulc(a[k],b[k]);
+  }
+  fft256s<T,-1,1>(a,xrow,j,0,tw16);
+#pragma unroll
+  for(int k=0; k < 16; ++k) {
+    cplx<T> y=mulc(a[k],(rt+16*k)[j]);
+    a[k]=mk(pair16_sum(y.x),pair16_sum(y.y));
+  }
+  if(ok) {
+    cplx<T> *orow=out+row*os;
+#pragma unroll
+    for(int k=0; k < 16; ++k)
+      if((k >> 3) == r && j+16*k < L) (orow+16*k)[j]=scale*a[k];
   }
 }
 
