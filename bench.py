@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched, 1D and 2D Hermitian workloads")
     ap.add_argument("--calibrate", action="store_true",
                     help="run a 512 MiB device-to-device copy first (the known-size "
                          "reference for HBM traffic counters)")
@@ -140,6 +142,126 @@ def cpu_baseline(f, g, L):
                       f"{dt:.2f} s wall"}
 
 
+def batched_lengths(count=4096, lo=64, hi=16384, seed=SEED):
+    """L = floor(exp(U(ln lo, ln hi))), PCG64(seed)."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return np.floor(np.exp(rng.uniform(math.log(lo), math.log(hi), size=count))).astype(np.int64)
+
+
+def _time_calls(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def bench_batched(torch, dev, steps, warmup, per_call=True):
+    """4096 independent 1D convolutions, L log-uniform in [64,16384], M=2L:
+    one autotuned plan replayed as a hipGraph, the same plan without tuning
+    run eagerly, and one DeviceConvolution call per item."""
+    import numpy as np
+    from fftwpp_amd.batch import BatchedConvolution
+    from fftwpp_amd.torch_conv import DeviceConvolution
+    Ls = batched_lengths()
+    total = int(Ls.sum())
+    rng = np.random.Generator(np.random.PCG64(SEED + 4))
+    fh = rng.uniform(-1, 1, total) + 1j * rng.uniform(-1, 1, total)
+    gh = rng.uniform(-1, 1, total) + 1j * rng.uniform(-1, 1, total)
+    res = {"workload": "4096 1D complex convolutions, L log-uniform in [64,16384] "
+                       f"(sum L = {total}), M = 2L, multBinary, uniform[-1,1] seed {SEED + 4}"}
+    outs = {}
+    for dt, key, E in [(torch.complex128, "f64", 16), (torch.complex64, "f32", 8)]:
+        f = torch.tensor(fh, dtype=dt, device=dev)
+        g = torch.tensor(gh, dtype=dt, device=dev)
+        out = torch.empty_like(f)
+        t0 = time.perf_counter()
+        plan = BatchedConvolution(Ls, M=2 * Ls, dtype=dt, tune=True, graph=True)
+        tune_s = time.perf_counter() - t0
+        ms = _time_calls(torch, lambda: plan(f, g, out), steps, warmup)
+        outs[key] = out.clone()
+        eager = BatchedConvolution(Ls, M=2 * Ls, dtype=dt, tune=False, graph=False)
+        ms_e = _time_calls(torch, lambda: eager(f, g, out), steps, warmup)
+        bytes_alg = 3 * E * total
+        r = {"batches_per_s": round(1000.0 / ms, 2), "ms_per_batch": round(ms, 4),
+             "achieved_GBs": round(bytes_alg / (ms * 1e-3) / 1e9, 1),
+             "frac_of_hbm_peak": round(bytes_alg / (ms * 1e-3) / 1e9 / PEAK_GBS, 4),
+             "bytes_alg": bytes_alg, "tune_s": round(tune_s, 2),
+             "untuned_eager_ms_per_batch": round(ms_e, 4),
+             "classes": plan.describe().strip().splitlines()}
+        if per_call and key == "f64":
+            plans = [DeviceConvolution((int(L),), dtype=dt, M=[2 * int(L)]) for L in Ls]
+            off = plan.offsets
+            fv = [f[off[i]:off[i + 1]] for i in range(len(Ls))]
+            gv = [g[off[i]:off[i + 1]].clone() for i in range(len(Ls))]
+            ov = [torch.empty_like(v) for v in fv]
+
+            def calls():
+                for i in range(len(Ls)):
+                    plans[i].convolve_out(ov[i], fv[i], gv[i])
+            ms_c = _time_calls(torch, calls, 2, 1)
+            ref = torch.cat(ov)
+            r["per_call_ms_per_batch"] = round(ms_c, 3)
+            r["speedup_vs_per_call"] = round(ms_c / ms, 2)
+            r["rel_l2_vs_per_call"] = float(torch.linalg.norm(outs[key] - ref) / torch.linalg.norm(ref))
+            del plans
+        res[key] = r
+        del f, g, out
+        torch.cuda.empty_cache()
+    a, b = outs["f64"], outs["f32"].to(torch.complex128)
+    res["f32"]["rel_l2_vs_f64"] = float(torch.linalg.norm(a - b) / torch.linalg.norm(a))
+    return res
+
+
+def bench_conv1d(torch, dev, steps):
+    """One 1D convolution L=1024, M=2048, FP64: latency per call."""
+    import numpy as np
+    from fftwpp_amd.torch_conv import DeviceConvolution
+    rng = np.random.Generator(np.random.PCG64(SEED))
+    f = torch.tensor(rng.uniform(-1, 1, 1024) + 1j * rng.uniform(-1, 1, 1024), device=dev)
+    g = torch.tensor(rng.uniform(-1, 1, 1024) + 1j * rng.uniform(-1, 1, 1024), device=dev)
+    out = torch.empty_like(f)
+    plan = DeviceConvolution((1024,), M=[2048])
+    ms = _time_calls(torch, lambda: plan.convolve_out(out, f, g), 50 * steps, 10)
+    return {"workload": "1D complex L=1024, M=2048, FP64", "us_per_call": round(ms * 1e3, 2),
+            "conv_per_s": round(1000.0 / ms, 1), "geometry": [list(v) for v in plan.geometry()]}
+
+
+def bench_hconv2d(torch, dev, steps, warmup):
+    """2D centred Hermitian convolution 2047 x 1024 stored, M = 3070, FP64."""
+    import numpy as np
+    from fftwpp_amd import _lib
+    from fftwpp_amd.torch_conv import DeviceConvolution
+    Lx, Hy = 2047, 1024
+    rng = np.random.Generator(np.random.PCG64(SEED))
+    kx = (np.arange(Lx) - Lx // 2)[:, None]
+    ky = np.arange(Hy)[None, :]
+    k = np.sqrt(kx * kx + ky * ky)
+    amp = np.where(k > 0, 1.0 / np.where(k > 0, k, 1.0), 0.0)
+    ts = []
+    for _ in range(2):
+        t = torch.tensor(amp * np.exp(1j * rng.uniform(0, 2 * math.pi, (Lx, Hy))), device=dev)
+        _lib.lib().fftwpp_HermitianSymmetrizeX((Lx + 1) // 2, Hy, Lx // 2, t.data_ptr())
+        ts.append(t)
+    f, g = ts
+    out = torch.empty_like(f)
+    plan = DeviceConvolution((Lx, Hy), kind="hermitian", M=[3070, 3070])
+    ms = _time_calls(torch, lambda: plan.convolve_out(out, f, g), steps, warmup)
+    bytes_alg = 12 * 16 * Lx * Hy
+    return {"workload": "2D centred Hermitian 2047x1024 stored, M=3070, |k|^-1 random phase, FP64",
+            "conv_per_s": round(1000.0 / ms, 1), "ms_per_conv": round(ms, 4),
+            "achieved_GBs": round(bytes_alg / (ms * 1e-3) / 1e9, 1),
+            "frac_of_hbm_peak": round(bytes_alg / (ms * 1e-3) / 1e9 / PEAK_GBS, 4),
+            "geometry": [list(v) for v in plan.geometry()]}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -220,6 +342,11 @@ def main():
         result["rel_l2"]["fp32_vs_fp64"] = err32
         del a, b, out, plan
         torch.cuda.empty_cache()
+
+    if not args.no_extra:
+        result["batched"] = bench_batched(torch, dev, args.steps, args.warmup)
+        result["conv1d"] = bench_conv1d(torch, dev, args.steps)
+        result["hconv2d"] = bench_hconv2d(torch, dev, args.steps, args.warmup)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(f, g, L)

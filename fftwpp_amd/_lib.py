@@ -23,10 +23,13 @@ SIGNATURES = {
     "fftwpp_amd_forward_many": (I, [P, I, P, Z, Z, Z, Z, P, Z, Z, Z, Z, P]),
     "fftwpp_amd_backward_many": (I, [P, I, P, Z, Z, P, Z, Z, Z, Z, Z, Z, I, D, P]),
     "fftwpp_amd_conv_rows": (I, [P, I, I, P, P, Z, Z, P, Z, D, P]),
+    "fftwpp_amd_batch_rows_lds": (Z, [P, I, Z, Z]),
+    "fftwpp_amd_batch_rows": (I, [P, I, I, P, Z, P, P, P, Z, Z, Z, Z, D, P]),
     "fftwpp_amd_multiply": (I, [I, I, P, P, Z, P]),
     "fftwpp_amd_scale": (I, [I, P, Z, D, P]),
     "fftwpp_amd_symmetrize_x": (I, [Z, Z, Z, P, Z, P]),
     "fftwpp_amd_symmetrize_xy": (I, [Z, Z, Z, Z, Z, P, Z, Z, P]),
+    "fftwpp_amd_hermitian_plane": (I, [I, I, Z, Z, Z, Z, P, Z, Z, P]),
     "fftwpp_amd_malloc": (I, [ctypes.POINTER(P), Z]),
     "fftwpp_amd_free": (I, [P]),
     "fftwpp_amd_memcpy": (I, [P, P, Z, P]),
@@ -48,6 +51,14 @@ SIGNATURES = {
     "fftwpp_amd_conv_execute_timed": (I, [P, P, P, P, P, ctypes.POINTER(ctypes.c_float), I,
                                           ctypes.POINTER(I)]),
     "fftwpp_amd_conv_stage_name": (ctypes.c_char_p, [P, I]),
+    # batches of 1D convolutions
+    "fftwpp_amd_batch_create": (I, [ctypes.POINTER(P), Z, ZP, ZP, I, I, I]),
+    "fftwpp_amd_batch_destroy": (I, [P]),
+    "fftwpp_amd_batch_size": (Z, [P]),
+    "fftwpp_amd_batch_offsets": (I, [P, ZP]),
+    "fftwpp_amd_batch_geometry": (I, [P, Z, ZP, ZP, ZP]),
+    "fftwpp_amd_batch_describe": (I, [P, ctypes.c_char_p, Z]),
+    "fftwpp_amd_batch_execute": (I, [P, P, P, P, P]),
     # cfftw++.h drop-in
     "create_doubleAlign": (P, [Z]),
     "delete_doubleAlign": (None, [P]),

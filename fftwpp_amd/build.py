@@ -21,7 +21,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Wno-unused-command-line-argument", "-I", INCLUDE, "-I", CSRC]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I", INCLUDE]
 
-HIP_SOURCES = ["kernels.hip"]
+HIP_SOURCES = ["kernels.hip", "batch_plan.hip"]
 HOST_SOURCES = ["host/convolve.cc", "host/capi.cc"]
 
 

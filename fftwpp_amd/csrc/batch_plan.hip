@@ -1,0 +1,507 @@
+// batch_plan.hip -- plans for batches of independent 1D complex dealiased
+// convolutions of uneven lengths (no reference counterpart: FFTW++ convolves
+// one set of arrays per Convolution object and call).
+//
+// Items are packed back to back: item i occupies [off_i, off_i + L_i) of f,
+// g and out.  Each item gets a sub-FFT size m; with the reference's parameter
+// rule (fftBase::parameters, convolve.cc:403-410, non-centred data) that
+// fixes p = ceil(L/m), P = (p == 2 ? 1 : p), n = ceil(M/(P m)), q = P n and
+// the padded length N = m q >= M.  Items with equal (m, q) form a class that
+// one persistent launch of fftwpp_amd_batch_rows convolves.  The classes are
+// forked over side streams (default priority) and joined back; with
+// FFTWPP_AMD_BATCH_GRAPH the first execute() on a set of arrays runs eagerly
+// and records the same fork/join into a hipGraph that later calls replay
+// with one launch.
+//
+// Geometry and tile shapes: with FFTWPP_AMD_BATCH_TUNE the plan times, for
+// every power-of-two length bucket (2^(k-1), 2^k], the candidate sub-FFT
+// sizes (explicit N = 2^ceil(log2 M) with q = 1, and m = 2^k, 2^(k-1),
+// 2^(k-2) within the LDS limit) and for each resulting class the tile shapes
+// (items per block TR, threads per block), keeping the fastest; this replaces
+// the reference's CPU timing optimizer (convolve.cc:177-470).  Without it a
+// fixed rule is used: m = min(2^ceil(log2 L), max_m).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/fftwpp_amd.h"
+#include "batch.h"
+
+extern "C" void fftwpp_amd_internal_set_error(const char *msg);
+
+using fftwpp_amd::BatchItem;
+
+namespace {
+
+int bfail(int code, const char *msg)
+{
+  fftwpp_amd_internal_set_error(msg);
+  return code;
+}
+
+int bhip(hipError_t e, const char *what)
+{
+  if(e == hipSuccess) return FFTWPP_AMD_OK;
+  char buf[256];
+  snprintf(buf,sizeof(buf),"%s: %s",what,hipGetErrorString(e));
+  return bfail(FFTWPP_AMD_ERR_HIP,buf);
+}
+
+size_t ceilquot(size_t a, size_t b) { return (a+b-1)/b; }
+
+size_t ceilpow2(size_t n)
+{
+  size_t p=1;
+  while(p < n) p <<= 1;
+  return p;
+}
+
+// q for an item of length L, minimum padded length M and sub-FFT size m
+// (fftBase::parameters, convolve.cc:403-410; P = p/2 only for p == 2 in the
+// non-centred case).
+size_t residues(size_t L, size_t M, size_t m)
+{
+  size_t p=ceilquot(L,m);
+  size_t P=p == 2 ? 1 : p;
+  size_t n=ceilquot(M,P*m);
+  return P*n;
+}
+
+struct Tile {
+  size_t TR=1, threads=256, nblocks=1, lacc=0;
+};
+
+struct Cls {
+  size_t m=0, q=0;
+  fftwpp_amd_dim *dim=nullptr;
+  std::vector<size_t> ids;      // items, longest first
+  BatchItem *items=nullptr;     // device copy
+  size_t Lmax=0;
+  Tile tile;
+  double work=0;                // N log2 N summed over items
+};
+
+struct Graph {
+  const void *out, *f, *g;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+const int NSIDE=4;
+
+} // namespace
+
+struct fftwpp_amd_batch {
+  int dtype=0, mult=0, flags=0;
+  size_t n=0, total=0;
+  std::vector<size_t> L, M, off, mof;
+  std::vector<Cls> cls;
+  hipStream_t side[NSIDE]={};
+  hipStream_t cap=nullptr;
+  hipEvent_t fork=nullptr, join[NSIDE]={};
+  std::vector<Graph> graphs;
+  int cus=256;
+};
+
+namespace {
+
+void free_class(Cls &c)
+{
+  if(c.items) (void) hipFree(c.items);
+  if(c.dim) fftwpp_amd_dim_destroy(c.dim);
+  c.items=nullptr;
+  c.dim=nullptr;
+}
+
+// Group items by (m, q) for the sub-FFT sizes `mof`; creates twiddle tables
+// and device item lists.
+int build_classes(const fftwpp_amd_batch &h, const std::vector<size_t> &ids,
+                  const std::vector<size_t> &mof, std::vector<Cls> &out)
+{
+  std::map<std::pair<size_t,size_t>,size_t> index;
+  for(size_t i : ids) {
+    size_t m=mof[i], q=residues(h.L[i],h.M[i],m);
+    auto key=std::make_pair(m,q);
+    auto it=index.find(key);
+    if(it == index.end()) {
+      index[key]=out.size();
+      Cls c;
+      c.m=m;
+      c.q=q;
+      out.push_back(c);
+      it=index.find(key);
+    }
+    Cls &c=out[it->second];
+    c.ids.push_back(i);
+    c.Lmax=std::max(c.Lmax,h.L[i]);
+    double N=(double) (m*q);
+    c.work += N*std::log2(N);
+  }
+  for(Cls &c : out) {
+    std::sort(c.ids.begin(),c.ids.end(),[&](size_t a, size_t b) {
+      return h.L[a] != h.L[b] ? h.L[a] > h.L[b] : a < b;
+    });
+    int rc=fftwpp_amd_dim_create(&c.dim,c.Lmax,c.m,c.q,FFTWPP_AMD_KIND_COMPLEX);
+    if(rc) return rc;
+    std::vector<BatchItem> host(c.ids.size());
+    for(size_t k=0; k < c.ids.size(); ++k) {
+      host[k].off=(int64_t) h.off[c.ids[k]];
+      host[k].L=(int64_t) h.L[c.ids[k]];
+    }
+    rc=bhip(hipMalloc(&c.items,host.size()*sizeof(BatchItem)),"hipMalloc batch items");
+    if(rc) return rc;
+    rc=bhip(hipMemcpy(c.items,host.data(),host.size()*sizeof(BatchItem),
+                      hipMemcpyHostToDevice),"copy batch items");
+    if(rc) return rc;
+  }
+  return FFTWPP_AMD_OK;
+}
+
+// Largest LDS accumulation that fits with TR items, else 0.
+Tile make_tile(const fftwpp_amd_batch &h, const Cls &c, size_t TR,
+               size_t threads)
+{
+  Tile t;
+  t.TR=TR;
+  t.threads=threads;
+  t.lacc=fftwpp_amd_batch_rows_lds(c.dim,h.dtype,TR,c.Lmax) ? c.Lmax : 0;
+  size_t tiles=ceilquot(c.ids.size(),TR);
+  t.nblocks=std::min(tiles,(size_t) (8*h.cus));
+  return t;
+}
+
+bool tile_fits(const fftwpp_amd_batch &h, const Cls &c, size_t TR)
+{
+  return fftwpp_amd_batch_rows_lds(c.dim,h.dtype,TR,0) != 0;
+}
+
+Tile default_tile(const fftwpp_amd_batch &h, const Cls &c)
+{
+  size_t TR=std::max<size_t>(1,std::min<size_t>(32,2048/c.m));
+  while(TR > 1 && !tile_fits(h,c,TR)) TR /= 2;
+  size_t threads=std::min<size_t>(256,std::max<size_t>(64,TR*c.m/4));
+  threads=(threads+63)/64*64;
+  return make_tile(h,c,TR,threads);
+}
+
+int launch_class(const fftwpp_amd_batch &h, const Cls &c, const Tile &t,
+                 void *out, const void *f, const void *g, hipStream_t s)
+{
+  return fftwpp_amd_batch_rows(c.dim,h.dtype,h.mult,c.items,c.ids.size(),f,g,out,
+                               t.TR,t.threads,t.nblocks,t.lacc,
+                               1.0/(double) (c.m*c.q),s);
+}
+
+// Average time of one launch (ms), or a negative value on failure.
+double time_launch(const fftwpp_amd_batch &h, const Cls &c, const Tile &t,
+                   void *out, const void *f, const void *g, hipStream_t s,
+                   hipEvent_t e0, hipEvent_t e1)
+{
+  if(launch_class(h,c,t,out,f,g,s)) return -1;
+  const int reps=3;
+  if(hipEventRecord(e0,s) != hipSuccess) return -1;
+  for(int k=0; k < reps; ++k)
+    if(launch_class(h,c,t,out,f,g,s)) return -1;
+  if(hipEventRecord(e1,s) != hipSuccess) return -1;
+  if(hipEventSynchronize(e1) != hipSuccess) return -1;
+  float ms=0;
+  if(hipEventElapsedTime(&ms,e0,e1) != hipSuccess) return -1;
+  return ms/reps;
+}
+
+// Best tile of a class by timing the candidates; returns its time.
+double tune_class(const fftwpp_amd_batch &h, Cls &c, void *out, const void *f,
+                  const void *g, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+{
+  double best=-1;
+  Tile bt=default_tile(h,c);
+  for(size_t TR=1; TR <= 64; TR *= 2) {
+    if(TR > 1 && TR*c.m > 16384) break;
+    if(TR > 1 && TR > c.ids.size()) break;
+    if(!tile_fits(h,c,TR)) break;
+    for(size_t threads=64; threads <= 512; threads *= 2) {
+      if(threads > 64 && threads > TR*c.m/2) break;
+      Tile t=make_tile(h,c,TR,threads);
+      double ms=time_launch(h,c,t,out,f,g,s,e0,e1);
+      if(ms < 0) continue;
+      if(best < 0 || ms < best) { best=ms; bt=t; }
+    }
+  }
+  c.tile=bt;
+  return best;
+}
+
+// Run every class: fork from s over the side streams, join back into s.
+int run_classes(fftwpp_amd_batch &h, void *out, const void *f, const void *g,
+                hipStream_t s)
+{
+  int rc=bhip(hipEventRecord(h.fork,s),"record fork");
+  if(rc) return rc;
+  for(int k=0; k < NSIDE; ++k)
+    if((rc=bhip(hipStreamWaitEvent(h.side[k],h.fork,0),"wait fork"))) return rc;
+  for(size_t i=0; i < h.cls.size(); ++i)
+    if((rc=launch_class(h,h.cls[i],h.cls[i].tile,out,f,g,h.side[i % NSIDE]))) return rc;
+  for(int k=0; k < NSIDE; ++k) {
+    if((rc=bhip(hipEventRecord(h.join[k],h.side[k]),"record join"))) return rc;
+    if((rc=bhip(hipStreamWaitEvent(s,h.join[k],0),"wait join"))) return rc;
+  }
+  return FFTWPP_AMD_OK;
+}
+
+void destroy(fftwpp_amd_batch *h)
+{
+  for(Graph &g : h->graphs) {
+    (void) hipGraphExecDestroy(g.exec);
+    (void) hipGraphDestroy(g.graph);
+  }
+  for(Cls &c : h->cls) free_class(c);
+  for(int k=0; k < NSIDE; ++k) {
+    if(h->side[k]) (void) hipStreamDestroy(h->side[k]);
+    if(h->join[k]) (void) hipEventDestroy(h->join[k]);
+  }
+  if(h->cap) (void) hipStreamDestroy(h->cap);
+  if(h->fork) (void) hipEventDestroy(h->fork);
+  delete h;
+}
+
+// Candidate sub-FFT sizes for the bucket whose longest item is Lmax.
+std::vector<size_t> candidates(size_t Lmax, size_t Mmax, size_t maxm)
+{
+  std::vector<size_t> c;
+  size_t e=ceilpow2(Mmax);
+  if(e <= maxm) c.push_back(e);
+  size_t top=std::min(ceilpow2(Lmax),maxm);
+  for(int s=0; s < 3; ++s) {
+    size_t m=top >> s;
+    if(m < 16) break;
+    if(std::find(c.begin(),c.end(),m) == c.end()) c.push_back(m);
+  }
+  return c;
+}
+
+int tune(fftwpp_amd_batch &h)
+{
+  const size_t elt=h.dtype == FFTWPP_AMD_F32 ? 8 : 16;
+  void *buf=nullptr;
+  int rc=bhip(hipMalloc(&buf,3*h.total*elt),"hipMalloc tuning arrays");
+  if(rc) return rc;
+  char *b=(char *) buf;
+  void *f=b, *g=b+h.total*elt, *out=b+2*h.total*elt;
+  hipEvent_t e0=nullptr, e1=nullptr;
+  rc=bhip(hipMemsetAsync(buf,0,3*h.total*elt,h.cap),"clear tuning arrays");
+  if(!rc) rc=bhip(hipEventCreate(&e0),"event");
+  if(!rc) rc=bhip(hipEventCreate(&e1),"event");
+  const size_t maxm=fftwpp_amd_max_m(h.dtype);
+  // power-of-two length buckets
+  std::map<size_t,std::vector<size_t>> buckets;
+  for(size_t i=0; i < h.n && !rc; ++i) buckets[ceilpow2(h.L[i])].push_back(i);
+  for(auto &kv : buckets) {
+    if(rc) break;
+    const std::vector<size_t> &ids=kv.second;
+    size_t Lmax=0, Mmax=0;
+    for(size_t i : ids) { Lmax=std::max(Lmax,h.L[i]); Mmax=std::max(Mmax,h.M[i]); }
+    double best=-1;
+    size_t bestm=0;
+    for(size_t m : candidates(Lmax,Mmax,maxm)) {
+      std::vector<size_t> mof(h.n,0);
+      for(size_t i : ids) mof[i]=m;
+      std::vector<Cls> cls;
+      rc=build_classes(h,ids,mof,cls);
+      double total=0;
+      for(Cls &c : cls) {
+        if(rc) break;
+        double t=tune_class(h,c,out,f,g,h.cap,e0,e1);
+        if(t < 0) { total=-1; break; }
+        total += t;
+      }
+      for(Cls &c : cls) free_class(c);
+      if(rc) break;
+      if(total >= 0 && (best < 0 || total < best)) { best=total; bestm=m; }
+    }
+    if(!rc && bestm == 0) rc=bfail(FFTWPP_AMD_ERR_HIP,"batch tuning found no working launch");
+    for(size_t i : ids) h.mof[i]=bestm;
+  }
+  if(e0) (void) hipEventDestroy(e0);
+  if(e1) (void) hipEventDestroy(e1);
+  (void) hipStreamSynchronize(h.cap);
+  (void) hipFree(buf);
+  return rc;
+}
+
+} // namespace
+
+extern "C" {
+
+int fftwpp_amd_batch_create(fftwpp_amd_batch **out, size_t n, const size_t *L,
+                            const size_t *M, int dtype, int mult, int flags)
+{
+  if(!out) return bfail(FFTWPP_AMD_ERR_ARG,"null output handle");
+  *out=nullptr;
+  if(n == 0 || !L) return bfail(FFTWPP_AMD_ERR_ARG,"empty batch");
+  if(dtype != FFTWPP_AMD_F64 && dtype != FFTWPP_AMD_F32)
+    return bfail(FFTWPP_AMD_ERR_ARG,"unknown dtype");
+  if(mult != FFTWPP_AMD_MULT_BINARY && mult != FFTWPP_AMD_MULT_CORRELATION &&
+     mult != FFTWPP_AMD_MULT_NONE)
+    return bfail(FFTWPP_AMD_ERR_UNSUPPORTED,"batched convolutions take multNone, multBinary or multcorrelation");
+  int ndev=0;
+  if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return bfail(FFTWPP_AMD_ERR_NODEVICE,"no HIP device");
+  fftwpp_amd_batch *h=new fftwpp_amd_batch;
+  h->dtype=dtype;
+  h->mult=mult;
+  h->flags=flags;
+  h->n=n;
+  h->L.assign(L,L+n);
+  h->M.resize(n);
+  h->off.resize(n+1);
+  h->off[0]=0;
+  for(size_t i=0; i < n; ++i) {
+    if(L[i] == 0 || L[i] > ((size_t) 1 << 30)) {
+      destroy(h);
+      return bfail(FFTWPP_AMD_ERR_ARG,"item length out of range");
+    }
+    // default M = 2L - 1 (wrappers/HybridConvolution.h:19)
+    h->M[i]=M ? M[i] : 2*L[i]-1;
+    if(h->M[i] < L[i] || h->M[i] > ((size_t) 1 << 31)) {
+      destroy(h);
+      return bfail(FFTWPP_AMD_ERR_ARG,"padded length must satisfy L <= M <= 2^31");
+    }
+    h->off[i+1]=h->off[i]+L[i];
+  }
+  h->total=h->off[n];
+  int dev=0;
+  int rc=bhip(hipGetDevice(&dev),"hipGetDevice");
+  if(!rc) rc=bhip(hipDeviceGetAttribute(&h->cus,hipDeviceAttributeMultiprocessorCount,dev),
+                  "device attribute");
+  if(!rc) rc=bhip(hipStreamCreateWithFlags(&h->cap,hipStreamNonBlocking),"stream");
+  for(int k=0; k < NSIDE && !rc; ++k) {
+    rc=bhip(hipStreamCreateWithFlags(&h->side[k],hipStreamNonBlocking),"stream");
+    if(!rc) rc=bhip(hipEventCreateWithFlags(&h->join[k],hipEventDisableTiming),"event");
+  }
+  if(!rc) rc=bhip(hipEventCreateWithFlags(&h->fork,hipEventDisableTiming),"event");
+  if(rc) { destroy(h); return rc; }
+  const size_t maxm=fftwpp_amd_max_m(dtype);
+  h->mof.resize(n);
+  for(size_t i=0; i < n; ++i) h->mof[i]=std::min(ceilpow2(h->L[i]),maxm);
+  if(flags & FFTWPP_AMD_BATCH_TUNE) {
+    rc=tune(*h);
+    if(rc) { destroy(h); return rc; }
+  }
+  std::vector<size_t> all(n);
+  for(size_t i=0; i < n; ++i) all[i]=i;
+  rc=build_classes(*h,all,h->mof,h->cls);
+  if(rc) { destroy(h); return rc; }
+  std::sort(h->cls.begin(),h->cls.end(),[](const Cls &a, const Cls &b) { return a.work > b.work; });
+  if(flags & FFTWPP_AMD_BATCH_TUNE) {
+    const size_t elt=dtype == FFTWPP_AMD_F32 ? 8 : 16;
+    void *buf=nullptr;
+    hipEvent_t e0=nullptr, e1=nullptr;
+    rc=bhip(hipMalloc(&buf,3*h->total*elt),"hipMalloc tuning arrays");
+    if(!rc) rc=bhip(hipMemsetAsync(buf,0,3*h->total*elt,h->cap),"clear tuning arrays");
+    if(!rc) rc=bhip(hipEventCreate(&e0),"event");
+    if(!rc) rc=bhip(hipEventCreate(&e1),"event");
+    char *b=(char *) buf;
+    for(Cls &c : h->cls) {
+      if(rc) break;
+      if(tune_class(*h,c,b+2*h->total*elt,b,b+h->total*elt,h->cap,e0,e1) < 0)
+        rc=bfail(FFTWPP_AMD_ERR_HIP,"batch tuning found no working launch");
+    }
+    if(e0) (void) hipEventDestroy(e0);
+    if(e1) (void) hipEventDestroy(e1);
+    (void) hipStreamSynchronize(h->cap);
+    if(buf) (void) hipFree(buf);
+  } else {
+    for(Cls &c : h->cls) c.tile=default_tile(*h,c);
+  }
+  if(rc) { destroy(h); return rc; }
+  *out=h;
+  return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_batch_destroy(fftwpp_amd_batch *h)
+{
+  if(!h) return FFTWPP_AMD_OK;
+  (void) hipDeviceSynchronize();
+  destroy(h);
+  return FFTWPP_AMD_OK;
+}
+
+size_t fftwpp_amd_batch_size(const fftwpp_amd_batch *h)
+{
+  return h ? h->total : 0;
+}
+
+int fftwpp_amd_batch_offsets(const fftwpp_amd_batch *h, size_t *off)
+{
+  if(!h || !off) return bfail(FFTWPP_AMD_ERR_ARG,"null argument");
+  memcpy(off,h->off.data(),(h->n+1)*sizeof(size_t));
+  return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_batch_geometry(const fftwpp_amd_batch *h, size_t i, size_t *m,
+                              size_t *q, size_t *N)
+{
+  if(!h || i >= h->n) return bfail(FFTWPP_AMD_ERR_ARG,"bad batch item");
+  size_t mm=h->mof[i], qq=residues(h->L[i],h->M[i],mm);
+  if(m) *m=mm;
+  if(q) *q=qq;
+  if(N) *N=mm*qq;
+  return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_batch_describe(const fftwpp_amd_batch *h, char *buf, size_t len)
+{
+  if(!h || !buf || len == 0) return bfail(FFTWPP_AMD_ERR_ARG,"null argument");
+  std::string s;
+  char line[160];
+  for(const Cls &c : h->cls) {
+    snprintf(line,sizeof(line),"m=%zu q=%zu items=%zu Lmax=%zu TR=%zu threads=%zu blocks=%zu lacc=%zu\n",
+             c.m,c.q,c.ids.size(),c.Lmax,c.tile.TR,c.tile.threads,c.tile.nblocks,c.tile.lacc);
+    s += line;
+  }
+  size_t k=std::min(len-1,s.size());
+  memcpy(buf,s.data(),k);
+  buf[k]=0;
+  return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_batch_execute(fftwpp_amd_batch *h, void *out, const void *f,
+                             const void *g, void *stream)
+{
+  if(!h || !out || !f || !g) return bfail(FFTWPP_AMD_ERR_ARG,"null argument");
+  if(!fftwpp_amd_is_device_pointer(out) || !fftwpp_amd_is_device_pointer(f) ||
+     !fftwpp_amd_is_device_pointer(g))
+    return bfail(FFTWPP_AMD_ERR_ARG,"fftwpp_amd_batch_execute: arguments must be device pointers");
+  if(out == f || out == g)
+    return bfail(FFTWPP_AMD_ERR_ARG,"fftwpp_amd_batch_execute: out must not be f or g");
+  hipStream_t s=(hipStream_t) stream;
+  if(!(h->flags & FFTWPP_AMD_BATCH_GRAPH)) return run_classes(*h,out,f,g,s);
+  for(Graph &gr : h->graphs)
+    if(gr.out == out && gr.f == f && gr.g == g)
+      return bhip(hipGraphLaunch(gr.exec,s),"hipGraphLaunch");
+  // first call on these arrays: run eagerly, then record the graph
+  int rc=run_classes(*h,out,f,g,s);
+  if(rc) return rc;
+  Graph gr;
+  gr.out=out; gr.f=f; gr.g=g;
+  rc=bhip(hipStreamBeginCapture(h->cap,hipStreamCaptureModeThreadLocal),"begin capture");
+  if(rc) return rc;
+  int rrc=run_classes(*h,out,f,g,h->cap);
+  rc=bhip(hipStreamEndCapture(h->cap,&gr.graph),"end capture");
+  if(rrc) {
+    if(!rc) (void) hipGraphDestroy(gr.graph);
+    return rrc;
+  }
+  if(rc) return rc;
+  rc=bhip(hipGraphInstantiate(&gr.exec,gr.graph,nullptr,nullptr,0),"hipGraphInstantiate");
+  if(rc) { (void) hipGraphDestroy(gr.graph); return rc; }
+  h->graphs.push_back(gr);
+  return FFTWPP_AMD_OK;
+}
+
+} // extern "C"

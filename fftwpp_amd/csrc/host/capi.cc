@@ -310,6 +310,7 @@ struct fftwpp_amd_conv {
   HybridConvolutionHermitian3 *h3;
   size_t words;   // complex words per array
   fftwpp::DeviceWork work;
+  void *adj[3]={nullptr,nullptr,nullptr};   // Hermitian adjoint temporaries
 
   fftBase *dims[3];
   fftwpp::Timeline timeline;
@@ -340,6 +341,81 @@ static int conv_status(const char *where, const std::exception &e)
   snprintf(capi_error,sizeof(capi_error),"%s: %s",where,e.what());
   fftwpp_amd_internal_set_error(capi_error);
   return fe ? fe->code : FFTWPP_AMD_ERR_ARG;
+}
+
+// Self-conjugate plane of the stored array of a Hermitian plan (see
+// fftwpp_amd_hermitian_plane); false when a centred dimension has even
+// length (its Nyquist mode has no stored partner).
+static bool hermitian_plane_of(const fftwpp_amd_conv *c, size_t p[6])
+{
+  if(c->dim == 1) {
+    size_t q[6]={1,1,0,0,1,1};
+    for(int i=0; i < 6; ++i) p[i]=q[i];
+    return true;
+  }
+  size_t Lx=c->dims[0]->L, Ly=c->dims[1]->L;
+  if(Lx % 2 == 0) return false;
+  if(c->dim == 2) {
+    size_t Hy=(Ly+1)/2;
+    size_t q[6]={(Lx+1)/2,1,Lx/2,0,Hy,1};
+    for(int i=0; i < 6; ++i) p[i]=q[i];
+    return true;
+  }
+  if(Ly % 2 == 0) return false;
+  size_t Hz=(c->dims[2]->L+1)/2;
+  size_t q[6]={(Lx+1)/2,(Ly+1)/2,Lx/2,Ly/2,Ly*Hz,Hz};
+  for(int i=0; i < 6; ++i) p[i]=q[i];
+  return true;
+}
+
+// Adjoint of the Hermitian convolution h = C(S a, S b), S the symmetrizer of
+// the self-conjugate plane (HermitianSymmetrize{,X,XY}), with respect to the
+// stored modes of a and b.  With W the stored cotangent w extended by zero,
+// dL/d conj U = W (*) conj-reflected G = W (*) G for Hermitian G = E(S b), and
+// folding the implied modes back onto the stored ones gives
+//   grad_a = P( C(w_H, S b) ),  grad_b = P( C(w_H, S a) ),
+// where w_H = w + conj w(-k) on the self-conjugate plane (plane op 1) and P
+// is the adjoint of S (plane op 2).  The multiplier is the plan's own
+// realMultBinary: the forward residue transforms are reused unchanged.
+static int hermitian_adjoint(fftwpp_amd_conv *c, const void *w, const void *a,
+                             const void *b, void *grad_a, void *grad_b,
+                             void *stream, size_t bytes)
+{
+  size_t p[6];
+  if(!hermitian_plane_of(c,p)) {
+    fftwpp_amd_internal_set_error("fftwpp_amd_conv_adjoint: Hermitian adjoint needs odd lengths in the centred dimensions");
+    return FFTWPP_AMD_ERR_UNSUPPORTED;
+  }
+  if((grad_a && !b) || (grad_b && !a)) return FFTWPP_AMD_ERR_ARG;
+  for(void *&q : c->adj)
+    if(!q) {
+      int rc=fftwpp_amd_malloc(&q,bytes);
+      if(rc) return rc;
+    }
+  void *wh=c->adj[0], *sa=c->adj[1], *sb=c->adj[2];
+  const int dt=c->dtype;
+  auto plane=[&](int op, void *f) {
+    return fftwpp_amd_hermitian_plane(dt,op,p[0],p[1],p[2],p[3],f,p[4],p[5],stream);
+  };
+  int rc=fftwpp_amd_memcpy(wh,w,bytes,stream);
+  if(!rc) rc=plane(1,wh);
+  try {
+    if(!rc && grad_a) {
+      rc=fftwpp_amd_memcpy(sb,b,bytes,stream);
+      if(!rc) rc=plane(0,sb);
+      if(!rc) c->exec(grad_a,wh,sb,stream,-1);
+      if(!rc) rc=plane(2,grad_a);
+    }
+    if(!rc && grad_b) {
+      rc=fftwpp_amd_memcpy(sa,a,bytes,stream);
+      if(!rc) rc=plane(0,sa);
+      if(!rc) c->exec(grad_b,wh,sa,stream,-1);
+      if(!rc) rc=plane(2,grad_b);
+    }
+  } catch(const std::exception &e) {
+    return conv_status("fftwpp_amd_conv_adjoint",e);
+  }
+  return rc;
 }
 
 extern "C" {
@@ -413,6 +489,8 @@ int fftwpp_amd_conv_destroy(fftwpp_amd_conv *c)
   if(!c) return FFTWPP_AMD_OK;
   delete c->c1; delete c->h1; delete c->k1; delete c->c2; delete c->h2; delete c->c3;
   delete c->h3;
+  for(void *p : c->adj)
+    if(p) fftwpp_amd_free(p);
   delete c;
   return FFTWPP_AMD_OK;
 }
@@ -487,10 +565,6 @@ int fftwpp_amd_conv_adjoint(fftwpp_amd_conv *c, const void *w, const void *a,
                             void *stream)
 {
   if(!c || !w) return FFTWPP_AMD_ERR_ARG;
-  if(c->hermitian == 1) {
-    fftwpp_amd_internal_set_error("fftwpp_amd_conv_adjoint: Hermitian adjoint not implemented");
-    return FFTWPP_AMD_ERR_UNSUPPORTED;
-  }
   size_t bytes=c->words*(c->dtype == FFTWPP_AMD_F32 ? 8 : 16);
   const void *ptrs[5]={w,a,b,grad_a,grad_b};
   for(const void *p : ptrs)
@@ -498,6 +572,7 @@ int fftwpp_amd_conv_adjoint(fftwpp_amd_conv *c, const void *w, const void *a,
       fftwpp_amd_internal_set_error("fftwpp_amd_conv_adjoint: arguments must be device pointers");
       return FFTWPP_AMD_ERR_ARG;
     }
+  if(c->hermitian == 1) return hermitian_adjoint(c,w,a,b,grad_a,grad_b,stream,bytes);
   try {
     if(grad_a) {
       if(!b) return FFTWPP_AMD_ERR_ARG;

@@ -20,10 +20,14 @@
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
+#include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "common.h"
 #include "fft_lds.h"
 #include "kernels256.h"
+#include "batch.h"
 #include "../../include/fftwpp_amd.h"
 
 using namespace fftwpp_amd;
@@ -369,6 +373,41 @@ __global__ void symxy3_kernel(int64_t Hx, int64_t Hy, int64_t Hz, int64_t x0,
 }
 
 // ------------------------------------------------------------------------
+// Self-conjugate plane of a centred Hermitian array (the kz = 0 plane in 3D,
+// the ky = 0 line in 2D, k = 0 in 1D): element (i, j), |i| < Hx, |j| < Hy,
+// at f[(x0+i) Sx + (y0+j) Sy] pairs with (-i, -j).  "Upper" elements are
+// j > 0, or j = 0 and i > 0 (the ones HermitianSymmetrizeX/XY copy from,
+// convolve.h:1177-1256); one thread per upper element or the origin.
+//   op 0: f(-i,-j) <- conj f(i,j), f(0,0) <- Re f(0,0)   (symmetrize)
+//   op 1: both members <- w + conj(partner), w(0,0) <- 2 Re w(0,0)
+//   op 2: upper kept, partner zeroed, origin halved (adjoint of op 0)
+// ------------------------------------------------------------------------
+template<typename T>
+__global__ void hplane_kernel(int op, int64_t Hx, int64_t Hy, int64_t x0,
+                              int64_t y0, cplx<T> *f, int64_t Sx, int64_t Sy)
+{
+  int64_t t=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  if(t >= (2*Hx-1)*Hy) return;
+  const int64_t i=t/Hy-(Hx-1), j=t % Hy;
+  if(j == 0 && i < 0) return;
+  cplx<T> *u=f+(x0+i)*Sx+(y0+j)*Sy;
+  if(i == 0 && j == 0) {
+    const cplx<T> v=*u;
+    if(op == 0) *u=mk(v.x,(T) 0);
+    else if(op == 1) *u=mk(2*v.x,(T) 0);
+    else *u=mk(v.x/2,v.y/2);
+    return;
+  }
+  cplx<T> *l=f+(x0-i)*Sx+(y0-j)*Sy;
+  if(op == 0) *l=conjg(*u);
+  else if(op == 1) {
+    const cplx<T> a=*u, b=*l;
+    *u=a+conjg(b);
+    *l=b+conjg(a);
+  } else *l=mk((T) 0,(T) 0);
+}
+
+// ------------------------------------------------------------------------
 // Host-side helpers of the C ABI
 // ------------------------------------------------------------------------
 thread_local char last_error[512];
@@ -406,15 +445,24 @@ static size_t lds_budget() { return 64*1024; }
 
 // Kernels that take more than the default 64 KB of dynamic LDS must raise
 // their limit first (gfx950 has 160 KB per workgroup).
+// The limit is raised once per kernel to the full 160 KB, so launches
+// recorded into a graph later make no further runtime calls.
+static size_t lds_max() { return 160*1024; }
+
 template<typename K>
 static int allow_lds(K kernel, size_t bytes)
 {
   if(bytes <= 64*1024) return FFTWPP_AMD_OK;
-  return hipcheck(hipFuncSetAttribute((const void *) kernel,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int) bytes),"hipFuncSetAttribute");
+  static std::mutex mu;
+  static std::vector<const void *> raised;
+  std::lock_guard<std::mutex> lock(mu);
+  const void *fn=(const void *) kernel;
+  if(std::find(raised.begin(),raised.end(),fn) != raised.end()) return FFTWPP_AMD_OK;
+  int rc=hipcheck(hipFuncSetAttribute(fn,hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int) lds_max()),"hipFuncSetAttribute");
+  if(!rc) raised.push_back(fn);
+  return rc;
 }
-static size_t lds_max() { return 160*1024; }
 
 // The register-resident m=256 kernels apply to non-centred complex
 // dimensions with p = 1.
@@ -498,6 +546,62 @@ static int bwd_many_t(const fftwpp_amd_dim *d, const void *F, size_t SF,
                      (cplx<T> *) f,(int64_t) S,(int64_t) C,(int64_t) bso,
                      (int64_t) r0,(int64_t) nres,accumulate,(T) scale,(int) TC);
   return launch_check("bwd_many_kernel");
+}
+
+// LDS bytes of batch_conv_kernel for sub-FFT size m, TR items and (with
+// LDS accumulation) rows of lacc words; ld = m+1 (bank padding) when that
+// fits, else m.
+static size_t batch_lds(size_t m, size_t elt, size_t TR, size_t lacc,
+                        size_t *ldout)
+{
+  size_t ld=m+1;
+  const size_t base=TR*sizeof(BatchItem);
+  if(base+(4*TR*ld+TR*lacc)*elt > lds_max()) ld=m;
+  if(ldout) *ldout=ld;
+  return base+(4*TR*ld+TR*lacc)*elt;
+}
+
+template<typename T, int MULT>
+static int batch_rows_launch(const DimArgs &a, const BatchItem *items,
+                             size_t count, const void *f, const void *g,
+                             void *out, size_t TR, size_t threads,
+                             size_t nblocks, size_t lacc, double scale,
+                             size_t ld, size_t lds, hipStream_t st)
+{
+  int rc;
+  if(lacc) {
+    if((rc=allow_lds(batch_conv_kernel<T,MULT,true>,lds))) return rc;
+    hipLaunchKernelGGL((batch_conv_kernel<T,MULT,true>),dim3((unsigned) nblocks),
+                       dim3((unsigned) threads),lds,st,a,items,(int64_t) count,
+                       (const cplx<T> *) f,(const cplx<T> *) g,(cplx<T> *) out,
+                       (T) scale,(int) TR,(int) ld,(int) lacc);
+  } else {
+    if((rc=allow_lds(batch_conv_kernel<T,MULT,false>,lds))) return rc;
+    hipLaunchKernelGGL((batch_conv_kernel<T,MULT,false>),dim3((unsigned) nblocks),
+                       dim3((unsigned) threads),lds,st,a,items,(int64_t) count,
+                       (const cplx<T> *) f,(const cplx<T> *) g,(cplx<T> *) out,
+                       (T) scale,(int) TR,(int) ld,0);
+  }
+  return launch_check("batch_conv_kernel");
+}
+
+template<typename T>
+static int batch_rows_t(const fftwpp_amd_dim *d, int mult, const void *items,
+                        size_t count, const void *f, const void *g, void *out,
+                        size_t TR, size_t threads, size_t nblocks, size_t lacc,
+                        double scale, hipStream_t st)
+{
+  const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
+  size_t ld;
+  const size_t lds=batch_lds(d->m,sizeof(cplx<T>),TR,lacc,&ld);
+  if(lds > lds_max())
+    return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"batch tile does not fit in LDS");
+  const BatchItem *it=(const BatchItem *) items;
+  if(mult == FFTWPP_AMD_MULT_CORRELATION)
+    return batch_rows_launch<T,3>(a,it,count,f,g,out,TR,threads,nblocks,lacc,scale,ld,lds,st);
+  if(mult == FFTWPP_AMD_MULT_NONE)
+    return batch_rows_launch<T,0>(a,it,count,f,g,out,TR,threads,nblocks,lacc,scale,ld,lds,st);
+  return batch_rows_launch<T,1>(a,it,count,f,g,out,TR,threads,nblocks,lacc,scale,ld,lds,st);
 }
 
 template<typename T>
@@ -716,6 +820,37 @@ int fftwpp_amd_conv_rows(const fftwpp_amd_dim *d, int dtype, int mult,
     conv_rows_t<double>(d,mult,f,g,rows,rowstride,out,outstride,scale,st);
 }
 
+size_t fftwpp_amd_batch_rows_lds(const fftwpp_amd_dim *d, int dtype,
+                                 size_t TR, size_t lacc)
+{
+  if(!d || TR == 0) return 0;
+  const size_t elt=dtype == FFTWPP_AMD_F32 ? sizeof(cplx<float>) : sizeof(cplx<double>);
+  const size_t b=batch_lds(d->m,elt,TR,lacc,NULL);
+  return b > lds_max() ? 0 : b;
+}
+
+int fftwpp_amd_batch_rows(const fftwpp_amd_dim *d, int dtype, int mult,
+                          const void *items, size_t count, const void *f,
+                          const void *g, void *out, size_t TR, size_t threads,
+                          size_t nblocks, size_t lacc, double scale,
+                          void *stream)
+{
+  if(!d || !items || !f || !g || !out) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  if(count == 0) return FFTWPP_AMD_OK;
+  if(d->kind != FFTWPP_AMD_KIND_COMPLEX)
+    return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"batched rows are non-centred complex");
+  if(mult != FFTWPP_AMD_MULT_BINARY && mult != FFTWPP_AMD_MULT_CORRELATION &&
+     mult != FFTWPP_AMD_MULT_NONE)
+    return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"batched rows take multNone, multBinary or multcorrelation");
+  if(TR == 0 || nblocks == 0 || nblocks > (1u << 30) || threads < 64 ||
+     threads > 1024 || threads % 64 != 0)
+    return fail(FFTWPP_AMD_ERR_ARG,"bad tile, block or thread count");
+  hipStream_t st=(hipStream_t) stream;
+  return dtype == FFTWPP_AMD_F32 ?
+    batch_rows_t<float>(d,mult,items,count,f,g,out,TR,threads,nblocks,lacc,scale,st) :
+    batch_rows_t<double>(d,mult,items,count,f,g,out,TR,threads,nblocks,lacc,scale,st);
+}
+
 int fftwpp_amd_multiply(int dtype, int mult, void *F0, const void *F1,
                         size_t n, void *stream)
 {
@@ -787,6 +922,27 @@ int fftwpp_amd_symmetrize_xy(size_t Hx, size_t Hy, size_t Hz, size_t x0,
     rc=launch_check("symxy3_kernel");
   }
   return rc;
+}
+
+int fftwpp_amd_hermitian_plane(int dtype, int op, size_t Hx, size_t Hy,
+                               size_t x0, size_t y0, void *f, size_t Sx,
+                               size_t Sy, void *stream)
+{
+  if(!f || Hx == 0 || Hy == 0) return fail(FFTWPP_AMD_ERR_ARG,"bad argument");
+  if(op < 0 || op > 2) return fail(FFTWPP_AMD_ERR_ARG,"unknown plane operation");
+  if(x0+1 < Hx || y0+1 < Hy) return fail(FFTWPP_AMD_ERR_ARG,"origin too small");
+  hipStream_t st=(hipStream_t) stream;
+  const size_t n=(2*Hx-1)*Hy;
+  const dim3 grid((unsigned) ((n+255)/256));
+  if(dtype == FFTWPP_AMD_F32)
+    hipLaunchKernelGGL(hplane_kernel<float>,grid,dim3(256),0,st,op,(int64_t) Hx,
+                       (int64_t) Hy,(int64_t) x0,(int64_t) y0,(cplx<float> *) f,
+                       (int64_t) Sx,(int64_t) Sy);
+  else
+    hipLaunchKernelGGL(hplane_kernel<double>,grid,dim3(256),0,st,op,(int64_t) Hx,
+                       (int64_t) Hy,(int64_t) x0,(int64_t) y0,(cplx<double> *) f,
+                       (int64_t) Sx,(int64_t) Sy);
+  return launch_check("hplane_kernel");
 }
 
 // ---- memory and stream plumbing for host code that includes no HIP header

@@ -86,13 +86,49 @@ class DeviceConvolution:
         check(lib().fftwpp_amd_conv_execute(self._h, a.data_ptr(), b.data_ptr(), _stream()))
         return a
 
+    def convolve_out(self, out, a, b):
+        """out <- a (*) b (normalised); a and b are only read."""
+        self._check(out)
+        self._check(a)
+        self._check(b)
+        check(lib().fftwpp_amd_conv_execute_out(self._h, out.data_ptr(), a.data_ptr(),
+                                                b.data_ptr(), _stream()))
+        return out
+
     def __call__(self, a, b):
         out = a.clone()
         return self.convolve_(out, b)
 
+    def _plane(self):
+        """Self-conjugate plane (Hx, Hy, x0, y0, Sx, Sy) of a Hermitian plan's
+        stored array (the arguments of fftwpp_amd_hermitian_plane)."""
+        if self.dim == 1:
+            return (1, 1, 0, 0, 1, 1)
+        Lx, Ly = self.lengths[0], self.lengths[1]
+        if Lx % 2 == 0 or (self.dim == 3 and Ly % 2 == 0):
+            raise ValueError("centred dimensions of a differentiable Hermitian "
+                             "convolution must have odd length")
+        if self.dim == 2:
+            return ((Lx + 1) // 2, 1, Lx // 2, 0, (Ly + 1) // 2, 1)
+        Hz = (self.lengths[2] + 1) // 2
+        return ((Lx + 1) // 2, (Ly + 1) // 2, Lx // 2, Ly // 2, Ly * Hz, Hz)
+
+    def symmetrized(self, t):
+        """Copy of t with the self-conjugate plane made Hermitian
+        (HermitianSymmetrize{,X,XY}, convolve.h:1168-1256)."""
+        self._check(t)
+        out = t.clone()
+        Hx, Hy, x0, y0, Sx, Sy = self._plane()
+        check(lib().fftwpp_amd_hermitian_plane(_dtype_code(self.dtype), 0, Hx, Hy, x0, y0,
+                                               out.data_ptr(), Sx, Sy, _stream()))
+        return out
+
     def adjoint(self, w, a, b, need_a=True, need_b=True):
         """Cotangents of (a, b) for output cotangent w (PyTorch convention
-        grad = dL/d conj(z)): grad_a = w (x) b, grad_b = w (x) a."""
+        grad = dL/d conj(z)).  Complex kinds: grad_a = w (x) b, grad_b =
+        w (x) a (dealiased correlations).  Hermitian kind: the cotangents
+        of h = conv(S a, S b) with S the symmetrizer of the self-conjugate
+        plane, with respect to the stored modes."""
         self._check(w)
         ga = torch.empty_like(w) if need_a else None
         gb = torch.empty_like(w) if need_b else None
@@ -110,6 +146,8 @@ class _ConvolveFn(torch.autograd.Function):
         b = b.contiguous()
         ctx.plan = plan
         ctx.save_for_backward(a, b)
+        if plan.kind == "hermitian":
+            return plan(plan.symmetrized(a), plan.symmetrized(b))
         return plan(a, b)
 
     @staticmethod
@@ -121,5 +159,7 @@ class _ConvolveFn(torch.autograd.Function):
 
 
 def convolve(a, b, plan):
-    """Differentiable dealiased convolution h = a (*) b with ``plan``."""
+    """Differentiable dealiased convolution h = a (*) b with ``plan``.  For a
+    Hermitian plan the inputs are symmetrized first (h = S a (*) S b), so the
+    function and its gradient are defined for any stored arrays."""
     return _ConvolveFn.apply(a, b, plan)

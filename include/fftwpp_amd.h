@@ -102,6 +102,24 @@ int fftwpp_amd_conv_rows(const fftwpp_amd_dim *d, int dtype, int mult,
                          size_t rowstride, void *out, size_t outstride,
                          double scale, void *stream);
 
+/* Fused convolutions of `count` independent items that share the geometry
+ * (m, q) of d.  items is a device array of {int64 off, int64 L} pairs: item
+ * i convolves f[off..off+L) with g[off..off+L) into out[off..off+L), which
+ * is exact when N = m q >= 2L - 1.  Tiles of TR items per block, `threads`
+ * per block (a multiple of 64, at most 1024), `nblocks` blocks looping over
+ * the tiles; lacc > 0 accumulates the residues in LDS rows of lacc >= max L
+ * words, lacc = 0 in out.  Each item follows the residue loop of
+ * Convolution::convolveRaw (convolve.cc:7513-7575); the batching itself has
+ * no reference counterpart.  fftwpp_amd_batch_rows_lds gives the dynamic
+ * LDS bytes of a tile, 0 when it does not fit. */
+size_t fftwpp_amd_batch_rows_lds(const fftwpp_amd_dim *d, int dtype,
+                                 size_t TR, size_t lacc);
+int fftwpp_amd_batch_rows(const fftwpp_amd_dim *d, int dtype, int mult,
+                          const void *items, size_t count, const void *f,
+                          const void *g, void *out, size_t TR, size_t threads,
+                          size_t nblocks, size_t lacc, double scale,
+                          void *stream);
+
 /* The multipliers on device arrays (convolve.cc:26-110): n counts complex
  * words for multBinary/multcorrelation and real words for realMultBinary. */
 int fftwpp_amd_multiply(int dtype, int mult, void *F0, const void *F1,
@@ -115,6 +133,20 @@ int fftwpp_amd_symmetrize_x(size_t Hx, size_t Hy, size_t x0, void *f,
 int fftwpp_amd_symmetrize_xy(size_t Hx, size_t Hy, size_t Hz, size_t x0,
                              size_t y0, void *f, size_t Sx, size_t Sy,
                              void *stream);
+
+/* Operations on the self-conjugate plane of a centred Hermitian array (the
+ * kz = 0 plane in 3D, the ky = 0 line in 2D with Hy = 1, k = 0 in 1D with
+ * Hx = Hy = 1): element (i, j), |i| < Hx, |j| < Hy, lives at
+ * f[(x0+i)*Sx + (y0+j)*Sy] and pairs with (-i, -j); "upper" elements have
+ * j > 0, or j = 0 and i > 0.  dtype selects complex128 or complex64.
+ *   op 0: lower <- conj(upper), origin <- Re origin: HermitianSymmetrizeX/XY
+ *         (convolve.h:1177-1256) without the Nyquist-row zeroing;
+ *   op 1: w(k) <- w(k) + conj w(-k) for both members, origin <- 2 Re origin;
+ *   op 2: upper kept, lower zeroed, origin halved (the adjoint of op 0).
+ * Used by the Hermitian adjoint; no reference counterpart for ops 1-2. */
+int fftwpp_amd_hermitian_plane(int dtype, int op, size_t Hx, size_t Hy,
+                               size_t x0, size_t y0, void *f, size_t Sx,
+                               size_t Sy, void *stream);
 
 /* Device memory and stream plumbing (host code includes no HIP header). */
 int fftwpp_amd_malloc(void **p, size_t bytes);
@@ -181,6 +213,45 @@ const char *fftwpp_amd_conv_stage_name(const fftwpp_amd_conv *h, int i);
 /* Padded sizes N[d] = m*q actually used and the sub-FFT sizes m[d], q[d]. */
 int fftwpp_amd_conv_geometry(const fftwpp_amd_conv *h, size_t *N,
                              size_t *m, size_t *q);
+
+/* ------------------------------------------------------------------------
+ * Batches of independent 1D complex convolutions of uneven lengths.  No
+ * reference counterpart: FFTW++ convolves one set of arrays per Convolution
+ * object and call (convolve.h:982-1166); a batch of n items replaces n such
+ * objects (each with A = 2, B = 1 and the given multiplier) and n calls.
+ *
+ * Items are packed: item i occupies [off_i, off_i + L_i) of the device arrays
+ * f, g and out, off_0 = 0, off_{i+1} = off_i + L_i.  M[i] is the minimum
+ * padded length of item i (M = NULL: 2 L_i - 1, as HybridConvolution's
+ * default, wrappers/HybridConvolution.h:19).  The result is normalised like
+ * Convolution::convolve.
+ * ------------------------------------------------------------------------ */
+typedef struct fftwpp_amd_batch fftwpp_amd_batch;
+
+/* time candidate sub-FFT sizes and tile shapes per length bucket at
+ * creation (in place of the reference's CPU timing optimizer) */
+#define FFTWPP_AMD_BATCH_TUNE 1
+/* record execute() into a hipGraph per (out, f, g) and replay it */
+#define FFTWPP_AMD_BATCH_GRAPH 2
+
+int fftwpp_amd_batch_create(fftwpp_amd_batch **h, size_t n, const size_t *L,
+                            const size_t *M, int dtype, int mult, int flags);
+int fftwpp_amd_batch_destroy(fftwpp_amd_batch *h);
+/* total packed length (sum of L) and the n+1 offsets */
+size_t fftwpp_amd_batch_size(const fftwpp_amd_batch *h);
+int fftwpp_amd_batch_offsets(const fftwpp_amd_batch *h, size_t *off);
+/* sub-FFT size, residue count and padded length chosen for item i */
+int fftwpp_amd_batch_geometry(const fftwpp_amd_batch *h, size_t i, size_t *m,
+                              size_t *q, size_t *N);
+/* one line per launch class: m, q, items, tile shape */
+int fftwpp_amd_batch_describe(const fftwpp_amd_batch *h, char *buf,
+                              size_t len);
+/* out <- f (*) g item by item (device pointers; out distinct from f, g) on the
+ * HIP stream; with FFTWPP_AMD_BATCH_GRAPH the first call on a given
+ * (out, f, g) runs directly and records the graph that later calls replay.
+ * Calls on one handle must not overlap. */
+int fftwpp_amd_batch_execute(fftwpp_amd_batch *h, void *out, const void *f,
+                             const void *g, void *stream);
 
 #ifdef __cplusplus
 }

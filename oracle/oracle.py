@@ -325,6 +325,64 @@ def explicit_hconv(f, g, N):
     return HH[np.ix_(*out_idx)]
 
 
+def hplane_of(lengths):
+    """Self-conjugate plane (Hx, Hy, x0, y0, Sx, Sy) of the stored array of a
+    centred Hermitian convolution with full lengths ``lengths``: k = 0 in 1D,
+    the ky = 0 line in 2D, the kz = 0 plane in 3D (odd centred lengths)."""
+    d = len(lengths)
+    if d == 1:
+        return (1, 1, 0, 0, 1, 1)
+    Lx, Ly = lengths[0], lengths[1]
+    if d == 2:
+        return ((Lx + 1) // 2, 1, Lx // 2, 0, (Ly + 1) // 2, 1)
+    Hz = (lengths[2] + 1) // 2
+    return ((Lx + 1) // 2, (Ly + 1) // 2, Lx // 2, Ly // 2, Ly * Hz, Hz)
+
+
+def hplane(a, op, plane):
+    """Operations on the self-conjugate plane (element (i,j) at
+    (x0+i) Sx + (y0+j) Sy pairs with (-i,-j); upper = j > 0 or j = 0, i > 0):
+    op 0 symmetrize (lower <- conj upper, origin real, as HermitianSymmetrizeX/XY,
+    convolve.h:1177-1256), op 1 w + conj w(-k) on both members and 2 Re at the
+    origin, op 2 keep upper, zero lower, halve the origin (adjoint of op 0)."""
+    Hx, Hy, x0, y0, Sx, Sy = plane
+    f = np.array(a, dtype=np.complex128, copy=True).reshape(-1)
+    for i in range(-(Hx - 1), Hx):
+        for j in range(Hy):
+            if j == 0 and i < 0:
+                continue
+            u = (x0 + i) * Sx + (y0 + j) * Sy
+            if i == 0 and j == 0:
+                v = f[u]
+                f[u] = v.real if op == 0 else (2 * v.real if op == 1 else v / 2)
+                continue
+            lo = (x0 - i) * Sx + (y0 - j) * Sy
+            if op == 0:
+                f[lo] = np.conj(f[u])
+            elif op == 1:
+                a_, b_ = f[u], f[lo]
+                f[u] = a_ + np.conj(b_)
+                f[lo] = b_ + np.conj(a_)
+            else:
+                f[lo] = 0
+    return f.reshape(np.shape(a))
+
+
+def hconv_adjoint(w, f, g, lengths):
+    """Cotangents (d/d conj f, d/d conj g) of h = C(S f, S g) for a real loss
+    with dL/d conj h = w, C the centred Hermitian convolution of stored modes
+    (direct sums) and S the symmetrizer of the self-conjugate plane:
+    grad_f = P(C(w_H, S g)), grad_g = P(C(w_H, S f)) with w_H = hplane op 1 and
+    P = hplane op 2 (the Hermitian G makes the correlation with G a
+    convolution with G)."""
+    plane = hplane_of(lengths)
+    conv = {1: direct_convh, 2: direct_convh2, 3: direct_convh3}[len(lengths)]
+    wh = hplane(w, 1, plane)
+    gf = hplane(conv(wh, hplane(g, 0, plane)), 2, plane)
+    gg = hplane(conv(wh, hplane(f, 0, plane)), 2, plane)
+    return gf, gg
+
+
 def rel_l2(a, b):
     a = np.asarray(a, dtype=np.complex128)
     b = np.asarray(b, dtype=np.complex128)

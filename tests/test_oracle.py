@@ -261,3 +261,44 @@ def test_parameters_worked_geometries(oracle):
     assert oracle.default_M(2047, hermitian=True) == 3070
     assert oracle.default_M(1023, hermitian=True) == 1534
     assert oracle.default_M(256) == 511
+
+
+@pytest.mark.parametrize("lengths", [(11,), (9, 9), (5, 7, 5)])
+def test_hermitian_adjoint_formula_fd(oracle, lengths):
+    """Central finite differences of L = sum |C(S f, S g) - t|^2 (direct
+    Hermitian sums of symmetrized inputs) pin the adjoint formula that the
+    GPU implements: dL = 2 Re <grad, delta> with grad = dL/d conj f."""
+    rng = np.random.default_rng(17 + len(lengths))
+    shape = tuple(lengths[:-1]) + ((lengths[-1] + 1) // 2,)
+    plane = oracle.hplane_of(lengths)
+    conv = {1: oracle.direct_convh, 2: oracle.direct_convh2, 3: oracle.direct_convh3}[len(lengths)]
+
+    def rnd():
+        return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    f, g, t = rnd(), rnd(), rnd()
+
+    def loss(f, g):
+        h = conv(oracle.hplane(f, 0, plane), oracle.hplane(g, 0, plane))
+        return np.sum(np.abs(h - t) ** 2)
+    h = conv(oracle.hplane(f, 0, plane), oracle.hplane(g, 0, plane))
+    gf, gg = oracle.hconv_adjoint(h - t, f, g, lengths)
+    eps = 1e-6
+    for _ in range(4):
+        d = rnd()
+        for grad, lp, lm in ((gf, loss(f + eps * d, g), loss(f - eps * d, g)),
+                             (gg, loss(f, g + eps * d), loss(f, g - eps * d))):
+            fd = (lp - lm) / (2 * eps)
+            an = 2 * np.real(np.sum(np.conj(grad) * d))
+            assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+
+
+def test_hplane_symmetrize_matches_reference_rule(oracle):
+    """hplane op 0 is HermitianSymmetrizeX / XY (odd lengths) as restated in
+    oracle.c."""
+    rng = np.random.default_rng(4)
+    f = rng.standard_normal((9, 5)) + 1j * rng.standard_normal((9, 5))
+    assert np.array_equal(oracle.hplane(f, 0, oracle.hplane_of((9, 9))),
+                          oracle.symmetrize_x(f.copy()))
+    f3 = rng.standard_normal((5, 7, 4)) + 1j * rng.standard_normal((5, 7, 4))
+    assert np.array_equal(oracle.hplane(f3, 0, oracle.hplane_of((5, 7, 7))),
+                          oracle.symmetrize_xy(f3.copy()))
