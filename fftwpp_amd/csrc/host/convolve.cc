@@ -9,6 +9,9 @@
 #include <iostream>
 #include <sstream>
 
+// kernel-layer query (kernels.hip): does the fused row kernel take a row?
+extern "C" int fftwpp_amd_internal_rows_fit(const fftwpp_amd_dim *d, int dtype);
+
 namespace fftwpp {
 
 const double twopi=2.0*3.14159265358979323846264338327950288;
@@ -636,7 +639,7 @@ void Convolution::execDevice(void *out, const void *f0, const void *f1, bool nor
   double s=normalize ? scale : 1.0;
   size_t n=fft->inputLength();
   if(timeline) timeline->begin(stream);
-  if(fft->m <= fftwpp_amd_max_m(dtype)) {
+  if(fft->m <= fftwpp_amd_max_m(dtype) && fftwpp_amd_internal_rows_fit(fft->dev,dtype)) {
     check(fftwpp_amd_conv_rows(fft->dev,dtype,code,f0,f1,1,n,out,n,s,
                                stream),"conv_rows");
     if(timeline) timeline->mark("rows");

@@ -703,6 +703,18 @@ const char *fftwpp_amd_last_error(void) { return last_error; }
 // Used by the host C++ layer to record its own failures in the same slot.
 void fftwpp_amd_internal_set_error(const char *msg) { fail(0,msg); }
 
+// Whether fftwpp_amd_conv_rows can take one row of dimension d in dtype: the
+// m = 256 register kernels always can; the generic kernel needs four
+// sequences of m words plus the L (or H) outputs in LDS.
+int fftwpp_amd_internal_rows_fit(const fftwpp_amd_dim *d, int dtype)
+{
+  if(!d) return 0;
+  if(fast256(d)) return 1;
+  const size_t elt=dtype == FFTWPP_AMD_F32 ? sizeof(cplx<float>) : sizeof(cplx<double>);
+  const size_t lout=d->kind == FFTWPP_AMD_KIND_HERMITIAN ? (d->L+1)/2 : d->L;
+  return (4*d->m+lout)*elt <= lds_max();
+}
+
 int fftwpp_amd_dim_create(fftwpp_amd_dim **out, size_t L, size_t m,
                           size_t q, int kind)
 {

@@ -415,3 +415,21 @@ def test_register_fft256_paths(gpu, oracle, L, M):
     _, _, q3 = oracle.parameters(6, 11, oracle.parameters(6, 11, 6, False)[0] and 6, False)
     want = oracle.explicit_conv(f3, g3, (11, N, 17))
     assert oracle.rel_l2(h3, want) < TOL64
+
+
+@pytest.mark.parametrize("L", [4097, 16384])
+def test_conv1d_long_rows_unfused(gpu, oracle, L):
+    """Lengths whose outputs do not fit beside the sub-FFTs in LDS take the
+    unfused residue path (forward, multiply, backward)."""
+    import torch
+    from fftwpp_amd.torch_conv import DeviceConvolution
+    rng = np.random.default_rng(L)
+    f, g = rnd(rng, L), rnd(rng, L)
+    want = oracle.explicit_conv(f, g, 1 << (2 * L - 1).bit_length())
+    for dtype, tol in ((torch.complex128, TOL64), (torch.complex64, 1e-5)):
+        plan = DeviceConvolution((L,), dtype=dtype, M=[2 * L])
+        out = torch.empty(L, dtype=dtype, device=gpu)
+        plan.convolve_out(out, torch.tensor(f, dtype=dtype, device=gpu),
+                          torch.tensor(g, dtype=dtype, device=gpu))
+        torch.cuda.synchronize()
+        assert oracle.rel_l2(out.cpu().numpy(), want) < tol
