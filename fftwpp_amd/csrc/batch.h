@@ -13,11 +13,14 @@
 // (convolve.cc:1468-1763: inverse m-point DFT, conjugate twiddle, accumulate
 // over residues) fused in one workgroup.
 //
-// Work decomposition: a tile is TR consecutive items; blocks take tiles
+// Work decomposition: a tile is TR consecutive items and a residue group is
+// D consecutive residues; a work unit is (tile, group) and blocks take units
 // blockIdx.x, blockIdx.x + gridDim.x, ... (a persistent grid that drains
-// the class).  The sums over residues accumulate in LDS when the tile's
-// outputs fit there (LACC) and otherwise in the output itself, which each
-// tile owns exclusively.
+// the class).  The sum over a unit's residues accumulates in LDS when the
+// tile's outputs fit there (LACC) and otherwise in its destination, which
+// the unit owns exclusively: the output itself when one group covers all q
+// residues (G = 1), else the group's slice of the partial sums, which
+// batch_reduce_kernel adds in group order (deterministic).
 #pragma once
 #include "common.h"
 #include "fft_lds.h"
@@ -27,7 +30,14 @@ namespace fftwpp_amd {
 struct BatchItem {
   int64_t off;   // first element of the item in the packed arrays
   int64_t L;     // item length
+  int64_t poff;  // first element of the item in its class's partial sums
 };
+
+// LDS bytes of the item table of a tile, rounded up to 16-byte alignment.
+__host__ __device__ inline size_t batch_items_bytes(size_t TR)
+{
+  return (TR*sizeof(BatchItem)+15) & ~(size_t) 15;
+}
 
 // sum_{j = sigma mod m, j < L} w_N^{j r} f[j]
 template<typename T>
@@ -46,28 +56,34 @@ __device__ __forceinline__ cplx<T> fold_len(const DimArgs &d,
 
 // Dynamic LDS: TR items, 4 TR sequences of ld complex words (two ping-pong
 // buffers of the f and g transforms), and with LACC TR rows of lacc words.
+// part: G = ceil(q/D) slices of ptotal words (unused when G = 1).
 template<typename T, int MULT, bool LACC>
 __global__ void __launch_bounds__(1024)
 batch_conv_kernel(DimArgs d, const BatchItem *items, int64_t count,
                   const cplx<T> *f, const cplx<T> *g, cplx<T> *out, T scale,
-                  int TR, int ld, int lacc)
+                  int TR, int ld, int lacc, int D, cplx<T> *part,
+                  int64_t ptotal)
 {
   extern __shared__ char smem_raw[];
   BatchItem *its=reinterpret_cast<BatchItem *>(smem_raw);
-  cplx<T> *A=reinterpret_cast<cplx<T> *>(its+TR);
+  cplx<T> *A=reinterpret_cast<cplx<T> *>(smem_raw+batch_items_bytes(TR));
   cplx<T> *B=A+(int64_t) 2*TR*ld;
   cplx<T> *acc=B+(int64_t) 2*TR*ld;
   const int m=(int) d.m;
   const int tid=threadIdx.x, nt=blockDim.x;
   const int64_t ntiles=(count+TR-1)/TR;
-  for(int64_t tile=blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  const int64_t G=(d.q+D-1)/D;
+  for(int64_t unit=blockIdx.x; unit < ntiles*G; unit += gridDim.x) {
+    const int64_t tile=unit/G, grp=unit-tile*G;
+    const int64_t rbeg=grp*D, rend=min(d.q,rbeg+D);
+    const T sc=G == 1 ? scale : (T) 1;
     const int64_t i0=tile*TR;
     const int nr=(int) min((int64_t) TR,count-i0);
     for(int i=tid; i < nr; i += nt) its[i]=items[i0+i];
     __syncthreads();
     int Lmax=0;
     for(int i=0; i < nr; ++i) Lmax=max(Lmax,(int) its[i].L);
-    for(int64_t r=0; r < d.q; ++r) {
+    for(int64_t r=rbeg; r < rend; ++r) {
       // folded, twiddled sequences: f of item i at row i, g at row nr + i
       for(int idx=tid; idx < 2*nr*m; idx += nt) {
         const int seq=idx/m, sigma=idx-seq*m;
@@ -88,25 +104,43 @@ batch_conv_kernel(DimArgs d, const BatchItem *items, int64_t count,
         __syncthreads();
       }
       cplx<T> *Z=lds_fft<T>(X,Y,nr,ld,d,-1);
-      const bool first=r == 0, last=r == d.q-1;
+      const bool first=r == rbeg, last=r == rend-1;
       for(int idx=tid; idx < nr*Lmax; idx += nt) {
         const int i=idx/Lmax, j=idx-i*Lmax;
         const BatchItem it=its[i];
         if(j >= it.L) continue;
         cplx<T> v=Z[(int64_t) i*ld+(j % m)];
         if(r != 0) v=mulc(v,zeta<T>(d,(int64_t) j*r));
+        cplx<T> *dst=G == 1 ? out+it.off+j : part+grp*ptotal+it.poff+j;
         if(LACC) {
           cplx<T> *a=acc+(int64_t) i*lacc+j;
-          const cplx<T> s=first ? v : *a+v;
-          if(last) out[it.off+j]=scale*s;
-          else *a=s;
+          const cplx<T> sum=first ? v : *a+v;
+          if(last) *dst=sc*sum;
+          else *a=sum;
         } else {
-          cplx<T> *o=out+it.off+j;
-          const cplx<T> s=first ? v : *o+v;
-          *o=last ? scale*s : s;
+          const cplx<T> sum=first ? v : *dst+v;
+          *dst=last ? sc*sum : sum;
         }
       }
       __syncthreads();
+    }
+  }
+}
+
+// out[off + j] = scale * sum_{g < G} part[g ptotal + poff + j], one block
+// per item (grid-stride), groups added in order.
+template<typename T>
+__global__ void __launch_bounds__(256)
+batch_reduce_kernel(const BatchItem *items, int64_t count,
+                    const cplx<T> *part, int64_t ptotal, int G,
+                    cplx<T> *out, T scale)
+{
+  for(int64_t i=blockIdx.x; i < count; i += gridDim.x) {
+    const BatchItem it=items[i];
+    for(int64_t j=threadIdx.x; j < it.L; j += blockDim.x) {
+      cplx<T> sum=part[it.poff+j];
+      for(int g=1; g < G; ++g) sum=sum+part[g*ptotal+it.poff+j];
+      out[it.off+j]=scale*sum;
     }
   }
 }

@@ -73,7 +73,7 @@ size_t residues(size_t L, size_t M, size_t m)
 }
 
 struct Tile {
-  size_t TR=1, threads=256, nblocks=1, lacc=0;
+  size_t TR=1, threads=256, nblocks=1, lacc=0, D=1;
 };
 
 struct Cls {
@@ -81,9 +81,11 @@ struct Cls {
   fftwpp_amd_dim *dim=nullptr;
   std::vector<size_t> ids;      // items, longest first
   BatchItem *items=nullptr;     // device copy
-  size_t Lmax=0;
+  size_t Lmax=0, ptotal=0;      // longest item, sum of lengths
+  void *part=nullptr;           // partial sums when the tile splits residues
   Tile tile;
   double work=0;                // N log2 N summed over items
+  double ms=0;                  // tuned launch time
 };
 
 struct Graph {
@@ -113,10 +115,14 @@ namespace {
 void free_class(Cls &c)
 {
   if(c.items) (void) hipFree(c.items);
+  if(c.part) (void) hipFree(c.part);
   if(c.dim) fftwpp_amd_dim_destroy(c.dim);
   c.items=nullptr;
+  c.part=nullptr;
   c.dim=nullptr;
 }
+
+size_t groups(const Cls &c, const Tile &t) { return ceilquot(c.q,t.D); }
 
 // Group items by (m, q) for the sub-FFT sizes `mof`; creates twiddle tables
 // and device item lists.
@@ -149,9 +155,12 @@ int build_classes(const fftwpp_amd_batch &h, const std::vector<size_t> &ids,
     int rc=fftwpp_amd_dim_create(&c.dim,c.Lmax,c.m,c.q,FFTWPP_AMD_KIND_COMPLEX);
     if(rc) return rc;
     std::vector<BatchItem> host(c.ids.size());
+    c.ptotal=0;
     for(size_t k=0; k < c.ids.size(); ++k) {
       host[k].off=(int64_t) h.off[c.ids[k]];
       host[k].L=(int64_t) h.L[c.ids[k]];
+      host[k].poff=(int64_t) c.ptotal;
+      c.ptotal += h.L[c.ids[k]];
     }
     rc=bhip(hipMalloc(&c.items,host.size()*sizeof(BatchItem)),"hipMalloc batch items");
     if(rc) return rc;
@@ -162,17 +171,30 @@ int build_classes(const fftwpp_amd_batch &h, const std::vector<size_t> &ids,
   return FFTWPP_AMD_OK;
 }
 
-// Largest LDS accumulation that fits with TR items, else 0.
+// Tile of TR items and D residues per work unit, with LDS accumulation
+// when it fits.
 Tile make_tile(const fftwpp_amd_batch &h, const Cls &c, size_t TR,
-               size_t threads)
+               size_t threads, size_t D)
 {
   Tile t;
   t.TR=TR;
   t.threads=threads;
+  t.D=std::min(D,c.q);
   t.lacc=fftwpp_amd_batch_rows_lds(c.dim,h.dtype,TR,c.Lmax) ? c.Lmax : 0;
-  size_t tiles=ceilquot(c.ids.size(),TR);
-  t.nblocks=std::min(tiles,(size_t) (8*h.cus));
+  size_t units=ceilquot(c.ids.size(),TR)*groups(c,t);
+  t.nblocks=std::min(units,(size_t) (8*h.cus));
   return t;
+}
+
+// Residue group sizes tried: q, q/2, q/4, q/8 (rounded up), at most 8 groups.
+std::vector<size_t> group_sizes(size_t q)
+{
+  std::vector<size_t> v;
+  for(size_t G=1; G <= 8 && G <= q; G *= 2) {
+    size_t D=ceilquot(q,G);
+    if(std::find(v.begin(),v.end(),D) == v.end()) v.push_back(D);
+  }
+  return v;
 }
 
 bool tile_fits(const fftwpp_amd_batch &h, const Cls &c, size_t TR)
@@ -180,33 +202,48 @@ bool tile_fits(const fftwpp_amd_batch &h, const Cls &c, size_t TR)
   return fftwpp_amd_batch_rows_lds(c.dim,h.dtype,TR,0) != 0;
 }
 
+// Without tuning: small tiles of short items, and enough residue groups
+// for about two work units per compute unit.
 Tile default_tile(const fftwpp_amd_batch &h, const Cls &c)
 {
   size_t TR=std::max<size_t>(1,std::min<size_t>(32,2048/c.m));
   while(TR > 1 && !tile_fits(h,c,TR)) TR /= 2;
   size_t threads=std::min<size_t>(256,std::max<size_t>(64,TR*c.m/4));
   threads=(threads+63)/64*64;
-  return make_tile(h,c,TR,threads);
+  size_t tiles=ceilquot(c.ids.size(),TR);
+  size_t G=std::min<size_t>({8,c.q,std::max<size_t>(1,ceilquot(2*(size_t) h.cus,tiles))});
+  return make_tile(h,c,TR,threads,ceilquot(c.q,G));
 }
 
 int launch_class(const fftwpp_amd_batch &h, const Cls &c, const Tile &t,
-                 void *out, const void *f, const void *g, hipStream_t s)
+                 void *part, void *out, const void *f, const void *g,
+                 hipStream_t s)
 {
   return fftwpp_amd_batch_rows(c.dim,h.dtype,h.mult,c.items,c.ids.size(),f,g,out,
-                               t.TR,t.threads,t.nblocks,t.lacc,
+                               t.TR,t.threads,t.nblocks,t.lacc,t.D,part,c.ptotal,
                                1.0/(double) (c.m*c.q),s);
+}
+
+// Partial-sum buffer for the class's chosen tile (none when G = 1).
+int alloc_part(const fftwpp_amd_batch &h, Cls &c)
+{
+  if(c.part) { (void) hipFree(c.part); c.part=nullptr; }
+  size_t G=groups(c,c.tile);
+  if(G <= 1) return FFTWPP_AMD_OK;
+  size_t elt=h.dtype == FFTWPP_AMD_F32 ? 8 : 16;
+  return bhip(hipMalloc(&c.part,G*c.ptotal*elt),"hipMalloc partial sums");
 }
 
 // Average time of one launch (ms), or a negative value on failure.
 double time_launch(const fftwpp_amd_batch &h, const Cls &c, const Tile &t,
-                   void *out, const void *f, const void *g, hipStream_t s,
-                   hipEvent_t e0, hipEvent_t e1)
+                   void *part, void *out, const void *f, const void *g,
+                   hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
-  if(launch_class(h,c,t,out,f,g,s)) return -1;
+  if(launch_class(h,c,t,part,out,f,g,s)) return -1;
   const int reps=3;
   if(hipEventRecord(e0,s) != hipSuccess) return -1;
   for(int k=0; k < reps; ++k)
-    if(launch_class(h,c,t,out,f,g,s)) return -1;
+    if(launch_class(h,c,t,part,out,f,g,s)) return -1;
   if(hipEventRecord(e1,s) != hipSuccess) return -1;
   if(hipEventSynchronize(e1) != hipSuccess) return -1;
   float ms=0;
@@ -214,25 +251,31 @@ double time_launch(const fftwpp_amd_batch &h, const Cls &c, const Tile &t,
   return ms/reps;
 }
 
-// Best tile of a class by timing the candidates; returns its time.
-double tune_class(const fftwpp_amd_batch &h, Cls &c, void *out, const void *f,
-                  const void *g, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+// Best tile of a class by timing the candidates (items per block, threads,
+// residues per work unit); part holds up to 8 partial-sum slices.  Returns
+// the best time.
+double tune_class(const fftwpp_amd_batch &h, Cls &c, void *part, void *out,
+                  const void *f, const void *g, hipStream_t s, hipEvent_t e0,
+                  hipEvent_t e1)
 {
   double best=-1;
   Tile bt=default_tile(h,c);
-  for(size_t TR=1; TR <= 64; TR *= 2) {
-    if(TR > 1 && TR*c.m > 16384) break;
-    if(TR > 1 && TR > c.ids.size()) break;
-    if(!tile_fits(h,c,TR)) break;
-    for(size_t threads=64; threads <= 512; threads *= 2) {
-      if(threads > 64 && threads > TR*c.m/2) break;
-      Tile t=make_tile(h,c,TR,threads);
-      double ms=time_launch(h,c,t,out,f,g,s,e0,e1);
-      if(ms < 0) continue;
-      if(best < 0 || ms < best) { best=ms; bt=t; }
+  for(size_t D : group_sizes(c.q)) {
+    for(size_t TR=1; TR <= 64; TR *= 2) {
+      if(TR > 1 && TR*c.m > 16384) break;
+      if(TR > 1 && TR > c.ids.size()) break;
+      if(!tile_fits(h,c,TR)) break;
+      for(size_t threads=64; threads <= 512; threads *= 2) {
+        if(threads > 64 && threads > TR*c.m/2) break;
+        Tile t=make_tile(h,c,TR,threads,D);
+        double ms=time_launch(h,c,t,part,out,f,g,s,e0,e1);
+        if(ms < 0) continue;
+        if(best < 0 || ms < best) { best=ms; bt=t; }
+      }
     }
   }
   c.tile=bt;
+  c.ms=best;
   return best;
 }
 
@@ -245,7 +288,8 @@ int run_classes(fftwpp_amd_batch &h, void *out, const void *f, const void *g,
   for(int k=0; k < NSIDE; ++k)
     if((rc=bhip(hipStreamWaitEvent(h.side[k],h.fork,0),"wait fork"))) return rc;
   for(size_t i=0; i < h.cls.size(); ++i)
-    if((rc=launch_class(h,h.cls[i],h.cls[i].tile,out,f,g,h.side[i % NSIDE]))) return rc;
+    if((rc=launch_class(h,h.cls[i],h.cls[i].tile,h.cls[i].part,out,f,g,
+                        h.side[i % NSIDE]))) return rc;
   for(int k=0; k < NSIDE; ++k) {
     if((rc=bhip(hipEventRecord(h.join[k],h.side[k]),"record join"))) return rc;
     if((rc=bhip(hipStreamWaitEvent(s,h.join[k],0),"wait join"))) return rc;
@@ -284,24 +328,45 @@ std::vector<size_t> candidates(size_t Lmax, size_t Mmax, size_t maxm)
   return c;
 }
 
-int tune(fftwpp_amd_batch &h)
-{
-  const size_t elt=h.dtype == FFTWPP_AMD_F32 ? 8 : 16;
+// Scratch arrays for timing candidates: f, g, out and 8 partial-sum slices
+// of the whole batch, zero-filled.
+struct TuneArrays {
   void *buf=nullptr;
-  int rc=bhip(hipMalloc(&buf,3*h.total*elt),"hipMalloc tuning arrays");
-  if(rc) return rc;
-  char *b=(char *) buf;
-  void *f=b, *g=b+h.total*elt, *out=b+2*h.total*elt;
+  void *f=nullptr, *g=nullptr, *out=nullptr, *part=nullptr;
   hipEvent_t e0=nullptr, e1=nullptr;
-  rc=bhip(hipMemsetAsync(buf,0,3*h.total*elt,h.cap),"clear tuning arrays");
-  if(!rc) rc=bhip(hipEventCreate(&e0),"event");
-  if(!rc) rc=bhip(hipEventCreate(&e1),"event");
+  int init(const fftwpp_amd_batch &h) {
+    const size_t elt=h.dtype == FFTWPP_AMD_F32 ? 8 : 16;
+    const size_t bytes=11*h.total*elt;
+    int rc=bhip(hipMalloc(&buf,bytes),"hipMalloc tuning arrays");
+    if(rc) return rc;
+    char *b=(char *) buf;
+    f=b; g=b+h.total*elt; out=b+2*h.total*elt; part=b+3*h.total*elt;
+    rc=bhip(hipMemsetAsync(buf,0,bytes,h.cap),"clear tuning arrays");
+    if(!rc) rc=bhip(hipEventCreate(&e0),"event");
+    if(!rc) rc=bhip(hipEventCreate(&e1),"event");
+    return rc;
+  }
+  double time(const fftwpp_amd_batch &h, Cls &c) {
+    return tune_class(h,c,part,out,f,g,h.cap,e0,e1);
+  }
+  void release(const fftwpp_amd_batch &h) {
+    if(e0) (void) hipEventDestroy(e0);
+    if(e1) (void) hipEventDestroy(e1);
+    (void) hipStreamSynchronize(h.cap);
+    if(buf) (void) hipFree(buf);
+    buf=nullptr; e0=e1=nullptr;
+  }
+};
+
+// Sub-FFT size per power-of-two length bucket: the candidate whose classes
+// (each with its best tile) take the least total time.
+int tune_geometry(fftwpp_amd_batch &h, TuneArrays &ta)
+{
   const size_t maxm=fftwpp_amd_max_m(h.dtype);
-  // power-of-two length buckets
   std::map<size_t,std::vector<size_t>> buckets;
-  for(size_t i=0; i < h.n && !rc; ++i) buckets[ceilpow2(h.L[i])].push_back(i);
+  for(size_t i=0; i < h.n; ++i) buckets[ceilpow2(h.L[i])].push_back(i);
+  int rc=FFTWPP_AMD_OK;
   for(auto &kv : buckets) {
-    if(rc) break;
     const std::vector<size_t> &ids=kv.second;
     size_t Lmax=0, Mmax=0;
     for(size_t i : ids) { Lmax=std::max(Lmax,h.L[i]); Mmax=std::max(Mmax,h.M[i]); }
@@ -315,21 +380,17 @@ int tune(fftwpp_amd_batch &h)
       double total=0;
       for(Cls &c : cls) {
         if(rc) break;
-        double t=tune_class(h,c,out,f,g,h.cap,e0,e1);
+        double t=ta.time(h,c);
         if(t < 0) { total=-1; break; }
         total += t;
       }
       for(Cls &c : cls) free_class(c);
-      if(rc) break;
+      if(rc) return rc;
       if(total >= 0 && (best < 0 || total < best)) { best=total; bestm=m; }
     }
-    if(!rc && bestm == 0) rc=bfail(FFTWPP_AMD_ERR_HIP,"batch tuning found no working launch");
+    if(bestm == 0) return bfail(FFTWPP_AMD_ERR_HIP,"batch tuning found no working launch");
     for(size_t i : ids) h.mof[i]=bestm;
   }
-  if(e0) (void) hipEventDestroy(e0);
-  if(e1) (void) hipEventDestroy(e1);
-  (void) hipStreamSynchronize(h.cap);
-  (void) hipFree(buf);
   return rc;
 }
 
@@ -388,36 +449,24 @@ int fftwpp_amd_batch_create(fftwpp_amd_batch **out, size_t n, const size_t *L,
   const size_t maxm=fftwpp_amd_max_m(dtype);
   h->mof.resize(n);
   for(size_t i=0; i < n; ++i) h->mof[i]=std::min(ceilpow2(h->L[i]),maxm);
-  if(flags & FFTWPP_AMD_BATCH_TUNE) {
-    rc=tune(*h);
-    if(rc) { destroy(h); return rc; }
+  const bool tuned=flags & FFTWPP_AMD_BATCH_TUNE;
+  TuneArrays ta;
+  if(tuned) {
+    rc=ta.init(*h);
+    if(!rc) rc=tune_geometry(*h,ta);
+    if(rc) { ta.release(*h); destroy(h); return rc; }
   }
   std::vector<size_t> all(n);
   for(size_t i=0; i < n; ++i) all[i]=i;
   rc=build_classes(*h,all,h->mof,h->cls);
-  if(rc) { destroy(h); return rc; }
   std::sort(h->cls.begin(),h->cls.end(),[](const Cls &a, const Cls &b) { return a.work > b.work; });
-  if(flags & FFTWPP_AMD_BATCH_TUNE) {
-    const size_t elt=dtype == FFTWPP_AMD_F32 ? 8 : 16;
-    void *buf=nullptr;
-    hipEvent_t e0=nullptr, e1=nullptr;
-    rc=bhip(hipMalloc(&buf,3*h->total*elt),"hipMalloc tuning arrays");
-    if(!rc) rc=bhip(hipMemsetAsync(buf,0,3*h->total*elt,h->cap),"clear tuning arrays");
-    if(!rc) rc=bhip(hipEventCreate(&e0),"event");
-    if(!rc) rc=bhip(hipEventCreate(&e1),"event");
-    char *b=(char *) buf;
-    for(Cls &c : h->cls) {
-      if(rc) break;
-      if(tune_class(*h,c,b+2*h->total*elt,b,b+h->total*elt,h->cap,e0,e1) < 0)
-        rc=bfail(FFTWPP_AMD_ERR_HIP,"batch tuning found no working launch");
-    }
-    if(e0) (void) hipEventDestroy(e0);
-    if(e1) (void) hipEventDestroy(e1);
-    (void) hipStreamSynchronize(h->cap);
-    if(buf) (void) hipFree(buf);
-  } else {
-    for(Cls &c : h->cls) c.tile=default_tile(*h,c);
+  for(Cls &c : h->cls) {
+    if(rc) break;
+    if(!tuned) c.tile=default_tile(*h,c);
+    else if(ta.time(*h,c) < 0) rc=bfail(FFTWPP_AMD_ERR_HIP,"batch tuning found no working launch");
+    if(!rc) rc=alloc_part(*h,c);
   }
+  if(tuned) ta.release(*h);
   if(rc) { destroy(h); return rc; }
   *out=h;
   return FFTWPP_AMD_OK;
@@ -460,8 +509,10 @@ int fftwpp_amd_batch_describe(const fftwpp_amd_batch *h, char *buf, size_t len)
   std::string s;
   char line[160];
   for(const Cls &c : h->cls) {
-    snprintf(line,sizeof(line),"m=%zu q=%zu items=%zu Lmax=%zu TR=%zu threads=%zu blocks=%zu lacc=%zu\n",
-             c.m,c.q,c.ids.size(),c.Lmax,c.tile.TR,c.tile.threads,c.tile.nblocks,c.tile.lacc);
+    snprintf(line,sizeof(line),
+             "m=%zu q=%zu items=%zu Lmax=%zu TR=%zu threads=%zu D=%zu blocks=%zu lacc=%zu ms=%.4f\n",
+             c.m,c.q,c.ids.size(),c.Lmax,c.tile.TR,c.tile.threads,c.tile.D,c.tile.nblocks,
+             c.tile.lacc,c.ms);
     s += line;
   }
   size_t k=std::min(len-1,s.size());

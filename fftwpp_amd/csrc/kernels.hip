@@ -555,53 +555,65 @@ static size_t batch_lds(size_t m, size_t elt, size_t TR, size_t lacc,
                         size_t *ldout)
 {
   size_t ld=m+1;
-  const size_t base=TR*sizeof(BatchItem);
+  const size_t base=batch_items_bytes(TR);
   if(base+(4*TR*ld+TR*lacc)*elt > lds_max()) ld=m;
   if(ldout) *ldout=ld;
   return base+(4*TR*ld+TR*lacc)*elt;
 }
 
+struct BatchLaunch {
+  const BatchItem *items;
+  size_t count;
+  const void *f, *g;
+  void *out, *part;
+  size_t TR, threads, nblocks, lacc, D, ptotal, ld, lds;
+  double scale;
+};
+
 template<typename T, int MULT>
-static int batch_rows_launch(const DimArgs &a, const BatchItem *items,
-                             size_t count, const void *f, const void *g,
-                             void *out, size_t TR, size_t threads,
-                             size_t nblocks, size_t lacc, double scale,
-                             size_t ld, size_t lds, hipStream_t st)
+static int batch_rows_launch(const DimArgs &a, const BatchLaunch &b,
+                             hipStream_t st)
 {
   int rc;
-  if(lacc) {
-    if((rc=allow_lds(batch_conv_kernel<T,MULT,true>,lds))) return rc;
-    hipLaunchKernelGGL((batch_conv_kernel<T,MULT,true>),dim3((unsigned) nblocks),
-                       dim3((unsigned) threads),lds,st,a,items,(int64_t) count,
-                       (const cplx<T> *) f,(const cplx<T> *) g,(cplx<T> *) out,
-                       (T) scale,(int) TR,(int) ld,(int) lacc);
+  if(b.lacc) {
+    if((rc=allow_lds(batch_conv_kernel<T,MULT,true>,b.lds))) return rc;
+    hipLaunchKernelGGL((batch_conv_kernel<T,MULT,true>),dim3((unsigned) b.nblocks),
+                       dim3((unsigned) b.threads),b.lds,st,a,b.items,(int64_t) b.count,
+                       (const cplx<T> *) b.f,(const cplx<T> *) b.g,(cplx<T> *) b.out,
+                       (T) b.scale,(int) b.TR,(int) b.ld,(int) b.lacc,(int) b.D,
+                       (cplx<T> *) b.part,(int64_t) b.ptotal);
   } else {
-    if((rc=allow_lds(batch_conv_kernel<T,MULT,false>,lds))) return rc;
-    hipLaunchKernelGGL((batch_conv_kernel<T,MULT,false>),dim3((unsigned) nblocks),
-                       dim3((unsigned) threads),lds,st,a,items,(int64_t) count,
-                       (const cplx<T> *) f,(const cplx<T> *) g,(cplx<T> *) out,
-                       (T) scale,(int) TR,(int) ld,0);
+    if((rc=allow_lds(batch_conv_kernel<T,MULT,false>,b.lds))) return rc;
+    hipLaunchKernelGGL((batch_conv_kernel<T,MULT,false>),dim3((unsigned) b.nblocks),
+                       dim3((unsigned) b.threads),b.lds,st,a,b.items,(int64_t) b.count,
+                       (const cplx<T> *) b.f,(const cplx<T> *) b.g,(cplx<T> *) b.out,
+                       (T) b.scale,(int) b.TR,(int) b.ld,0,(int) b.D,
+                       (cplx<T> *) b.part,(int64_t) b.ptotal);
   }
-  return launch_check("batch_conv_kernel");
+  rc=launch_check("batch_conv_kernel");
+  if(rc) return rc;
+  const size_t G=(a.q+b.D-1)/b.D;
+  if(G > 1) {
+    const size_t nb=b.count < 4096 ? b.count : 4096;
+    hipLaunchKernelGGL(batch_reduce_kernel<T>,dim3((unsigned) nb),dim3(256),0,st,
+                       b.items,(int64_t) b.count,(const cplx<T> *) b.part,
+                       (int64_t) b.ptotal,(int) G,(cplx<T> *) b.out,(T) b.scale);
+    rc=launch_check("batch_reduce_kernel");
+  }
+  return rc;
 }
 
 template<typename T>
-static int batch_rows_t(const fftwpp_amd_dim *d, int mult, const void *items,
-                        size_t count, const void *f, const void *g, void *out,
-                        size_t TR, size_t threads, size_t nblocks, size_t lacc,
-                        double scale, hipStream_t st)
+static int batch_rows_t(const fftwpp_amd_dim *d, int mult, BatchLaunch b,
+                        hipStream_t st)
 {
   const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
-  size_t ld;
-  const size_t lds=batch_lds(d->m,sizeof(cplx<T>),TR,lacc,&ld);
-  if(lds > lds_max())
+  b.lds=batch_lds(d->m,sizeof(cplx<T>),b.TR,b.lacc,&b.ld);
+  if(b.lds > lds_max())
     return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"batch tile does not fit in LDS");
-  const BatchItem *it=(const BatchItem *) items;
-  if(mult == FFTWPP_AMD_MULT_CORRELATION)
-    return batch_rows_launch<T,3>(a,it,count,f,g,out,TR,threads,nblocks,lacc,scale,ld,lds,st);
-  if(mult == FFTWPP_AMD_MULT_NONE)
-    return batch_rows_launch<T,0>(a,it,count,f,g,out,TR,threads,nblocks,lacc,scale,ld,lds,st);
-  return batch_rows_launch<T,1>(a,it,count,f,g,out,TR,threads,nblocks,lacc,scale,ld,lds,st);
+  if(mult == FFTWPP_AMD_MULT_CORRELATION) return batch_rows_launch<T,3>(a,b,st);
+  if(mult == FFTWPP_AMD_MULT_NONE) return batch_rows_launch<T,0>(a,b,st);
+  return batch_rows_launch<T,1>(a,b,st);
 }
 
 template<typename T>
@@ -844,10 +856,12 @@ size_t fftwpp_amd_batch_rows_lds(const fftwpp_amd_dim *d, int dtype,
 int fftwpp_amd_batch_rows(const fftwpp_amd_dim *d, int dtype, int mult,
                           const void *items, size_t count, const void *f,
                           const void *g, void *out, size_t TR, size_t threads,
-                          size_t nblocks, size_t lacc, double scale,
-                          void *stream)
+                          size_t nblocks, size_t lacc, size_t D, void *part,
+                          size_t ptotal, double scale, void *stream)
 {
   if(!d || !items || !f || !g || !out) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  if(D == 0) return fail(FFTWPP_AMD_ERR_ARG,"residue group size must be positive");
+  if(D < d->q && !part) return fail(FFTWPP_AMD_ERR_ARG,"residue groups need a partial-sum buffer");
   if(count == 0) return FFTWPP_AMD_OK;
   if(d->kind != FFTWPP_AMD_KIND_COMPLEX)
     return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"batched rows are non-centred complex");
@@ -858,9 +872,12 @@ int fftwpp_amd_batch_rows(const fftwpp_amd_dim *d, int dtype, int mult,
      threads > 1024 || threads % 64 != 0)
     return fail(FFTWPP_AMD_ERR_ARG,"bad tile, block or thread count");
   hipStream_t st=(hipStream_t) stream;
-  return dtype == FFTWPP_AMD_F32 ?
-    batch_rows_t<float>(d,mult,items,count,f,g,out,TR,threads,nblocks,lacc,scale,st) :
-    batch_rows_t<double>(d,mult,items,count,f,g,out,TR,threads,nblocks,lacc,scale,st);
+  BatchLaunch b;
+  b.items=(const BatchItem *) items; b.count=count; b.f=f; b.g=g; b.out=out;
+  b.part=part; b.TR=TR; b.threads=threads; b.nblocks=nblocks; b.lacc=lacc;
+  b.D=D < d->q ? D : d->q; b.ptotal=ptotal; b.ld=0; b.lds=0; b.scale=scale;
+  return dtype == FFTWPP_AMD_F32 ? batch_rows_t<float>(d,mult,b,st) :
+    batch_rows_t<double>(d,mult,b,st);
 }
 
 int fftwpp_amd_multiply(int dtype, int mult, void *F0, const void *F1,

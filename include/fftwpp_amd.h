@@ -103,12 +103,15 @@ int fftwpp_amd_conv_rows(const fftwpp_amd_dim *d, int dtype, int mult,
                          double scale, void *stream);
 
 /* Fused convolutions of `count` independent items that share the geometry
- * (m, q) of d.  items is a device array of {int64 off, int64 L} pairs: item
- * i convolves f[off..off+L) with g[off..off+L) into out[off..off+L), which
- * is exact when N = m q >= 2L - 1.  Tiles of TR items per block, `threads`
- * per block (a multiple of 64, at most 1024), `nblocks` blocks looping over
- * the tiles; lacc > 0 accumulates the residues in LDS rows of lacc >= max L
- * words, lacc = 0 in out.  Each item follows the residue loop of
+ * (m, q) of d.  items is a device array of {int64 off, int64 L, int64 poff}
+ * triples: item i convolves f[off..off+L) with g[off..off+L) into
+ * out[off..off+L), which is exact when N = m q >= 2L - 1.  Tiles of TR items
+ * per block, `threads` per block (a multiple of 64, at most 1024), `nblocks`
+ * blocks looping over the work units; lacc > 0 accumulates the residues in
+ * LDS rows of lacc >= max L words, lacc = 0 in the destination.  D residues
+ * per work unit: D >= q writes out directly; D < q writes G = ceil(q/D)
+ * partial sums into part (G slices of ptotal words, item i at poff) and adds
+ * them into out in a second launch.  Each item follows the residue loop of
  * Convolution::convolveRaw (convolve.cc:7513-7575); the batching itself has
  * no reference counterpart.  fftwpp_amd_batch_rows_lds gives the dynamic
  * LDS bytes of a tile, 0 when it does not fit. */
@@ -117,8 +120,8 @@ size_t fftwpp_amd_batch_rows_lds(const fftwpp_amd_dim *d, int dtype,
 int fftwpp_amd_batch_rows(const fftwpp_amd_dim *d, int dtype, int mult,
                           const void *items, size_t count, const void *f,
                           const void *g, void *out, size_t TR, size_t threads,
-                          size_t nblocks, size_t lacc, double scale,
-                          void *stream);
+                          size_t nblocks, size_t lacc, size_t D, void *part,
+                          size_t ptotal, double scale, void *stream);
 
 /* The multipliers on device arrays (convolve.cc:26-110): n counts complex
  * words for multBinary/multcorrelation and real words for realMultBinary. */
