@@ -24,6 +24,7 @@ SIGNATURES = {
     "fftwpp_amd_backward_many": (I, [P, I, P, Z, Z, P, Z, Z, Z, Z, Z, Z, I, D, P]),
     "fftwpp_amd_conv_rows": (I, [P, I, I, P, P, Z, Z, P, Z, D, P]),
     "fftwpp_amd_batch_rows_lds": (Z, [P, I, Z, Z]),
+    "fftwpp_amd_batch_rows_tps": (Z, [P, I]),
     "fftwpp_amd_batch_rows": (I, [P, I, I, P, Z, P, P, P, Z, Z, Z, Z, Z, P, Z, D, P]),
     "fftwpp_amd_multiply": (I, [I, I, P, P, Z, P]),
     "fftwpp_amd_scale": (I, [I, P, Z, D, P]),

@@ -24,6 +24,7 @@
 #pragma once
 #include "common.h"
 #include "fft_lds.h"
+#include "fftr.h"
 
 namespace fftwpp_amd {
 
@@ -123,6 +124,96 @@ batch_conv_kernel(DimArgs d, const BatchItem *items, int64_t count,
         }
       }
       __syncthreads();
+    }
+  }
+}
+
+// Values per thread of the register FFT: 16 in FP32, 8 in FP64 (two
+// transforms and the FFT's temporaries then fit two waves per SIMD).
+template<typename T> struct BatchV { static constexpr int V=sizeof(T) == 8 ? 8 : 16; };
+
+// The same work units with the register-resident FFT (fftr.h) of a
+// power-of-two sub-FFT size M: a block has blockDim.x/TPS sequence slots of
+// TPS = M/V threads and each slot convolves one work unit (item, residue
+// group) at a time; the sums over residues accumulate in the unit's
+// destination (thread-private words: thread t owns j = t + TPS k + M b).
+// When TPS > 64 the host launches one slot per block, so the block barriers
+// inside fftr() see the same calls from every thread.  Dynamic LDS: M
+// complex twiddles + slots * RPlan<M,V>::PAD scalar words.
+template<typename T, int M>
+__global__ void __launch_bounds__(512)
+batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
+                 const cplx<T> *f, const cplx<T> *g, cplx<T> *out, T scale,
+                 int mult, int D, cplx<T> *part, int64_t ptotal)
+{
+  constexpr int V=BatchV<T>::V;
+  typedef RPlan<M,V> P;
+  constexpr int TPS=P::TPS;
+  extern __shared__ char smem_raw[];
+  cplx<T> *tw=reinterpret_cast<cplx<T> *>(smem_raw);
+  const cplx<T> *zN=(const cplx<T> *) d.zN;
+  fill_twr<T,M,V>(tw,zN,(int) d.q,threadIdx.x,blockDim.x);
+  __syncthreads();
+  const int slots=blockDim.x/TPS;
+  const int slot=threadIdx.x/TPS, t=threadIdx.x-slot*TPS;
+  T *xb=reinterpret_cast<T *>(tw+M)+slot*P::PAD;
+  const int64_t G=(d.q+D-1)/D;
+  const int64_t units=count*G;
+  const int N=(int) d.N;
+  for(int64_t ub=(int64_t) blockIdx.x*slots; ub < units; ub += (int64_t) gridDim.x*slots) {
+    const int64_t unit=ub+slot;
+    if(unit >= units) break;            // only when slots > 1 (TPS <= 64)
+    const int64_t ii=unit/G, grp=unit-ii*G;
+    const BatchItem it=items[ii];
+    const int rbeg=(int) (grp*D), rend=(int) min(d.q,(int64_t) rbeg+D);
+    const T sc=G == 1 ? scale : (T) 1;
+    cplx<T> *dst=G == 1 ? out+it.off : part+grp*ptotal+it.poff;
+    const cplx<T> *fi=f+it.off, *gi=g+it.off;
+    const int L=(int) it.L;
+    for(int r=rbeg; r < rend; ++r) {
+      // w_N^{j r} for j = sigma + M b: index sigma r + b (M r mod N) mod N
+      const int step=(M*r) % N;
+      cplx<T> X[V], Y[V];
+#pragma unroll
+      for(int k=0; k < V; ++k) X[k]=Y[k]=mk((T) 0,(T) 0);
+      // blocks b of M: element j = sigma + M b, twiddle index sigma r + eb
+      for(int jb=0, eb=0; jb < L; jb += M) {
+#pragma unroll
+        for(int k=0; k < V; ++k) {
+          const int j=jb+t+TPS*k;
+          if(j < L) {
+            int e=(t+TPS*k)*r+eb;
+            if(e >= N) e -= N;
+            const cplx<T> w=zN[e];
+            X[k]=X[k]+w*fi[j];
+            Y[k]=Y[k]+w*gi[j];
+          }
+        }
+        eb += step;
+        if(eb >= N) eb -= N;
+      }
+      fftr<T,M,V,1>(X,xb,tw,t);
+      fftr<T,M,V,1>(Y,xb,tw,t);
+#pragma unroll
+      for(int k=0; k < V; ++k)
+        X[k]=mult == 1 ? X[k]*Y[k] : (mult == 3 ? mulc(X[k],Y[k]) : X[k]);
+      fftr<T,M,V,-1>(X,xb,tw,t);
+      const bool first=r == rbeg, last=r == rend-1;
+      for(int jb=0, eb=0; jb < L; jb += M) {
+#pragma unroll
+        for(int k=0; k < V; ++k) {
+          const int j=jb+t+TPS*k;
+          if(j < L) {
+            int e=(t+TPS*k)*r+eb;
+            if(e >= N) e -= N;
+            cplx<T> v=mulc(X[k],zN[e]);
+            if(!first) v=v+dst[j];
+            dst[j]=last ? sc*v : v;
+          }
+        }
+        eb += step;
+        if(eb >= N) eb -= N;
+      }
     }
   }
 }

@@ -96,6 +96,9 @@ struct Graph {
 
 const int NSIDE=4;
 
+// Largest sub-FFT size: the register-resident FFT of the batch kernel.
+const size_t BATCH_MAXM=4096;
+
 } // namespace
 
 struct fftwpp_amd_batch {
@@ -182,7 +185,11 @@ Tile make_tile(const fftwpp_amd_batch &h, const Cls &c, size_t TR,
   t.D=std::min(D,c.q);
   t.lacc=fftwpp_amd_batch_rows_lds(c.dim,h.dtype,TR,c.Lmax) ? c.Lmax : 0;
   size_t units=ceilquot(c.ids.size(),TR)*groups(c,t);
-  t.nblocks=std::min(units,(size_t) (8*h.cus));
+  // the register path runs threads/tps work units per block at once
+  size_t tps=fftwpp_amd_batch_rows_tps(c.dim,h.dtype);
+  size_t slots=tps ? std::max<size_t>(1,std::max(threads,tps)/tps) : 1;
+  if(tps > 64) slots=1;
+  t.nblocks=std::min(ceilquot(units,slots),(size_t) (8*h.cus));
   return t;
 }
 
@@ -206,6 +213,13 @@ bool tile_fits(const fftwpp_amd_batch &h, const Cls &c, size_t TR)
 // for about two work units per compute unit.
 Tile default_tile(const fftwpp_amd_batch &h, const Cls &c)
 {
+  if(size_t tps=fftwpp_amd_batch_rows_tps(c.dim,h.dtype)) {
+    size_t threads=std::max<size_t>(256,tps);
+    size_t slots=tps > 64 ? 1 : threads/tps;
+    size_t G=std::min<size_t>({8,c.q,std::max<size_t>(1,ceilquot(2*(size_t) h.cus*slots,
+                                                                  c.ids.size()))});
+    return make_tile(h,c,1,threads,ceilquot(c.q,G));
+  }
   size_t TR=std::max<size_t>(1,std::min<size_t>(32,2048/c.m));
   while(TR > 1 && !tile_fits(h,c,TR)) TR /= 2;
   size_t threads=std::min<size_t>(256,std::max<size_t>(64,TR*c.m/4));
@@ -362,7 +376,7 @@ struct TuneArrays {
 // (each with its best tile) take the least total time.
 int tune_geometry(fftwpp_amd_batch &h, TuneArrays &ta)
 {
-  const size_t maxm=fftwpp_amd_max_m(h.dtype);
+  const size_t maxm=BATCH_MAXM;
   std::map<size_t,std::vector<size_t>> buckets;
   for(size_t i=0; i < h.n; ++i) buckets[ceilpow2(h.L[i])].push_back(i);
   int rc=FFTWPP_AMD_OK;
@@ -446,7 +460,7 @@ int fftwpp_amd_batch_create(fftwpp_amd_batch **out, size_t n, const size_t *L,
   }
   if(!rc) rc=bhip(hipEventCreateWithFlags(&h->fork,hipEventDisableTiming),"event");
   if(rc) { destroy(h); return rc; }
-  const size_t maxm=fftwpp_amd_max_m(dtype);
+  const size_t maxm=BATCH_MAXM;
   h->mof.resize(n);
   for(size_t i=0; i < n; ++i) h->mof[i]=std::min(ceilpow2(h->L[i]),maxm);
   const bool tuned=flags & FFTWPP_AMD_BATCH_TUNE;

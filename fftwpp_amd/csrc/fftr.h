@@ -1,0 +1,152 @@
+// fftr.h -- register-resident Stockham FFT of a compile-time power-of-two
+// length M (16 <= M <= 4096): the sub-FFT layer (the role of FFTW's
+// mfft1d in the reference, fftw++.h:576-896) for the fused kernels whose
+// sub-FFT size is not 256.
+//
+// TPS = M/V threads hold one sequence, thread t keeping x[t + TPS k] in
+// v[k], k < V (V = 16, or 8 to halve the registers of FP64 kernels), before
+// and after the transform.  The stages are radix V (and one final smaller
+// power of two when log2 M is not a multiple of log2 V); stage s with radix
+// R and Ns = (product of the earlier radices) does, for each butterfly
+// j = t + TPS b (b < V/R) with k' = j mod Ns,
+//     u_t' = v[b + t' V/R] w^{t' k'},   w = e^{S 2 pi i/(Ns R)},
+//     out[(j/Ns) Ns R + k' + u Ns] = sum_t' u_t' e^{S 2 pi i u t'/R},
+// in place in registers; between stages the outputs go through LDS (real
+// and imaginary parts in two rounds, scalar words at e + e/16, which keeps
+// every access of the gfx950 LDS bank-conflict free) and come back as
+// v[k] = out[t + TPS k].  After the last stage v[k] holds X[t + TPS k].
+//
+// Twiddles: one LDS table of M-1 complex words, stage s at offset off_s
+// holding e^{2 pi i t' k'/(Ns R)} at off_s + (t'-1) Ns + k', so the threads
+// of a stage read consecutive words; fill_twr() builds it from the
+// dimension's FP64-generated table zN (N = M q): e^{2 pi i n/M} = zN[q n].
+#pragma once
+#include "common.h"
+#include "fft_lds.h"
+#include "fft256.h"
+
+namespace fftwpp_amd {
+
+template<int M, int V=16> struct RPlan {
+  static_assert(M >= 16 && M <= 4096 && (M & (M-1)) == 0, "M must be 2^k, 16..4096");
+  static_assert(V == 8 || V == 16, "8 or 16 values per thread");
+  static constexpr int TPS=M/V;
+  static constexpr int LOG=__builtin_ctz(M);
+  static constexpr int LV=__builtin_ctz(V);
+  static constexpr int NV=LOG/LV;                       // radix-V stages
+  static constexpr int LAST=1 << (LOG-LV*NV);           // remaining factor
+  static constexpr int NST=NV+(LAST > 1 ? 1 : 0);
+  static constexpr int PAD=M+M/16;                      // exchange words
+  __host__ __device__ static constexpr int radix(int s) { return s < NV ? V : LAST; }
+  __host__ __device__ static constexpr int ns(int s) { return s == 0 ? 1 : ns(s-1)*radix(s-1); }
+  __host__ __device__ static constexpr int off(int s) { return s <= 1 ? 0 : off(s-1)+(radix(s-1)-1)*ns(s-1); }
+};
+
+__device__ __forceinline__ int padr(int e) { return e+(e >> 4); }
+
+// Twiddle table (M - 1 complex words) from zN of a dimension with N = M q.
+template<typename T, int M, int V=16>
+__device__ __forceinline__ void fill_twr(cplx<T> *tw, const cplx<T> *zN, int q,
+                                         int tid, int nthreads)
+{
+  typedef RPlan<M,V> P;
+  for(int e=tid; e < M-1; e += nthreads) {
+    // locate the stage of entry e
+    int s=1;
+#pragma unroll
+    for(int k=2; k < P::NST; ++k)
+      if(e >= P::off(k)) s=k;
+    const int Ns=P::ns(s), R=P::radix(s);
+    const int rel=e-P::off(s);
+    const int tp=rel/Ns+1, kp=rel-(tp-1)*Ns;
+    // e^{2 pi i tp kp/(Ns R)} = zN[q tp kp M/(Ns R)]
+    tw[e]=zN[(int64_t) q*tp*kp*(M/(Ns*R))];
+  }
+}
+
+// Radix-R butterfly (R = 2, 4, 8, 16) with kernel e^{S 2 pi i ut/R}.
+template<typename T, int R, int S>
+__device__ __forceinline__ void bfly(cplx<T> *u)
+{
+  if constexpr(R == 16) dft16<T,S>(u);
+  else DFT<T,R>::run(u,S);
+}
+
+// Synchronisation of the threads of a sequence: the wave when a sequence
+// does not span waves, else the block.
+template<int TPS>
+__device__ __forceinline__ void rsync()
+{
+  if constexpr(TPS <= 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE,"wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE,"wavefront");
+  } else
+    __syncthreads();
+}
+
+// Stage s of the transform (see the top of this file); recurses into s+1.
+template<typename T, int M, int V, int S, int s>
+__device__ __forceinline__ void fftr_stage(cplx<T> *v, T *xb, const cplx<T> *tw,
+                                           int t)
+{
+  typedef RPlan<M,V> P;
+  constexpr int TPS=P::TPS;
+  constexpr int R=P::radix(s), NB=V/R, Ns=P::ns(s), OFF=P::off(s);
+  // butterflies in place: slot b + t' NB holds input t' / output u = t'
+#pragma unroll
+  for(int b=0; b < NB; ++b) {
+    const int j=t+TPS*b;
+    const int kp=j & (Ns-1);
+    cplx<T> u[R];
+#pragma unroll
+    for(int tp=0; tp < R; ++tp) u[tp]=v[b+tp*NB];
+    if constexpr(s > 0) {
+      const cplx<T> *w=tw+OFF+kp;
+#pragma unroll
+      for(int tp=1; tp < R; ++tp) {
+        const cplx<T> z=w[(tp-1)*Ns];
+        u[tp]=S > 0 ? u[tp]*z : mulc(u[tp],z);
+      }
+    }
+    bfly<T,R,S>(u);
+#pragma unroll
+    for(int tp=0; tp < R; ++tp) v[b+tp*NB]=u[tp];
+  }
+  if constexpr(s+1 < P::NST) {
+    // exchange: output u of butterfly b goes to (j/Ns) Ns R + k' + u Ns
+#pragma unroll
+    for(int part=0; part < 2; ++part) {
+#pragma unroll
+      for(int b=0; b < NB; ++b) {
+        const int j=t+TPS*b;
+        const int kp=j & (Ns-1);
+        const int base=(j-kp)*R+kp;
+#pragma unroll
+        for(int u=0; u < R; ++u)
+          xb[padr(base+u*Ns)]=part == 0 ? v[b+u*NB].x : v[b+u*NB].y;
+      }
+      rsync<TPS>();
+#pragma unroll
+      for(int k=0; k < V; ++k) {
+        const T val=xb[padr(t+TPS*k)];
+        if(part == 0) v[k].x=val;
+        else v[k].y=val;
+      }
+      rsync<TPS>();
+    }
+    fftr_stage<T,M,V,S,s+1>(v,xb,tw,t);
+  }
+}
+
+// In-place transform of the sequence held by threads t < TPS (v[V] each)
+// with kernel e^{S 2 pi i jk/M}; xb is the sequence's exchange area
+// (RPlan<M,V>::PAD scalar words), tw the twiddle table of RPlan<M,V>.  When
+// TPS > 64 every thread of the block must call this.
+template<typename T, int M, int V, int S>
+__device__ __forceinline__ void fftr(cplx<T> *v, T *xb, const cplx<T> *tw, int t)
+{
+  fftr_stage<T,M,V,S,0>(v,xb,tw,t);
+}
+
+} // namespace fftwpp_amd

@@ -603,11 +603,77 @@ static int batch_rows_launch(const DimArgs &a, const BatchLaunch &b,
   return rc;
 }
 
+// Threads per sequence of the register-FFT batch kernel for sub-FFT size m
+// (a power of two in [16, 4096]) and element size elt, 0 when the LDS kernel
+// runs.
+static size_t batch_tps(size_t m, size_t elt)
+{
+  if(m < 16 || m > 4096 || (m & (m-1))) return 0;
+  return m/(elt == sizeof(cplx<double>) ? BatchV<double>::V : BatchV<float>::V);
+}
+
+// Threads per block and LDS bytes of the register-FFT kernel.
+static void batch_reg_shape(size_t m, size_t elt, size_t threads,
+                            size_t *nt, size_t *lds)
+{
+  const size_t tps=batch_tps(m,elt);
+  size_t n=tps > 64 ? tps : std::max(threads,tps)/tps*tps;
+  if(n > 512) n=512/tps*tps;
+  const size_t pad=m+m/16;
+  *nt=n;
+  *lds=m*elt+(n/tps)*pad*(elt/2);
+}
+
+template<typename T, int M>
+static int batch_reg_launch(const DimArgs &a, const BatchLaunch &b, int mult,
+                            hipStream_t st)
+{
+  size_t nt, lds;
+  batch_reg_shape(M,sizeof(cplx<T>),b.threads,&nt,&lds);
+  int rc=allow_lds(batch_reg_kernel<T,M>,lds);
+  if(rc) return rc;
+  hipLaunchKernelGGL((batch_reg_kernel<T,M>),dim3((unsigned) b.nblocks),dim3((unsigned) nt),
+                     lds,st,a,b.items,(int64_t) b.count,(const cplx<T> *) b.f,
+                     (const cplx<T> *) b.g,(cplx<T> *) b.out,(T) b.scale,mult,(int) b.D,
+                     (cplx<T> *) b.part,(int64_t) b.ptotal);
+  return launch_check("batch_reg_kernel");
+}
+
+template<typename T>
+static int batch_reg_t(const DimArgs &a, const BatchLaunch &b, int mult,
+                       hipStream_t st)
+{
+  int rc;
+  switch(a.m) {
+  case 16: rc=batch_reg_launch<T,16>(a,b,mult,st); break;
+  case 32: rc=batch_reg_launch<T,32>(a,b,mult,st); break;
+  case 64: rc=batch_reg_launch<T,64>(a,b,mult,st); break;
+  case 128: rc=batch_reg_launch<T,128>(a,b,mult,st); break;
+  case 256: rc=batch_reg_launch<T,256>(a,b,mult,st); break;
+  case 512: rc=batch_reg_launch<T,512>(a,b,mult,st); break;
+  case 1024: rc=batch_reg_launch<T,1024>(a,b,mult,st); break;
+  case 2048: rc=batch_reg_launch<T,2048>(a,b,mult,st); break;
+  case 4096: rc=batch_reg_launch<T,4096>(a,b,mult,st); break;
+  default: return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"register FFT size");
+  }
+  if(rc) return rc;
+  const size_t G=(a.q+b.D-1)/b.D;
+  if(G > 1) {
+    const size_t nb=b.count < 4096 ? b.count : 4096;
+    hipLaunchKernelGGL(batch_reduce_kernel<T>,dim3((unsigned) nb),dim3(256),0,st,
+                       b.items,(int64_t) b.count,(const cplx<T> *) b.part,
+                       (int64_t) b.ptotal,(int) G,(cplx<T> *) b.out,(T) b.scale);
+    rc=launch_check("batch_reduce_kernel");
+  }
+  return rc;
+}
+
 template<typename T>
 static int batch_rows_t(const fftwpp_amd_dim *d, int mult, BatchLaunch b,
                         hipStream_t st)
 {
   const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
+  if(batch_tps(d->m,sizeof(cplx<T>))) return batch_reg_t<T>(a,b,mult,st);
   b.lds=batch_lds(d->m,sizeof(cplx<T>),b.TR,b.lacc,&b.ld);
   if(b.lds > lds_max())
     return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"batch tile does not fit in LDS");
@@ -849,8 +915,19 @@ size_t fftwpp_amd_batch_rows_lds(const fftwpp_amd_dim *d, int dtype,
 {
   if(!d || TR == 0) return 0;
   const size_t elt=dtype == FFTWPP_AMD_F32 ? sizeof(cplx<float>) : sizeof(cplx<double>);
+  if(batch_tps(d->m,elt)) {
+    if(TR > 1 || lacc) return 0;
+    size_t nt, lds;
+    batch_reg_shape(d->m,elt,64,&nt,&lds);
+    return lds;
+  }
   const size_t b=batch_lds(d->m,elt,TR,lacc,NULL);
   return b > lds_max() ? 0 : b;
+}
+
+size_t fftwpp_amd_batch_rows_tps(const fftwpp_amd_dim *d, int dtype)
+{
+  return d ? batch_tps(d->m,dtype == FFTWPP_AMD_F32 ? sizeof(cplx<float>) : sizeof(cplx<double>)) : 0;
 }
 
 int fftwpp_amd_batch_rows(const fftwpp_amd_dim *d, int dtype, int mult,

@@ -117,6 +117,11 @@ int fftwpp_amd_conv_rows(const fftwpp_amd_dim *d, int dtype, int mult,
  * LDS bytes of a tile, 0 when it does not fit. */
 size_t fftwpp_amd_batch_rows_lds(const fftwpp_amd_dim *d, int dtype,
                                  size_t TR, size_t lacc);
+/* Threads per sequence of the register-resident FFT path, taken for
+ * power-of-two m in [16, 4096] (TR and lacc are then ignored, `threads` is
+ * rounded to a multiple of it, at most 512, one sequence per block when it
+ * exceeds 64); 0 when the LDS path runs. */
+size_t fftwpp_amd_batch_rows_tps(const fftwpp_amd_dim *d, int dtype);
 int fftwpp_amd_batch_rows(const fftwpp_amd_dim *d, int dtype, int mult,
                           const void *items, size_t count, const void *f,
                           const void *g, void *out, size_t TR, size_t threads,
