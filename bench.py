@@ -37,6 +37,12 @@ def parse():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the batched, 1D and 2D Hermitian workloads")
+    ap.add_argument("--traffic", action="store_true",
+                    help="measure the dominant kernel's HBM bytes per launch with two "
+                         "rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over a short "
+                         "child run, scaled by a 512 MiB copy")
+    ap.add_argument("--traffic-child", default=None, choices=["f64", "f32"],
+                    help=argparse.SUPPRESS)
     ap.add_argument("--calibrate", action="store_true",
                     help="run a 512 MiB device-to-device copy first (the known-size "
                          "reference for HBM traffic counters)")
@@ -140,6 +146,74 @@ def cpu_baseline(f, g, L):
             "sample": f"one full {L}^3 complex128 convolution (m={L}, q=2 per dimension) "
                       f"with the oracle's C port of the residue algorithm, OpenMP, "
                       f"{dt:.2f} s wall"}
+
+
+# Kernel of each stage of the 3D plan, as named in rocprofv3 records.
+STAGE_KERNEL = {"z rows": "rowconv256p_kernel", "x forward": "colfwd256_kernel",
+                "y forward": "colfwd256_kernel", "y backward": "colbwd256_kernel",
+                "x backward": "colbwd256_kernel"}
+COPY_BYTES = 512 << 20
+
+
+def traffic_child(key, L):
+    """Body profiled by --traffic: a 512 MiB device copy (the counters'
+    calibration), then two steps of the 3D plan."""
+    import torch
+    from fftwpp_amd.torch_conv import DeviceConvolution
+    dev = torch.device("cuda:0")
+    src = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    dt = torch.complex128 if key == "f64" else torch.complex64
+    f, g = inputs(L)
+    a = torch.tensor(f, device=dev, dtype=dt)
+    b = torch.tensor(g, device=dev, dtype=dt)
+    out = torch.empty_like(a)
+    plan = DeviceConvolution((L, L, L), dtype=dt, M=[2 * L] * 3)
+    for _ in range(2):
+        plan.convolve_out(out, a, b)
+    torch.cuda.synchronize()
+
+
+def measure_traffic(L, key="f64"):
+    """Run two rocprofv3 counter passes over traffic_child in child processes
+    (before this process touches the GPU) and return {kernel: bytes per
+    launch} scaled by the copy kernel's reading, plus the scale factors."""
+    import csv
+    import glob
+    import subprocess
+    import tempfile
+    out = {}
+    scale = {}
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, counter)
+            cmd = ["timeout", "-s", "KILL", "300", "rocprofv3", "--pmc", counter,
+                   "--output-format", "csv", "-d", d, "-o", "pmc", "--",
+                   sys.executable, os.path.abspath(__file__), "--traffic-child", key,
+                   "--size", str(L)]
+            subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                           cwd=tmp)
+            rows = []
+            for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+                with open(path) as fh:
+                    rows += list(csv.DictReader(fh))
+            per = {}
+            for r in rows:
+                if r["Counter_Name"] != counter:
+                    continue
+                per.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]) * 1024)
+            copy = [v for k, vs in per.items() if "copyBuffer" in k for v in vs]
+            if not copy:
+                raise RuntimeError("no copy kernel in the counter records")
+            scale[counter] = COPY_BYTES / max(copy)
+            for k, vs in per.items():
+                name = next((n for n in set(STAGE_KERNEL.values()) if n in k), None)
+                if name:
+                    out.setdefault(name, {})[counter] = sum(vs) / len(vs) * scale[counter]
+    return out, scale
 
 
 def batched_lengths(count=4096, lo=64, hi=16384, seed=SEED):
@@ -264,10 +338,17 @@ def bench_hconv2d(torch, dev, steps, warmup):
 
 def main():
     args = parse()
+    if args.traffic_child:
+        traffic_child(args.traffic_child, args.size)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N>1 is not supported "
                                    "(see DESIGN.md for how the path would shard)"}))
         sys.exit(2)
+    traffic = None
+    if args.traffic:
+        # child processes first: this process has not touched the GPU yet
+        traffic = measure_traffic(args.size)
     import numpy as np
     import torch
     from fftwpp_amd.torch_conv import DeviceConvolution
@@ -319,6 +400,16 @@ def main():
                           "frac": round(CONV_UNITS * 16 * L3 / (ms64 * 1e-3) / 1e9 / PEAK_GBS, 4)},
         "stages_ms": {k: round(v, 4) for k, v in st64.items()},
     }
+    if traffic is not None:
+        per, scale = traffic
+        kname = STAGE_KERNEL.get(result["roofline"]["kernel"])
+        if kname in per and len(per[kname]) == 2:
+            t = per[kname]["FETCH_SIZE"] + per[kname]["WRITE_SIZE"]
+            result["roofline"]["traffic"] = round(t)
+            result["roofline"]["traffic_kernel"] = kname
+        result["traffic_scale"] = {k: round(v, 4) for k, v in scale.items()}
+        result["traffic_per_kernel"] = {k: {c: round(v) for c, v in d.items()}
+                                        for k, d in per.items()}
     del a, b, out, plan
     torch.cuda.empty_cache()
 
