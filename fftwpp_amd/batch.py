@@ -63,7 +63,10 @@ class BatchedConvolution:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().fftwpp_amd_batch_destroy(h)
+            try:
+                lib().fftwpp_amd_batch_destroy(h)
+            except Exception:  # interpreter shutdown: module globals already gone
+                pass
             self._h = None
 
     def geometry(self, i):

@@ -73,11 +73,11 @@ __device__ __forceinline__ void bfly(cplx<T> *u)
 }
 
 // Synchronisation of the threads of a sequence: the wave when a sequence
-// does not span waves, else the block.
-template<int TPS>
+// does not span waves (its TPS threads are consecutive lanes), else the block.
+template<int TPS, bool BLK=false>
 __device__ __forceinline__ void rsync()
 {
-  if constexpr(TPS <= 64) {
+  if constexpr(TPS <= 64 && !BLK) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE,"wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE,"wavefront");
@@ -86,7 +86,7 @@ __device__ __forceinline__ void rsync()
 }
 
 // Stage s of the transform (see the top of this file); recurses into s+1.
-template<typename T, int M, int V, int S, int s>
+template<typename T, int M, int V, int S, int s, bool BLK>
 __device__ __forceinline__ void fftr_stage(cplx<T> *v, T *xb, const cplx<T> *tw,
                                            int t)
 {
@@ -126,16 +126,16 @@ __device__ __forceinline__ void fftr_stage(cplx<T> *v, T *xb, const cplx<T> *tw,
         for(int u=0; u < R; ++u)
           xb[padr(base+u*Ns)]=part == 0 ? v[b+u*NB].x : v[b+u*NB].y;
       }
-      rsync<TPS>();
+      rsync<TPS,BLK>();
 #pragma unroll
       for(int k=0; k < V; ++k) {
         const T val=xb[padr(t+TPS*k)];
         if(part == 0) v[k].x=val;
         else v[k].y=val;
       }
-      rsync<TPS>();
+      rsync<TPS,BLK>();
     }
-    fftr_stage<T,M,V,S,s+1>(v,xb,tw,t);
+    fftr_stage<T,M,V,S,s+1,BLK>(v,xb,tw,t);
   }
 }
 
@@ -146,7 +146,16 @@ __device__ __forceinline__ void fftr_stage(cplx<T> *v, T *xb, const cplx<T> *tw,
 template<typename T, int M, int V, int S>
 __device__ __forceinline__ void fftr(cplx<T> *v, T *xb, const cplx<T> *tw, int t)
 {
-  fftr_stage<T,M,V,S,0>(v,xb,tw,t);
+  fftr_stage<T,M,V,S,0,false>(v,xb,tw,t);
+}
+
+// The same transform when the threads of a sequence are not consecutive
+// lanes (block barriers; every thread of the block must call it).
+template<typename T, int M, int V, int S>
+__device__ __forceinline__ void fftr_blk(cplx<T> *v, T *xb, const cplx<T> *tw,
+                                         int t)
+{
+  fftr_stage<T,M,V,S,0,true>(v,xb,tw,t);
 }
 
 } // namespace fftwpp_amd

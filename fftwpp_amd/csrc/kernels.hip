@@ -28,6 +28,7 @@
 #include "fft_lds.h"
 #include "kernels256.h"
 #include "batch.h"
+#include "rowsreg.h"
 #include "../../include/fftwpp_amd.h"
 
 using namespace fftwpp_amd;
@@ -477,6 +478,74 @@ static bool fast256(const fftwpp_amd_dim *d, size_t s1=0, size_t s2=0)
 template<typename T>
 static constexpr int rows256() { return sizeof(T) == 8 ? 8 : 16; }
 
+// Whether the register-FFT column kernels take dimension d: power-of-two m in
+// [16, 4096], complex or centred, p = ceil(L/m) <= 2.
+static bool cols_reg(const fftwpp_amd_dim *d)
+{
+  const size_t m=d->m;
+  if(m < 16 || m > 4096 || (m & (m-1))) return false;
+  if(d->kind == FFTWPP_AMD_KIND_HERMITIAN) return false;
+  return d->L <= 2*m;
+}
+
+template<typename T, int M>
+static int colreg_launch(const DimArgs &a, bool fwd, const void *in,
+                         size_t s_in, size_t bsi, void *out, size_t s_out,
+                         size_t bso, size_t C, size_t nbatch, size_t r0,
+                         size_t nres, int accumulate, double scale,
+                         hipStream_t st)
+{
+  typedef ColReg<T,M> CR;
+  const size_t lds=M*sizeof(cplx<T>)+CR::TC*CR::CS*sizeof(T);
+  const dim3 grid((unsigned) ((C+CR::TC-1)/CR::TC),(unsigned) nbatch);
+  const bool p2=a.L > M;
+  int rc;
+  if(fwd) {
+    if(p2) {
+      if((rc=allow_lds(colfwd_reg_kernel<T,M,2>,lds))) return rc;
+      hipLaunchKernelGGL((colfwd_reg_kernel<T,M,2>),grid,dim3(CR::NT),lds,st,a,
+                         (const cplx<T> *) in,(int64_t) s_in,(int64_t) C,(int64_t) bsi,
+                         (cplx<T> *) out,(int64_t) s_out,(int64_t) bso,(int64_t) r0,(int64_t) nres);
+    } else {
+      if((rc=allow_lds(colfwd_reg_kernel<T,M,1>,lds))) return rc;
+      hipLaunchKernelGGL((colfwd_reg_kernel<T,M,1>),grid,dim3(CR::NT),lds,st,a,
+                         (const cplx<T> *) in,(int64_t) s_in,(int64_t) C,(int64_t) bsi,
+                         (cplx<T> *) out,(int64_t) s_out,(int64_t) bso,(int64_t) r0,(int64_t) nres);
+    }
+    return launch_check("colfwd_reg_kernel");
+  }
+  if(p2) {
+    if((rc=allow_lds(colbwd_reg_kernel<T,M,2>,lds))) return rc;
+    hipLaunchKernelGGL((colbwd_reg_kernel<T,M,2>),grid,dim3(CR::NT),lds,st,a,
+                       (const cplx<T> *) in,(int64_t) s_in,(int64_t) bsi,(cplx<T> *) out,
+                       (int64_t) s_out,(int64_t) C,(int64_t) bso,(int64_t) r0,(int64_t) nres,
+                       accumulate,(T) scale);
+  } else {
+    if((rc=allow_lds(colbwd_reg_kernel<T,M,1>,lds))) return rc;
+    hipLaunchKernelGGL((colbwd_reg_kernel<T,M,1>),grid,dim3(CR::NT),lds,st,a,
+                       (const cplx<T> *) in,(int64_t) s_in,(int64_t) bsi,(cplx<T> *) out,
+                       (int64_t) s_out,(int64_t) C,(int64_t) bso,(int64_t) r0,(int64_t) nres,
+                       accumulate,(T) scale);
+  }
+  return launch_check("colbwd_reg_kernel");
+}
+
+template<typename T>
+static int colreg_t(const DimArgs &a, bool fwd, const void *in, size_t s_in,
+                    size_t bsi, void *out, size_t s_out, size_t bso, size_t C,
+                    size_t nbatch, size_t r0, size_t nres, int accumulate,
+                    double scale, hipStream_t st)
+{
+#define FFTWPP_AMD_COLREG(MM) case MM: return colreg_launch<T,MM>(a,fwd,in,s_in,bsi,out,s_out,bso,C,nbatch,r0,nres,accumulate,scale,st);
+  switch(a.m) {
+  FFTWPP_AMD_COLREG(16) FFTWPP_AMD_COLREG(32) FFTWPP_AMD_COLREG(64)
+  FFTWPP_AMD_COLREG(128) FFTWPP_AMD_COLREG(256) FFTWPP_AMD_COLREG(512)
+  FFTWPP_AMD_COLREG(1024) FFTWPP_AMD_COLREG(2048) FFTWPP_AMD_COLREG(4096)
+  default: return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"register FFT size");
+  }
+#undef FFTWPP_AMD_COLREG
+}
+
 template<typename T>
 static int fwd_many_t(const fftwpp_amd_dim *d, const void *f, size_t S,
                       size_t C, size_t nbatch, size_t bsi, void *F,
@@ -496,6 +565,8 @@ static int fwd_many_t(const fftwpp_amd_dim *d, const void *f, size_t S,
                        (int64_t) r0,(int64_t) nres,(const cplx<T> *) a.zN,(int64_t) d->q);
     return launch_check("colfwd256_kernel");
   }
+  if(cols_reg(d))
+    return colreg_t<T>(a,true,f,S,bsi,F,SF,bso,C,nbatch,r0,nres,0,1.0,st);
   size_t elt=sizeof(cplx<T>);
   size_t per=2*(d->m+1)*elt;
   if(per > lds_max()) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"m too large for the LDS transform");
@@ -532,6 +603,8 @@ static int bwd_many_t(const fftwpp_amd_dim *d, const void *F, size_t SF,
                        (int64_t) d->q);
     return launch_check("colbwd256_kernel");
   }
+  if(cols_reg(d))
+    return colreg_t<T>(a,false,F,SF,bsi,f,S,bso,C,nbatch,r0,nres,accumulate,scale,st);
   size_t elt=sizeof(cplx<T>);
   size_t per=2*(d->m+1)*elt;
   if(per > lds_max()) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"m too large for the LDS transform");
@@ -682,6 +755,55 @@ static int batch_rows_t(const fftwpp_amd_dim *d, int mult, BatchLaunch b,
   return batch_rows_launch<T,1>(a,b,st);
 }
 
+// Whether the register-FFT row kernel takes rows of dimension d: power-of-two
+// m in [16, 4096] covering the row in one block.
+static bool rows_reg(const fftwpp_amd_dim *d)
+{
+  const size_t m=d->m;
+  if(m < 16 || m > 4096 || (m & (m-1))) return false;
+  if(d->kind == FFTWPP_AMD_KIND_HERMITIAN) return (d->L+1)/2 <= m;
+  return d->L <= m;
+}
+
+template<typename T, int M, bool HERM>
+static int rowreg_launch(const DimArgs &a, const void *f, const void *g,
+                         size_t rows, size_t rs, void *out, size_t os,
+                         double scale, int mult, hipStream_t st)
+{
+  constexpr int V=RowV::V;
+  constexpr int TPS=M/V;
+  const size_t nt=TPS > 64 ? TPS : 256;
+  const size_t slots=nt/TPS;
+  const size_t lds=M*sizeof(cplx<T>)+slots*RPlan<M,V>::PAD*sizeof(T);
+  int rc=allow_lds(rowreg_kernel<T,M,HERM>,lds);
+  if(rc) return rc;
+  size_t nb=(rows+slots-1)/slots;
+  if(nb > 65536) nb=65536;
+  hipLaunchKernelGGL((rowreg_kernel<T,M,HERM>),dim3((unsigned) nb),dim3((unsigned) nt),lds,st,
+                     a,(const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
+                     (cplx<T> *) out,(int64_t) os,(T) scale,mult);
+  return launch_check("rowreg_kernel");
+}
+
+template<typename T, bool HERM>
+static int rowreg_t(const DimArgs &a, const void *f, const void *g, size_t rows,
+                    size_t rs, void *out, size_t os, double scale, int mult,
+                    hipStream_t st)
+{
+  switch(a.m) {
+  case 16: return rowreg_launch<T,16,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
+  case 32: return rowreg_launch<T,32,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
+  case 64: return rowreg_launch<T,64,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
+  case 128: return rowreg_launch<T,128,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
+  case 256: return rowreg_launch<T,256,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
+  case 512: return rowreg_launch<T,512,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
+  case 1024: return rowreg_launch<T,1024,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
+  case 2048: return rowreg_launch<T,2048,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
+  case 4096: return rowreg_launch<T,4096,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
+  default: return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"register FFT size");
+  }
+}
+
 template<typename T>
 static int conv_rows_t(const fftwpp_amd_dim *d, int mult, const void *f,
                        const void *g, size_t rows, size_t rs, void *out,
@@ -737,6 +859,11 @@ static int conv_rows_t(const fftwpp_amd_dim *d, int mult, const void *f,
     }
     return launch_check("rowconv256_kernel");
   }
+  if(rows_reg(d)) {
+    if(d->kind == FFTWPP_AMD_KIND_HERMITIAN)
+      return rowreg_t<T,true>(a,f,g,rows,rs,out,os,scale,mult,st);
+    return rowreg_t<T,false>(a,f,g,rows,rs,out,os,scale,mult,st);
+  }
   size_t elt=sizeof(cplx<T>);
   bool herm=d->kind == FFTWPP_AMD_KIND_HERMITIAN;
   size_t lout=herm ? (d->L+1)/2 : d->L;
@@ -787,7 +914,7 @@ void fftwpp_amd_internal_set_error(const char *msg) { fail(0,msg); }
 int fftwpp_amd_internal_rows_fit(const fftwpp_amd_dim *d, int dtype)
 {
   if(!d) return 0;
-  if(fast256(d)) return 1;
+  if(fast256(d) || rows_reg(d)) return 1;
   const size_t elt=dtype == FFTWPP_AMD_F32 ? sizeof(cplx<float>) : sizeof(cplx<double>);
   const size_t lout=d->kind == FFTWPP_AMD_KIND_HERMITIAN ? (d->L+1)/2 : d->L;
   return (4*d->m+lout)*elt <= lds_max();

@@ -1,0 +1,304 @@
+// rowsreg.h -- fused row convolutions on the register-resident FFT (fftr.h)
+// for a power-of-two sub-FFT size M (16..4096) when one block of M covers the
+// row: complex or centred rows with L <= M, Hermitian rows with H <= M.
+//
+// A slot of TPS = M/V threads owns one row (thread t holds sigma = t + TPS k,
+// k < V), loops over the residues with the sums kept in registers and writes
+// the row once at the end, so out may equal f.  Rows are contiguous.
+//
+// Complex rows (Convolution::convolveRaw, convolve.cc:7513-7575, with the
+// forward1/backward1 residue transforms, convolve.cc:836-1030, 1468-1613, and
+// the centred folds of fftPadCentered): element j has wavenumber k = j + k0
+// and sits at sigma = k mod M; residue r is DFT_M[w_N^{k r} f](s).
+//
+// Hermitian rows (fftPadHermitian, convolve.cc:4309-5447): residues r1, r2 =
+// r1 + 1 are packed as W1 + i W2 with W_sigma = sum over k = sigma mod M,
+// |k| < H, of w_N^{k r} f(k), f(-k) = conj f(k) (k = sigma and sigma - M),
+// multiplied as realMultBinary (real parts, imaginary parts) and unpacked
+// after the inverse with the mirror element Z(M - sigma), fetched through the
+// slot's LDS exchange area.
+#pragma once
+#include "batch.h"
+
+namespace fftwpp_amd {
+
+__device__ __forceinline__ int wrapN(int e, int N)
+{
+  e %= N;
+  return e < 0 ? e+N : e;
+}
+
+// Values per thread: 8 in both precisions (the register sums over residues
+// sit beside the two transforms).
+struct RowV { static constexpr int V=8; };
+
+// Dynamic LDS: M complex twiddles + slots * RPlan<M,V>::PAD scalar words.
+// When TPS > 64 the host launches one slot (row) per block.
+template<typename T, int M, bool HERM>
+__global__ void __launch_bounds__(512)
+rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
+              int64_t rs, cplx<T> *out, int64_t os, T scale, int mult)
+{
+  constexpr int V=RowV::V;
+  typedef RPlan<M,V> P;
+  constexpr int TPS=P::TPS;
+  extern __shared__ char smem_raw[];
+  cplx<T> *tw=reinterpret_cast<cplx<T> *>(smem_raw);
+  const cplx<T> *zN=(const cplx<T> *) d.zN;
+  fill_twr<T,M,V>(tw,zN,(int) d.q,threadIdx.x,blockDim.x);
+  __syncthreads();
+  const int slots=blockDim.x/TPS;
+  const int slot=threadIdx.x/TPS, t=threadIdx.x-slot*TPS;
+  T *xb=reinterpret_cast<T *>(tw+M)+slot*P::PAD;
+  const int N=(int) d.N, q=(int) d.q, L=(int) d.L, H=(int) d.H, k0=(int) d.k0;
+  for(int64_t rb=(int64_t) blockIdx.x*slots; rb < rows; rb += (int64_t) gridDim.x*slots) {
+    const int64_t row=rb+slot;
+    if(row >= rows) break;               // only when slots > 1 (TPS <= 64)
+    const cplx<T> *fr=f+row*rs, *gr=g+row*rs;
+    cplx<T> acc[V];
+#pragma unroll
+    for(int k=0; k < V; ++k) acc[k]=mk((T) 0,(T) 0);
+    if constexpr(!HERM) {
+      for(int r=0; r < q; ++r) {
+        cplx<T> X[V], Y[V];
+#pragma unroll
+        for(int k=0; k < V; ++k) {
+          const int sigma=t+TPS*k;
+          const int j=(sigma-k0) & (M-1);      // k0 <= 0, M a power of two
+          cplx<T> a=mk((T) 0,(T) 0), b=a;
+          if(j < L) {
+            a=fr[j];
+            b=gr[j];
+            if(r) {
+              const cplx<T> w=zN[wrapN((j+k0)*r,N)];
+              a=w*a;
+              b=w*b;
+            }
+          }
+          X[k]=a;
+          Y[k]=b;
+        }
+        fftr<T,M,V,1>(X,xb,tw,t);
+        fftr<T,M,V,1>(Y,xb,tw,t);
+#pragma unroll
+        for(int k=0; k < V; ++k)
+          X[k]=mult == 1 ? X[k]*Y[k] : (mult == 3 ? mulc(X[k],Y[k]) : X[k]);
+        fftr<T,M,V,-1>(X,xb,tw,t);
+#pragma unroll
+        for(int k=0; k < V; ++k) {
+          const int sigma=t+TPS*k;
+          const int j=(sigma-k0) & (M-1);
+          if(j < L) acc[k]=acc[k]+(r ? mulc(X[k],zN[wrapN((j+k0)*r,N)]) : X[k]);
+        }
+      }
+#pragma unroll
+      for(int k=0; k < V; ++k) {
+        const int sigma=t+TPS*k;
+        const int j=(sigma-k0) & (M-1);
+        if(j < L) out[row*os+j]=scale*acc[k];
+      }
+    } else {
+      for(int r1=0; r1 < q; r1 += 2) {
+        const int r2=r1+1 < q ? r1+1 : -1;
+        cplx<T> X[V], Y[V];
+#pragma unroll
+        for(int k=0; k < V; ++k) {
+          const int sigma=t+TPS*k;
+          cplx<T> a=mk((T) 0,(T) 0), b=a;
+          // k = sigma (when < H) and k = sigma - M (when > -H)
+#pragma unroll
+          for(int h=0; h < 2; ++h) {
+            const int kk=h == 0 ? sigma : sigma-M;
+            if(h == 0 ? kk < H : kk > -H) {
+              const cplx<T> fv=kk >= 0 ? fr[kk] : conjg(fr[-kk]);
+              const cplx<T> gv=kk >= 0 ? gr[kk] : conjg(gr[-kk]);
+              cplx<T> w=zN[wrapN(kk*r1,N)];
+              if(r2 >= 0) w=w+muli(zN[wrapN(kk*r2,N)],1);
+              a=a+w*fv;
+              b=b+w*gv;
+            }
+          }
+          X[k]=a;
+          Y[k]=b;
+        }
+        fftr<T,M,V,1>(X,xb,tw,t);
+        fftr<T,M,V,1>(Y,xb,tw,t);
+#pragma unroll
+        for(int k=0; k < V; ++k) X[k]=mk(X[k].x*Y[k].x,X[k].y*Y[k].y);
+        fftr<T,M,V,-1>(X,xb,tw,t);
+        // mirror elements Z(M - sigma) through LDS, real then imaginary parts
+        cplx<T> Zm[V];
+#pragma unroll
+        for(int part=0; part < 2; ++part) {
+#pragma unroll
+          for(int k=0; k < V; ++k)
+            xb[padr(t+TPS*k)]=part == 0 ? X[k].x : X[k].y;
+          rsync<TPS>();
+#pragma unroll
+          for(int k=0; k < V; ++k) {
+            const T val=xb[padr((M-(t+TPS*k)) & (M-1))];
+            if(part == 0) Zm[k].x=val;
+            else Zm[k].y=val;
+          }
+          rsync<TPS>();
+        }
+#pragma unroll
+        for(int k=0; k < V; ++k) {
+          const int sigma=t+TPS*k;
+          if(sigma < H) {
+            const cplx<T> z=X[k], zc=conjg(Zm[k]);
+            const cplx<T> y1=(T) 0.5*(z+zc);
+            cplx<T> v=mulc(y1,zN[wrapN(sigma*r1,N)]);
+            if(r2 >= 0) {
+              const cplx<T> y2=muli((T) 0.5*(z-zc),-1);
+              v=v+mulc(y2,zN[wrapN(sigma*r2,N)]);
+            }
+            acc[k]=acc[k]+v;
+          }
+        }
+      }
+#pragma unroll
+      for(int k=0; k < V; ++k) {
+        const int sigma=t+TPS*k;
+        if(sigma < H) out[row*os+sigma]=scale*acc[k];
+      }
+    }
+  }
+}
+
+
+// ------------------------------------------------------------------------
+// Column residue transforms on the register FFT (fftPad forward/backward of
+// strided columns, convolve.cc:771-1965) for power-of-two M and at most P2 = 2
+// blocks of M per column (p <= 2), complex or centred data.  Lane (c, t) =
+// (threadIdx.x % TC, threadIdx.x / TC): consecutive lanes take consecutive
+// columns, so every load and store of a wave covers TC adjacent columns; the
+// TPS threads of a column span waves and exchange with block barriers through
+// the column's LDS area (CS scalar words apart, a stride chosen to spread the
+// columns of a wave over the banks).  Dynamic LDS: M complex twiddles +
+// TC * CS scalar words.
+// ------------------------------------------------------------------------
+template<typename T, int M> struct ColReg {
+  static constexpr int V=8;
+  static constexpr int TPS=M/V;
+  static constexpr int TC=TPS >= 512 ? 1 : (512/TPS < 64 ? 512/TPS : 64);
+  static constexpr int CS=RPlan<M,V>::PAD+(sizeof(T) == 8 ? 4 : 8);
+  static constexpr int NT=TPS*TC;
+};
+
+// F[b][(r-r0) M + s][c] = sum_j f[b][j][c] w_N^{(j+k0)(q s + r)}, j < L.
+template<typename T, int M, int P2>
+__global__ void __launch_bounds__(512)
+colfwd_reg_kernel(DimArgs d, const cplx<T> *f, int64_t S, int64_t C,
+                  int64_t bsi, cplx<T> *F, int64_t SF, int64_t bso, int64_t r0,
+                  int64_t nres)
+{
+  typedef ColReg<T,M> CR;
+  constexpr int V=CR::V, TPS=CR::TPS, TC=CR::TC;
+  extern __shared__ char smem_raw[];
+  cplx<T> *tw=reinterpret_cast<cplx<T> *>(smem_raw);
+  const cplx<T> *zN=(const cplx<T> *) d.zN;
+  fill_twr<T,M,V>(tw,zN,(int) d.q,threadIdx.x,CR::NT);
+  __syncthreads();
+  const int c=threadIdx.x % TC, t=threadIdx.x/TC;
+  T *xb=reinterpret_cast<T *>(tw+M)+c*CR::CS;
+  const int64_t c0=(int64_t) blockIdx.x*TC;
+  const bool ok=c0+c < C;
+  const cplx<T> *fb=f+(int64_t) blockIdx.y*bsi+c0+c;
+  cplx<T> *Fb=F+(int64_t) blockIdx.y*bso+c0+c;
+  const int N=(int) d.N, L=(int) d.L, k0=(int) d.k0;
+  // the column, read once: element j = j0 + M i with j0 = (sigma - k0) mod M
+  cplx<T> raw[P2][V];
+#pragma unroll
+  for(int k=0; k < V; ++k) {
+    const int j0=(t+TPS*k-k0) & (M-1);
+#pragma unroll
+    for(int i=0; i < P2; ++i) {
+      const int j=j0+M*i;
+      raw[i][k]=ok && j < L ? fb[(int64_t) j*S] : mk((T) 0,(T) 0);
+    }
+  }
+  for(int64_t rr=0; rr < nres; ++rr) {
+    const int r=(int) (r0+rr);
+    cplx<T> X[V];
+#pragma unroll
+    for(int k=0; k < V; ++k) {
+      const int j0=(t+TPS*k-k0) & (M-1);
+      cplx<T> a=r ? raw[0][k]*zN[wrapN((j0+k0)*r,N)] : raw[0][k];
+#pragma unroll
+      for(int i=1; i < P2; ++i)
+        a=a+(r ? raw[i][k]*zN[wrapN((j0+M*i+k0)*r,N)] : raw[i][k]);
+      X[k]=a;
+    }
+    fftr_blk<T,M,V,1>(X,xb,tw,t);
+    if(ok) {
+      cplx<T> *Fr=Fb+rr*M*SF;
+#pragma unroll
+      for(int k=0; k < V; ++k) Fr[(int64_t) (t+TPS*k)*SF]=X[k];
+    }
+  }
+}
+
+// f[b][j][c] = scale * ((accumulate ? f : 0) + sum_r w_N^{-(j+k0) r}
+// DFT^{-1}[F_r]((j+k0) mod M)), j < L.
+template<typename T, int M, int P2>
+__global__ void __launch_bounds__(512)
+colbwd_reg_kernel(DimArgs d, const cplx<T> *F, int64_t SF, int64_t bsi,
+                  cplx<T> *f, int64_t S, int64_t C, int64_t bso, int64_t r0,
+                  int64_t nres, int accumulate, T scale)
+{
+  typedef ColReg<T,M> CR;
+  constexpr int V=CR::V, TPS=CR::TPS, TC=CR::TC;
+  extern __shared__ char smem_raw[];
+  cplx<T> *tw=reinterpret_cast<cplx<T> *>(smem_raw);
+  const cplx<T> *zN=(const cplx<T> *) d.zN;
+  fill_twr<T,M,V>(tw,zN,(int) d.q,threadIdx.x,CR::NT);
+  __syncthreads();
+  const int c=threadIdx.x % TC, t=threadIdx.x/TC;
+  T *xb=reinterpret_cast<T *>(tw+M)+c*CR::CS;
+  const int64_t c0=(int64_t) blockIdx.x*TC;
+  const bool ok=c0+c < C;
+  const cplx<T> *Fb=F+(int64_t) blockIdx.y*bsi+c0+c;
+  cplx<T> *fb=f+(int64_t) blockIdx.y*bso+c0+c;
+  const int N=(int) d.N, L=(int) d.L, k0=(int) d.k0;
+  cplx<T> acc[P2][V];
+#pragma unroll
+  for(int i=0; i < P2; ++i)
+#pragma unroll
+    for(int k=0; k < V; ++k) acc[i][k]=mk((T) 0,(T) 0);
+  for(int64_t rr=0; rr < nres; ++rr) {
+    const int r=(int) (r0+rr);
+    const cplx<T> *Fr=Fb+rr*M*SF;
+    cplx<T> X[V];
+#pragma unroll
+    for(int k=0; k < V; ++k) X[k]=ok ? Fr[(int64_t) (t+TPS*k)*SF] : mk((T) 0,(T) 0);
+    fftr_blk<T,M,V,-1>(X,xb,tw,t);
+#pragma unroll
+    for(int k=0; k < V; ++k) {
+      const int j0=(t+TPS*k-k0) & (M-1);
+#pragma unroll
+      for(int i=0; i < P2; ++i) {
+        const int j=j0+M*i;
+        acc[i][k]=acc[i][k]+(r ? mulc(X[k],zN[wrapN((j+k0)*r,N)]) : X[k]);
+      }
+    }
+  }
+  if(ok) {
+#pragma unroll
+    for(int k=0; k < V; ++k) {
+      const int j0=(t+TPS*k-k0) & (M-1);
+#pragma unroll
+      for(int i=0; i < P2; ++i) {
+        const int j=j0+M*i;
+        if(j < L) {
+          cplx<T> *o=fb+(int64_t) j*S;
+          cplx<T> v=acc[i][k];
+          if(accumulate) v=v+*o;
+          *o=scale*v;
+        }
+      }
+    }
+  }
+}
+
+} // namespace fftwpp_amd

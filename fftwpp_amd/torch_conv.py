@@ -61,7 +61,10 @@ class DeviceConvolution:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            lib().fftwpp_amd_conv_destroy(h)
+            try:
+                lib().fftwpp_amd_conv_destroy(h)
+            except Exception:  # interpreter shutdown: module globals already gone
+                pass
             self._h = None
 
     def geometry(self):
