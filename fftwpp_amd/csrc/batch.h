@@ -159,7 +159,6 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
   T *xb=reinterpret_cast<T *>(tw+M)+slot*P::PAD;
   const int64_t G=(d.q+D-1)/D;
   const int64_t units=count*G;
-  const int N=(int) d.N;
   for(int64_t ub=(int64_t) blockIdx.x*slots; ub < units; ub += (int64_t) gridDim.x*slots) {
     const int64_t unit=ub+slot;
     if(unit >= units) break;            // only when slots > 1 (TPS <= 64)
@@ -171,26 +170,32 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
     const cplx<T> *fi=f+it.off, *gi=g+it.off;
     const int L=(int) it.L;
     for(int r=rbeg; r < rend; ++r) {
-      // w_N^{j r} for j = sigma + M b: index sigma r + b (M r mod N) mod N
-      const int step=(M*r) % N;
+      // w_N^{(sigma + M b) r} = w_N^{sigma r} w_q^{b r}: one per-thread factor
+      // and one factor shared by the slot per block b
+      const int qi=(int) d.q;
       cplx<T> X[V], Y[V];
 #pragma unroll
       for(int k=0; k < V; ++k) X[k]=Y[k]=mk((T) 0,(T) 0);
-      // blocks b of M: element j = sigma + M b, twiddle index sigma r + eb
       for(int jb=0, eb=0; jb < L; jb += M) {
+        const cplx<T> wb=zN[eb*M];               // w_q^{b r}, eb = b r mod q
 #pragma unroll
         for(int k=0; k < V; ++k) {
           const int j=jb+t+TPS*k;
           if(j < L) {
-            int e=(t+TPS*k)*r+eb;
-            if(e >= N) e -= N;
-            const cplx<T> w=zN[e];
-            X[k]=X[k]+w*fi[j];
-            Y[k]=Y[k]+w*gi[j];
+            X[k]=X[k]+wb*fi[j];
+            Y[k]=Y[k]+wb*gi[j];
           }
         }
-        eb += step;
-        if(eb >= N) eb -= N;
+        eb += r;
+        if(eb >= qi) eb -= qi;
+      }
+      if(r) {
+#pragma unroll
+        for(int k=0; k < V; ++k) {
+          const cplx<T> w=zN[(t+TPS*k)*r];      // sigma r < M q = N
+          X[k]=w*X[k];
+          Y[k]=w*Y[k];
+        }
       }
       fftr<T,M,V,1>(X,xb,tw,t);
       fftr<T,M,V,1>(Y,xb,tw,t);
@@ -198,21 +203,24 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
       for(int k=0; k < V; ++k)
         X[k]=mult == 1 ? X[k]*Y[k] : (mult == 3 ? mulc(X[k],Y[k]) : X[k]);
       fftr<T,M,V,-1>(X,xb,tw,t);
+      if(r) {
+#pragma unroll
+        for(int k=0; k < V; ++k) X[k]=mulc(X[k],zN[(t+TPS*k)*r]);
+      }
       const bool first=r == rbeg, last=r == rend-1;
       for(int jb=0, eb=0; jb < L; jb += M) {
+        const cplx<T> wb=zN[eb*M];
 #pragma unroll
         for(int k=0; k < V; ++k) {
           const int j=jb+t+TPS*k;
           if(j < L) {
-            int e=(t+TPS*k)*r+eb;
-            if(e >= N) e -= N;
-            cplx<T> v=mulc(X[k],zN[e]);
+            cplx<T> v=mulc(X[k],wb);
             if(!first) v=v+dst[j];
             dst[j]=last ? sc*v : v;
           }
         }
-        eb += step;
-        if(eb >= N) eb -= N;
+        eb += r;
+        if(eb >= qi) eb -= qi;
       }
     }
   }

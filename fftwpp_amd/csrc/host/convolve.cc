@@ -350,6 +350,27 @@ static Choice choose(size_t L, size_t M, Application &app, size_t C, size_t S,
       best_N=N;
     }
   }
+  // The register-resident kernels take power-of-two m up to 4096 (rows when
+  // one block of m covers the row: L <= m, or H <= m for Hermitian data);
+  // they run several times faster than the LDS engine, so when the smallest
+  // padded length needs another m, a covering power of two is preferred while
+  // its padded length stays within twice the smallest one.
+  if(best_m != 0 && (best_m & (best_m-1)) != 0) {
+    size_t cover=kind == 2 ? (L+1)/2 : L;
+    size_t m=utils::ceilpow2(std::max<size_t>(cover,16));
+    if(m <= std::min<size_t>(lim,4096)) {
+      size_t p, n, q;
+      fftBase::parameters(L,M,m,centered,p,n,q);
+      bool ok;
+      if(kind == 0) ok=fftPad::valid(m,p,q,n,1,S);
+      else if(kind == 1) ok=fftPadCentered::validCentered(m,p,q,n,1,S);
+      else ok=fftPadHermitian::validHermitian(m,p,q,n,q > 1 ? 2 : 1,C);
+      if(ok && m*q <= 2*best_N) {
+        best_m=m;
+        best_N=m*q;
+      }
+    }
+  }
   if(best_m == 0) {
     // explicit padding as the last resort (convolve.cc:431-434)
     c.m=nextfftsize(M);
