@@ -765,24 +765,40 @@ static bool rows_reg(const fftwpp_amd_dim *d)
   return d->L <= m;
 }
 
+// RP residue lanes per row when rows are too few to fill the chip (under 512)
+// and the row has at least two residue passes (residues, or residue pairs for
+// Hermitian rows).
+template<typename T, int M, bool HERM, int RP>
+static int rowreg_launch_rp(const DimArgs &a, const void *f, const void *g,
+                            size_t rows, size_t rs, void *out, size_t os,
+                            double scale, int mult, hipStream_t st)
+{
+  constexpr int V=RowV::V;
+  constexpr int TPS=M/V;
+  const size_t nt=RP > 1 ? RP*TPS : (TPS > 64 ? TPS : 256);
+  const size_t slots=nt/TPS;
+  const size_t lds=M*sizeof(cplx<T>)+slots*RPlan<M,V>::PAD*sizeof(T);
+  int rc=allow_lds(rowreg_kernel<T,M,HERM,RP>,lds);
+  if(rc) return rc;
+  const size_t rpb=RP > 1 ? 1 : slots;
+  size_t nb=(rows+rpb-1)/rpb;
+  if(nb > 65536) nb=65536;
+  hipLaunchKernelGGL((rowreg_kernel<T,M,HERM,RP>),dim3((unsigned) nb),dim3((unsigned) nt),lds,st,
+                     a,(const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
+                     (cplx<T> *) out,(int64_t) os,(T) scale,mult);
+  return launch_check("rowreg_kernel");
+}
+
 template<typename T, int M, bool HERM>
 static int rowreg_launch(const DimArgs &a, const void *f, const void *g,
                          size_t rows, size_t rs, void *out, size_t os,
                          double scale, int mult, hipStream_t st)
 {
-  constexpr int V=RowV::V;
-  constexpr int TPS=M/V;
-  const size_t nt=TPS > 64 ? TPS : 256;
-  const size_t slots=nt/TPS;
-  const size_t lds=M*sizeof(cplx<T>)+slots*RPlan<M,V>::PAD*sizeof(T);
-  int rc=allow_lds(rowreg_kernel<T,M,HERM>,lds);
-  if(rc) return rc;
-  size_t nb=(rows+slots-1)/slots;
-  if(nb > 65536) nb=65536;
-  hipLaunchKernelGGL((rowreg_kernel<T,M,HERM>),dim3((unsigned) nb),dim3((unsigned) nt),lds,st,
-                     a,(const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
-                     (cplx<T> *) out,(int64_t) os,(T) scale,mult);
-  return launch_check("rowreg_kernel");
+  constexpr int TPS=M/RowV::V;
+  const int64_t passes=HERM ? (a.q+1)/2 : a.q;
+  if(TPS <= 256 && rows < 512 && passes >= 2)
+    return rowreg_launch_rp<T,M,HERM,2>(a,f,g,rows,rs,out,os,scale,mult,st);
+  return rowreg_launch_rp<T,M,HERM,1>(a,f,g,rows,rs,out,os,scale,mult,st);
 }
 
 template<typename T, bool HERM>

@@ -32,9 +32,44 @@ __device__ __forceinline__ int wrapN(int e, int N)
 // sit beside the two transforms).
 struct RowV { static constexpr int V=8; };
 
+// acc of residue lane 0 <- sum of acc over the RP residue lanes of the row,
+// through the lanes' exchange areas (real, then imaginary parts).  All threads
+// of the block call it when RP > 1.
+template<typename T, int M, int RP>
+__device__ __forceinline__ void rowreg_combine(cplx<T> *acc, T *xbase, T *xb,
+                                               int rp, int t)
+{
+  if constexpr(RP > 1) {
+    constexpr int V=RowV::V;
+    typedef RPlan<M,V> P;
+    constexpr int TPS=P::TPS;
+#pragma unroll
+    for(int part=0; part < 2; ++part) {
+      if(rp > 0) {
+#pragma unroll
+        for(int k=0; k < V; ++k) xb[padr(t+TPS*k)]=part == 0 ? acc[k].x : acc[k].y;
+      }
+      __syncthreads();
+      if(rp == 0) {
+#pragma unroll
+        for(int k=0; k < V; ++k) {
+          T sum=part == 0 ? acc[k].x : acc[k].y;
+#pragma unroll
+          for(int o=1; o < RP; ++o) sum += xbase[o*P::PAD+padr(t+TPS*k)];
+          if(part == 0) acc[k].x=sum;
+          else acc[k].y=sum;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // Dynamic LDS: M complex twiddles + slots * RPlan<M,V>::PAD scalar words.
-// When TPS > 64 the host launches one slot (row) per block.
-template<typename T, int M, bool HERM>
+// RP = 1: each slot owns a row (one slot per block when TPS > 64).  RP > 1:
+// the block holds one row and its RP slots take every RP-th residue (pair);
+// their sums are added through LDS before the row is written.
+template<typename T, int M, bool HERM, int RP>
 __global__ void __launch_bounds__(512)
 rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
               int64_t rs, cplx<T> *out, int64_t os, T scale, int mult)
@@ -47,19 +82,21 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
   const cplx<T> *zN=(const cplx<T> *) d.zN;
   fill_twr<T,M,V>(tw,zN,(int) d.q,threadIdx.x,blockDim.x);
   __syncthreads();
-  const int slots=blockDim.x/TPS;
+  const int slots=RP > 1 ? 1 : blockDim.x/TPS;      // rows per block
   const int slot=threadIdx.x/TPS, t=threadIdx.x-slot*TPS;
-  T *xb=reinterpret_cast<T *>(tw+M)+slot*P::PAD;
+  const int rp=RP > 1 ? slot : 0;                     // residue lane
+  T *xbase=reinterpret_cast<T *>(tw+M);
+  T *xb=xbase+slot*P::PAD;
   const int N=(int) d.N, q=(int) d.q, L=(int) d.L, H=(int) d.H, k0=(int) d.k0;
   for(int64_t rb=(int64_t) blockIdx.x*slots; rb < rows; rb += (int64_t) gridDim.x*slots) {
-    const int64_t row=rb+slot;
-    if(row >= rows) break;               // only when slots > 1 (TPS <= 64)
+    const int64_t row=rb+(RP > 1 ? 0 : slot);
+    if(row >= rows) break;               // uniform over the block when RP > 1
     const cplx<T> *fr=f+row*rs, *gr=g+row*rs;
     cplx<T> acc[V];
 #pragma unroll
     for(int k=0; k < V; ++k) acc[k]=mk((T) 0,(T) 0);
     if constexpr(!HERM) {
-      for(int r=0; r < q; ++r) {
+      for(int r=rp; r < q; r += RP) {
         cplx<T> X[V], Y[V];
 #pragma unroll
         for(int k=0; k < V; ++k) {
@@ -91,14 +128,17 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
           if(j < L) acc[k]=acc[k]+(r ? mulc(X[k],zN[wrapN((j+k0)*r,N)]) : X[k]);
         }
       }
+      rowreg_combine<T,M,RP>(acc,xbase,xb,rp,t);
+      if(rp == 0) {
 #pragma unroll
-      for(int k=0; k < V; ++k) {
-        const int sigma=t+TPS*k;
-        const int j=(sigma-k0) & (M-1);
-        if(j < L) out[row*os+j]=scale*acc[k];
+        for(int k=0; k < V; ++k) {
+          const int sigma=t+TPS*k;
+          const int j=(sigma-k0) & (M-1);
+          if(j < L) out[row*os+j]=scale*acc[k];
+        }
       }
     } else {
-      for(int r1=0; r1 < q; r1 += 2) {
+      for(int r1=2*rp; r1 < q; r1 += 2*RP) {
         const int r2=r1+1 < q ? r1+1 : -1;
         cplx<T> X[V], Y[V];
 #pragma unroll
@@ -157,12 +197,16 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
           }
         }
       }
+      rowreg_combine<T,M,RP>(acc,xbase,xb,rp,t);
+      if(rp == 0) {
 #pragma unroll
-      for(int k=0; k < V; ++k) {
-        const int sigma=t+TPS*k;
-        if(sigma < H) out[row*os+sigma]=scale*acc[k];
+        for(int k=0; k < V; ++k) {
+          const int sigma=t+TPS*k;
+          if(sigma < H) out[row*os+sigma]=scale*acc[k];
+        }
       }
     }
+    if(RP > 1) __syncthreads();          // exchange areas are reused by the next row
   }
 }
 
