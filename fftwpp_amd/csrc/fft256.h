@@ -82,32 +82,32 @@ __device__ __forceinline__ void lanes_sync()
 
 // 256-point transform with kernel e^{S 2 pi i jk/256} of the sequence held by
 // the 16 lanes j of a group, exchanging real and imaginary parts in two
-// rounds through the group's scalar LDS row area `row` (at least 16*17
-// words).  Lane j writes its stage-1 output u to row[17 j + (u ^ h)] and
-// reads stage-2 input t from row[17 t + (j ^ h)]: the swizzle h (0 or 2, the
+// rounds through the group's scalar LDS row area `row` (at least 16*RS
+// words, RS >= 17).  Lane j writes its stage-1 output u to row[RS j + (u ^ h)]
+// and reads stage-2 input t from row[RS t + (j ^ h)]: the swizzle h (0 or 2, the
 // same for the 16 lanes of a group) lets groups whose areas start on the same
 // bank use disjoint banks.  Stage twiddles w_256^{t j} come from the 16 x 16
 // table tw16[16 t + j], so the 16 lanes read consecutive words.  With WAVE=0
 // every thread of the block must call this (block barriers).
-template<typename T, int S, int WAVE>
+template<typename T, int S, int WAVE, int RS=17>
 __device__ __forceinline__ void fft256s(cplx<T> *v, T *row, int j, int h,
                                         const cplx<T> *tw16)
 {
   dft16<T,S>(v);
   // u ^ h = u + h when bit 1 of u is clear, u - h when it is set (h = 0, 2)
-  T *wa=row+17*j+h, *wb=row+17*j-h;
+  T *wa=row+RS*j+h, *wb=row+RS*j-h;
   const T *ra=row+(j ^ h);
 #pragma unroll
   for(int u=0; u < 16; ++u) ((u & 2) ? wb : wa)[u]=v[u].x;
   lanes_sync<WAVE>();
 #pragma unroll
-  for(int t=0; t < 16; ++t) v[t].x=ra[17*t];
+  for(int t=0; t < 16; ++t) v[t].x=ra[RS*t];
   lanes_sync<WAVE>();
 #pragma unroll
   for(int u=0; u < 16; ++u) ((u & 2) ? wb : wa)[u]=v[u].y;
   lanes_sync<WAVE>();
 #pragma unroll
-  for(int t=0; t < 16; ++t) v[t].y=ra[17*t];
+  for(int t=0; t < 16; ++t) v[t].y=ra[RS*t];
   lanes_sync<WAVE>();
 #pragma unroll
   for(int t=1; t < 16; ++t) {

@@ -313,7 +313,8 @@ rowconv256p_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows,
   const cplx<T> *rt=rtab+256*r;
   const cplx<T> *fr=f+(ok ? row : 0)*rs;
   const cplx<T> *gr=g+(ok ? row : 0)*rs;
-  T *xrow=lds+grp*16*17;
+  constexpr int RS=17;                     // row stride of the exchange areas
+  T *xrow=lds+grp*16*RS;
   cplx<T> a[16], b[16];
 #pragma unroll
   for(int k=0; k < 16; ++k) {
@@ -324,14 +325,14 @@ rowconv256p_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows,
     a[k]=x*w;
     b[k]=y*w;
   }
-  fft256s<T,1,1>(a,xrow,j,0,tw16);
-  fft256s<T,1,1>(b,xrow,j,0,tw16);
+  fft256s<T,1,1,RS>(a,xrow,j,0,tw16);
+  fft256s<T,1,1,RS>(b,xrow,j,0,tw16);
 #pragma unroll
   for(int k=0; k < 16; ++k) {
     if(MULT == 1) a[k]=a[k]*b[k];
     else if(MULT == 3) a[k]=mulc(a[k],b[k]);
   }
-  fft256s<T,-1,1>(a,xrow,j,0,tw16);
+  fft256s<T,-1,1,RS>(a,xrow,j,0,tw16);
 #pragma unroll
   for(int k=0; k < 16; ++k) {
     cplx<T> y=mulc(a[k],(rt+16*k)[j]);
