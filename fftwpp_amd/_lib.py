@@ -85,6 +85,12 @@ SIGNATURES = {
     "fftwpp_create_hconv3d": (P, [Z, Z, Z]),
     "fftwpp_hconv3d_convolve": (None, [P, P, P]),
     "fftwpp_hconv3d_delete": (None, [P]),
+    "fftwpp_conv1d_convolve_device": (I, [P, P, P, P]),
+    "fftwpp_hconv1d_convolve_device": (I, [P, P, P, P]),
+    "fftwpp_conv2d_convolve_device": (I, [P, P, P, P]),
+    "fftwpp_hconv2d_convolve_device": (I, [P, P, P, P]),
+    "fftwpp_conv3d_convolve_device": (I, [P, P, P, P]),
+    "fftwpp_hconv3d_convolve_device": (I, [P, P, P, P]),
     "fftwpp_HermitianSymmetrize": (None, [P]),
     "fftwpp_HermitianSymmetrizeX": (None, [Z, Z, Z, P]),
     "fftwpp_HermitianSymmetrizeXY": (None, [Z, Z, Z, Z, Z, P]),

@@ -172,6 +172,28 @@ static auto guard(const char *where, F f) -> decltype(f())
   }
 }
 
+// Device-pointer variants (extensions): status codes instead of messages.
+template<typename Conv>
+static int convolve_device(const char *where, Conv *conv, void *a, const void *b,
+                           void *stream)
+{
+  if(!conv || !a || !b) return FFTWPP_AMD_ERR_ARG;
+  if(!fftwpp_amd_is_device_pointer(a) || !fftwpp_amd_is_device_pointer(b)) {
+    snprintf(capi_error,sizeof(capi_error),"%s: a and b must be device pointers",where);
+    fftwpp_amd_internal_set_error(capi_error);
+    return FFTWPP_AMD_ERR_ARG;
+  }
+  try {
+    conv->execDevice(a,a,b,true,FFTWPP_AMD_F64,stream);
+  } catch(const std::exception &e) {
+    const fftwpp::Error *fe=dynamic_cast<const fftwpp::Error *>(&e);
+    snprintf(capi_error,sizeof(capi_error),"%s: %s",where,e.what());
+    fftwpp_amd_internal_set_error(capi_error);
+    return fe ? fe->code : FFTWPP_AMD_ERR_ARG;
+  }
+  return FFTWPP_AMD_OK;
+}
+
 extern "C" {
 namespace fftwpp {
 
@@ -269,6 +291,37 @@ void fftwpp_hconv3d_convolve(HybridConvolutionHermitian3 *conv, double *a,
     conv->convolve3->convolve(F);
     return 0;
   });
+}
+
+int fftwpp_conv1d_convolve_device(HybridConvolution *conv, double *a,
+                                  const double *b, void *stream)
+{
+  return convolve_device("fftwpp_conv1d_convolve_device",conv ? conv->convolve : NULL,a,b,stream);
+}
+int fftwpp_hconv1d_convolve_device(HybridConvolutionHermitian *conv, double *a,
+                                   const double *b, void *stream)
+{
+  return convolve_device("fftwpp_hconv1d_convolve_device",conv ? conv->convolve : NULL,a,b,stream);
+}
+int fftwpp_conv2d_convolve_device(HybridConvolution2 *conv, double *a,
+                                  const double *b, void *stream)
+{
+  return convolve_device("fftwpp_conv2d_convolve_device",conv ? conv->convolve2 : NULL,a,b,stream);
+}
+int fftwpp_hconv2d_convolve_device(HybridConvolutionHermitian2 *conv, double *a,
+                                   const double *b, void *stream)
+{
+  return convolve_device("fftwpp_hconv2d_convolve_device",conv ? conv->convolve2 : NULL,a,b,stream);
+}
+int fftwpp_conv3d_convolve_device(HybridConvolution3 *conv, double *a,
+                                  const double *b, void *stream)
+{
+  return convolve_device("fftwpp_conv3d_convolve_device",conv ? conv->convolve3 : NULL,a,b,stream);
+}
+int fftwpp_hconv3d_convolve_device(HybridConvolutionHermitian3 *conv, double *a,
+                                   const double *b, void *stream)
+{
+  return convolve_device("fftwpp_hconv3d_convolve_device",conv ? conv->convolve3 : NULL,a,b,stream);
 }
 
 void fftwpp_HermitianSymmetrize(double *f)

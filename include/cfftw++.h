@@ -94,6 +94,29 @@ void fftwpp_HermitianSymmetrizeX(size_t Hx, size_t Hy, size_t x0,
 void fftwpp_HermitianSymmetrizeXY(size_t Hx, size_t Hy, size_t Hz, size_t x0,
                                   size_t y0, FFTWPP_AMD_DCOMPLEX *f);
 
+/* ------------------------------------------------------------------------
+ * Extensions with no reference counterpart: the same handles on device
+ * pointers and a HIP stream (hipStream_t passed as void *, NULL = default
+ * stream), asynchronous, returning 0 or a negative fftwpp_amd status
+ * (FFTWPP_AMD_ERR_*; fftwpp_amd_last_error() holds the message).  a <- a (*) b
+ * normalised, as *_convolve; b is only read.
+ * ------------------------------------------------------------------------ */
+int fftwpp_conv1d_convolve_device(HybridConvolution *conv, FFTWPP_AMD_DCOMPLEX *a,
+                                  const FFTWPP_AMD_DCOMPLEX *b, void *stream);
+int fftwpp_hconv1d_convolve_device(HybridConvolutionHermitian *conv,
+                                   FFTWPP_AMD_DCOMPLEX *a, const FFTWPP_AMD_DCOMPLEX *b,
+                                   void *stream);
+int fftwpp_conv2d_convolve_device(HybridConvolution2 *conv, FFTWPP_AMD_DCOMPLEX *a,
+                                  const FFTWPP_AMD_DCOMPLEX *b, void *stream);
+int fftwpp_hconv2d_convolve_device(HybridConvolutionHermitian2 *conv,
+                                   FFTWPP_AMD_DCOMPLEX *a, const FFTWPP_AMD_DCOMPLEX *b,
+                                   void *stream);
+int fftwpp_conv3d_convolve_device(HybridConvolution3 *conv, FFTWPP_AMD_DCOMPLEX *a,
+                                  const FFTWPP_AMD_DCOMPLEX *b, void *stream);
+int fftwpp_hconv3d_convolve_device(HybridConvolutionHermitian3 *conv,
+                                   FFTWPP_AMD_DCOMPLEX *a, const FFTWPP_AMD_DCOMPLEX *b,
+                                   void *stream);
+
 #ifdef __cplusplus
 } }
 #endif

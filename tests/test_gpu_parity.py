@@ -433,3 +433,33 @@ def test_conv1d_long_rows_unfused(gpu, oracle, L):
                           torch.tensor(g, dtype=dtype, device=gpu))
         torch.cuda.synchronize()
         assert oracle.rel_l2(out.cpu().numpy(), want) < tol
+
+
+def test_cfftw_device_entry_points(gpu, oracle):
+    """fftwpp_*_convolve_device on device arrays equals *_convolve on host
+    arrays for every handle type, and rejects host pointers."""
+    import torch
+    from fftwpp_amd import _lib
+    L = _lib.lib()
+    rng = np.random.default_rng(123)
+    cases = [("conv1d", (37,), (37,)), ("hconv1d", (19,), (37,)), ("conv2d", (9, 11), (9, 11)),
+             ("hconv2d", (9, 6), (9, 11)), ("conv3d", (5, 6, 7), (5, 6, 7)),
+             ("hconv3d", (5, 7, 4), (5, 7, 7))]
+    for name, shape, lengths in cases:
+        a, b = rnd(rng, *shape), rnd(rng, *shape)
+        if name.startswith("h"):
+            plane = oracle.hplane_of(lengths)
+            a, b = oracle.hplane(a, 0, plane), oracle.hplane(b, 0, plane)
+        h = getattr(L, f"fftwpp_create_{name}")(*lengths)
+        host = np.ascontiguousarray(a.copy())
+        hb = np.ascontiguousarray(b.copy())
+        getattr(L, f"fftwpp_{name}_convolve")(h, host.ctypes.data, hb.ctypes.data)
+        da = torch.tensor(a, device=gpu)
+        db = torch.tensor(b, device=gpu)
+        rc = getattr(L, f"fftwpp_{name}_convolve_device")(h, da.data_ptr(), db.data_ptr(), None)
+        torch.cuda.synchronize()
+        assert rc == 0, (name, L.fftwpp_amd_last_error())
+        assert oracle.rel_l2(da.cpu().numpy(), host) < TOL64, name
+        assert getattr(L, f"fftwpp_{name}_convolve_device")(h, host.ctypes.data, hb.ctypes.data,
+                                                           None) != 0
+        getattr(L, f"fftwpp_{name}_delete")(h)
