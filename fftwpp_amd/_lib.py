@@ -20,6 +20,8 @@ SIGNATURES = {
     "fftwpp_amd_dim_create": (I, [ctypes.POINTER(P), Z, Z, Z, I]),
     "fftwpp_amd_dim_destroy": (I, [P]),
     "fftwpp_amd_max_m": (Z, [I]),
+    "fftwpp_amd_internal_engine": (I, [P, I]),
+    "fftwpp_amd_internal_rows_fit": (I, [P, I]),
     "fftwpp_amd_forward_many": (I, [P, I, P, Z, Z, Z, Z, P, Z, Z, Z, Z, P]),
     "fftwpp_amd_backward_many": (I, [P, I, P, Z, Z, P, Z, Z, Z, Z, Z, Z, I, D, P]),
     "fftwpp_amd_conv_rows": (I, [P, I, I, P, P, Z, Z, P, Z, D, P]),

@@ -660,7 +660,7 @@ void Convolution::execDevice(void *out, const void *f0, const void *f1, bool nor
   double s=normalize ? scale : 1.0;
   size_t n=fft->inputLength();
   if(timeline) timeline->begin(stream);
-  if(fft->m <= fftwpp_amd_max_m(dtype) && fftwpp_amd_internal_rows_fit(fft->dev,dtype)) {
+  if(fftwpp_amd_internal_rows_fit(fft->dev,dtype)) {
     check(fftwpp_amd_conv_rows(fft->dev,dtype,code,f0,f1,1,n,out,n,s,
                                stream),"conv_rows");
     if(timeline) timeline->mark("rows");
@@ -781,8 +781,8 @@ void Convolution2::execDevice(void *out, const void *f0, const void *f1, bool no
 {
   int code=mult >= 0 ? mult : multCode;
   requireBinary(A,B,code);
-  if(ffty->m > fftwpp_amd_max_m(dtype))
-    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner sub-FFT size beyond the LDS limit");
+  if(!fftwpp_amd_internal_rows_fit(ffty->dev,dtype))
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner sub-FFT size beyond the fused row kernels");
   double s=normalize ? scale : 1.0;
   size_t E=eltsize(dtype);
   size_t qx=fftx->q, mx=fftx->m;
@@ -844,8 +844,8 @@ void Convolution3::execDevice(void *out, const void *f0, const void *f1, bool no
 {
   int code=mult >= 0 ? mult : multCode;
   requireBinary(A,B,code);
-  if(fftz->m > fftwpp_amd_max_m(dtype))
-    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner sub-FFT size beyond the LDS limit");
+  if(!fftwpp_amd_internal_rows_fit(fftz->dev,dtype))
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner sub-FFT size beyond the fused row kernels");
   double s=normalize ? scale : 1.0;
   size_t E=eltsize(dtype);
   size_t qx=fftx->q, mx=fftx->m, qy=ffty->q, my=ffty->m;

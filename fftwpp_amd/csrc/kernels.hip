@@ -936,6 +936,18 @@ int fftwpp_amd_internal_rows_fit(const fftwpp_amd_dim *d, int dtype)
   return (4*d->m+lout)*elt <= lds_max();
 }
 
+// Which engine the dispatchers above pick for dimension d (strides below
+// 2^30): stage 0 = column transforms (forward_many / backward_many), 1 =
+// fused rows (conv_rows).  Returns 0 for the generic LDS kernels, 1 for the
+// m = 256 register kernels, 2 for the register-FFT kernels of rowsreg.h.
+int fftwpp_amd_internal_engine(const fftwpp_amd_dim *d, int stage)
+{
+  if(!d) return -1;
+  if(fast256(d)) return 1;
+  if(stage == 0) return cols_reg(d) ? 2 : 0;
+  return rows_reg(d) ? 2 : 0;
+}
+
 int fftwpp_amd_dim_create(fftwpp_amd_dim **out, size_t L, size_t m,
                           size_t q, int kind)
 {
