@@ -38,11 +38,36 @@ template<int M, int V=16> struct RPlan {
   static constexpr int NST=NV+(LAST > 1 ? 1 : 0);
   static constexpr int PAD=M+M/16;                      // exchange words
   __host__ __device__ static constexpr int radix(int s) { return s < NV ? V : LAST; }
-  __host__ __device__ static constexpr int ns(int s) { return s == 0 ? 1 : ns(s-1)*radix(s-1); }
-  __host__ __device__ static constexpr int off(int s) { return s <= 1 ? 0 : off(s-1)+(radix(s-1)-1)*ns(s-1); }
+  // loops rather than recursion, so that calls with a run-time s inline
+  __host__ __device__ static constexpr int ns(int s)
+  {
+    int n=1;
+    for(int i=0; i < s; ++i) n *= radix(i);
+    return n;
+  }
+  __host__ __device__ static constexpr int off(int s)
+  {
+    int o=0;
+    for(int i=1; i < s; ++i) o += (radix(i)-1)*ns(i);
+    return o;
+  }
 };
 
 __device__ __forceinline__ int padr(int e) { return e+(e >> 4); }
+
+// Word of sequence element e in a one-round (CX) exchange area: an XOR
+// swizzle of the low bits by higher bits that makes every exchange of the
+// transform conflict-free on gfx950's LDS (64-bit words: writes served 16
+// lanes per pass over 32 banks, reads 32 lanes over 64 banks; 32-bit words:
+// 32 lanes over 32 banks) for M >= 128 (FP64) or M >= 256 (FP32); e + e/16
+// below that.  Indices stay below M + M/16.
+template<typename T, int M>
+__device__ __forceinline__ int xidx(int e)
+{
+  if constexpr(sizeof(T) == 8 && M >= 128) return e ^ ((e >> 4) & 7) ^ (((e >> 6) & 1) << 3);
+  else if constexpr(sizeof(T) == 4 && M >= 256) return e ^ ((e >> 5) & 7) ^ (((e >> 6) & 3) << 3);
+  else return padr(e);
+}
 
 // Twiddle table (M - 1 complex words) from zN of a dimension with N = M q.
 template<typename T, int M, int V=16>
@@ -51,13 +76,12 @@ __device__ __forceinline__ void fill_twr(cplx<T> *tw, const cplx<T> *zN, int q,
 {
   typedef RPlan<M,V> P;
   for(int e=tid; e < M-1; e += nthreads) {
-    // locate the stage of entry e
-    int s=1;
+    // locate the stage of entry e (constants per unrolled k)
+    int Ns=P::ns(1), R=P::radix(1), off=0;
 #pragma unroll
     for(int k=2; k < P::NST; ++k)
-      if(e >= P::off(k)) s=k;
-    const int Ns=P::ns(s), R=P::radix(s);
-    const int rel=e-P::off(s);
+      if(e >= P::off(k)) { Ns=P::ns(k); R=P::radix(k); off=P::off(k); }
+    const int rel=e-off;
     const int tp=rel/Ns+1, kp=rel-(tp-1)*Ns;
     // e^{2 pi i tp kp/(Ns R)} = zN[q tp kp M/(Ns R)]
     tw[e]=zN[(int64_t) q*tp*kp*(M/(Ns*R))];
@@ -86,7 +110,10 @@ __device__ __forceinline__ void rsync()
 }
 
 // Stage s of the transform (see the top of this file); recurses into s+1.
-template<typename T, int M, int V, int S, int s, bool BLK>
+// CX: xb holds 2 PAD words, real parts in the first PAD and imaginary parts in
+// the second, and each exchange is one round (two synchronisations instead
+// of four).
+template<typename T, int M, int V, int S, int s, bool BLK, bool CX=false>
 __device__ __forceinline__ void fftr_stage(cplx<T> *v, T *xb, const cplx<T> *tw,
                                            int t)
 {
@@ -113,7 +140,29 @@ __device__ __forceinline__ void fftr_stage(cplx<T> *v, T *xb, const cplx<T> *tw,
 #pragma unroll
     for(int tp=0; tp < R; ++tp) v[b+tp*NB]=u[tp];
   }
-  if constexpr(s+1 < P::NST) {
+  if constexpr(s+1 < P::NST && CX) {
+    // exchange: output u of butterfly b goes to (j/Ns) Ns R + k' + u Ns
+#pragma unroll
+    for(int b=0; b < NB; ++b) {
+      const int j=t+TPS*b;
+      const int kp=j & (Ns-1);
+      const int base=(j-kp)*R+kp;
+#pragma unroll
+      for(int u=0; u < R; ++u) {
+        const int e=xidx<T,M>(base+u*Ns);
+        xb[e]=v[b+u*NB].x;
+        xb[P::PAD+e]=v[b+u*NB].y;
+      }
+    }
+    rsync<TPS,BLK>();
+#pragma unroll
+    for(int k=0; k < V; ++k) {
+      const int e=xidx<T,M>(t+TPS*k);
+      v[k]=mk(xb[e],xb[P::PAD+e]);
+    }
+    rsync<TPS,BLK>();
+    fftr_stage<T,M,V,S,s+1,BLK,CX>(v,xb,tw,t);
+  } else if constexpr(s+1 < P::NST) {
     // exchange: output u of butterfly b goes to (j/Ns) Ns R + k' + u Ns
 #pragma unroll
     for(int part=0; part < 2; ++part) {
@@ -143,19 +192,19 @@ __device__ __forceinline__ void fftr_stage(cplx<T> *v, T *xb, const cplx<T> *tw,
 // with kernel e^{S 2 pi i jk/M}; xb is the sequence's exchange area
 // (RPlan<M,V>::PAD scalar words), tw the twiddle table of RPlan<M,V>.  When
 // TPS > 64 every thread of the block must call this.
-template<typename T, int M, int V, int S>
+template<typename T, int M, int V, int S, bool CX=false>
 __device__ __forceinline__ void fftr(cplx<T> *v, T *xb, const cplx<T> *tw, int t)
 {
-  fftr_stage<T,M,V,S,0,false>(v,xb,tw,t);
+  fftr_stage<T,M,V,S,0,false,CX>(v,xb,tw,t);
 }
 
 // The same transform when the threads of a sequence are not consecutive
 // lanes (block barriers; every thread of the block must call it).
-template<typename T, int M, int V, int S>
+template<typename T, int M, int V, int S, bool CX=false>
 __device__ __forceinline__ void fftr_blk(cplx<T> *v, T *xb, const cplx<T> *tw,
                                          int t)
 {
-  fftr_stage<T,M,V,S,0,true>(v,xb,tw,t);
+  fftr_stage<T,M,V,S,0,true,CX>(v,xb,tw,t);
 }
 
 } // namespace fftwpp_amd

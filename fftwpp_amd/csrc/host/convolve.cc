@@ -371,6 +371,23 @@ static Choice choose(size_t L, size_t M, Application &app, size_t C, size_t S,
       }
     }
   }
+  // Complex and centred rows (C = 1) run fused with their residues in
+  // sequence.  At m = 1024 the register row kernel is faster at m = 2048 when
+  // that lowers q, although the padded length may grow: measured 10% for a
+  // 1024-point row with M = 2048 (q 2 -> 1).  (Hermitian rows of length 2047
+  // gain too at q 3 -> 2, but that needs p = 1 with conjugate pairs, which
+  // the reference's validity rules exclude, convolve.h:721-723.)
+  if(C == 1 && kind != 2 && best_m == 1024 && L <= 2048) {
+    size_t p, n, q, p2, n2, q2;
+    fftBase::parameters(L,M,best_m,centered,p,n,q);
+    fftBase::parameters(L,M,2048,centered,p2,n2,q2);
+    bool ok=kind == 0 ? fftPad::valid(2048,p2,q2,n2,1,S) :
+      fftPadCentered::validCentered(2048,p2,q2,n2,1,S);
+    if(ok && q2 < q) {
+      best_m=2048;
+      best_N=2048*q2;
+    }
+  }
   if(best_m == 0) {
     // explicit padding as the last resort (convolve.cc:431-434)
     c.m=nextfftsize(M);

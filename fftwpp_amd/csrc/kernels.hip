@@ -485,7 +485,7 @@ static bool cols_reg(const fftwpp_amd_dim *d)
   const size_t m=d->m;
   if(m < 16 || m > 4096 || (m & (m-1))) return false;
   if(d->kind == FFTWPP_AMD_KIND_HERMITIAN) return false;
-  return d->L <= 2*m;
+  return d->L <= 2*m && m*d->q < ((size_t) 1 << 30);
 }
 
 template<typename T, int M>
@@ -761,6 +761,7 @@ static bool rows_reg(const fftwpp_amd_dim *d)
 {
   const size_t m=d->m;
   if(m < 16 || m > 4096 || (m & (m-1))) return false;
+  if(m*d->q >= ((size_t) 1 << 30)) return false;   // 32-bit twiddle exponents
   if(d->kind == FFTWPP_AMD_KIND_HERMITIAN) return (d->L+1)/2 <= m;
   return d->L <= m;
 }
@@ -777,7 +778,7 @@ static int rowreg_launch_rp(const DimArgs &a, const void *f, const void *g,
   constexpr int TPS=M/V;
   const size_t nt=RP > 1 ? RP*TPS : (TPS > 64 ? TPS : 256);
   const size_t slots=nt/TPS;
-  const size_t lds=M*sizeof(cplx<T>)+slots*RPlan<M,V>::PAD*sizeof(T);
+  const size_t lds=M*sizeof(cplx<T>)+slots*2*RPlan<M,V>::PAD*sizeof(T);
   int rc=allow_lds(rowreg_kernel<T,M,HERM,RP>,lds);
   if(rc) return rc;
   const size_t rpb=RP > 1 ? 1 : slots;

@@ -22,10 +22,12 @@
 
 namespace fftwpp_amd {
 
+// e mod N for -N <= e < 2N: the twiddle exponents k r of these kernels have
+// |k| < M (rows) or -M <= k < 2M (columns with p <= 2) and 0 <= r < q, so
+// they lie in [-N, 2N) with N = M q (kept below 2^30 by the dispatchers).
 __device__ __forceinline__ int wrapN(int e, int N)
 {
-  e %= N;
-  return e < 0 ? e+N : e;
+  return e < 0 ? e+N : (e >= N ? e-N : e);
 }
 
 // Values per thread: 8 in both precisions (the register sums over residues
@@ -33,8 +35,8 @@ __device__ __forceinline__ int wrapN(int e, int N)
 struct RowV { static constexpr int V=8; };
 
 // acc of residue lane 0 <- sum of acc over the RP residue lanes of the row,
-// through the lanes' exchange areas (real, then imaginary parts).  All threads
-// of the block call it when RP > 1.
+// through the lanes' exchange areas (2 PAD words each: real parts, then
+// imaginary parts).  All threads of the block call it when RP > 1.
 template<typename T, int M, int RP>
 __device__ __forceinline__ void rowreg_combine(cplx<T> *acc, T *xbase, T *xb,
                                                int rp, int t)
@@ -43,29 +45,31 @@ __device__ __forceinline__ void rowreg_combine(cplx<T> *acc, T *xbase, T *xb,
     constexpr int V=RowV::V;
     typedef RPlan<M,V> P;
     constexpr int TPS=P::TPS;
+    if(rp > 0) {
 #pragma unroll
-    for(int part=0; part < 2; ++part) {
-      if(rp > 0) {
-#pragma unroll
-        for(int k=0; k < V; ++k) xb[padr(t+TPS*k)]=part == 0 ? acc[k].x : acc[k].y;
+      for(int k=0; k < V; ++k) {
+        const int e=xidx<T,M>(t+TPS*k);
+        xb[e]=acc[k].x;
+        xb[P::PAD+e]=acc[k].y;
       }
-      __syncthreads();
-      if(rp == 0) {
-#pragma unroll
-        for(int k=0; k < V; ++k) {
-          T sum=part == 0 ? acc[k].x : acc[k].y;
-#pragma unroll
-          for(int o=1; o < RP; ++o) sum += xbase[o*P::PAD+padr(t+TPS*k)];
-          if(part == 0) acc[k].x=sum;
-          else acc[k].y=sum;
-        }
-      }
-      __syncthreads();
     }
+    __syncthreads();
+    if(rp == 0) {
+#pragma unroll
+      for(int k=0; k < V; ++k) {
+        const int e=xidx<T,M>(t+TPS*k);
+#pragma unroll
+        for(int o=1; o < RP; ++o)
+          acc[k]=acc[k]+mk(xbase[2*o*P::PAD+e],xbase[(2*o+1)*P::PAD+e]);
+      }
+    }
+    __syncthreads();
   }
 }
 
-// Dynamic LDS: M complex twiddles + slots * RPlan<M,V>::PAD scalar words.
+// Dynamic LDS: M complex twiddles + slots * 2 RPlan<M,V>::PAD scalar words
+// (each slot's exchange area holds a whole complex sequence, so every
+// exchange is one round).
 // RP = 1: each slot owns a row (one slot per block when TPS > 64).  RP > 1:
 // the block holds one row and its RP slots take every RP-th residue (pair);
 // their sums are added through LDS before the row is written.
@@ -86,8 +90,8 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
   const int slot=threadIdx.x/TPS, t=threadIdx.x-slot*TPS;
   const int rp=RP > 1 ? slot : 0;                     // residue lane
   T *xbase=reinterpret_cast<T *>(tw+M);
-  T *xb=xbase+slot*P::PAD;
-  const int N=(int) d.N, q=(int) d.q, L=(int) d.L, H=(int) d.H, k0=(int) d.k0;
+  T *xb=xbase+slot*2*P::PAD;
+  const int q=(int) d.q, L=(int) d.L, H=(int) d.H, k0=(int) d.k0;
   for(int64_t rb=(int64_t) blockIdx.x*slots; rb < rows; rb += (int64_t) gridDim.x*slots) {
     const int64_t row=rb+(RP > 1 ? 0 : slot);
     if(row >= rows) break;               // uniform over the block when RP > 1
@@ -97,6 +101,8 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
     for(int k=0; k < V; ++k) acc[k]=mk((T) 0,(T) 0);
     if constexpr(!HERM) {
       for(int r=rp; r < q; r += RP) {
+        // w_N^{(j+k0) r}: j + k0 is sigma or sigma - M
+        const cplx<T> bt=zN[t*r], zm=conjg(zN[M*r]);
         cplx<T> X[V], Y[V];
 #pragma unroll
         for(int k=0; k < V; ++k) {
@@ -107,7 +113,8 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
             a=fr[j];
             b=gr[j];
             if(r) {
-              const cplx<T> w=zN[wrapN((j+k0)*r,N)];
+              cplx<T> w=bt*zN[TPS*k*r];
+              if(j+k0 < 0) w=w*zm;
               a=w*a;
               b=w*b;
             }
@@ -115,17 +122,24 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
           X[k]=a;
           Y[k]=b;
         }
-        fftr<T,M,V,1>(X,xb,tw,t);
-        fftr<T,M,V,1>(Y,xb,tw,t);
+        fftr<T,M,V,1,true>(X,xb,tw,t);
+        fftr<T,M,V,1,true>(Y,xb,tw,t);
 #pragma unroll
         for(int k=0; k < V; ++k)
           X[k]=mult == 1 ? X[k]*Y[k] : (mult == 3 ? mulc(X[k],Y[k]) : X[k]);
-        fftr<T,M,V,-1>(X,xb,tw,t);
+        fftr<T,M,V,-1,true>(X,xb,tw,t);
 #pragma unroll
         for(int k=0; k < V; ++k) {
           const int sigma=t+TPS*k;
           const int j=(sigma-k0) & (M-1);
-          if(j < L) acc[k]=acc[k]+(r ? mulc(X[k],zN[wrapN((j+k0)*r,N)]) : X[k]);
+          if(j < L) {
+            if(r) {
+              cplx<T> w=bt*zN[TPS*k*r];
+              if(j+k0 < 0) w=w*zm;
+              acc[k]=acc[k]+mulc(X[k],w);
+            } else
+              acc[k]=acc[k]+X[k];
+          }
         }
       }
       rowreg_combine<T,M,RP>(acc,xbase,xb,rp,t);
@@ -140,58 +154,62 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
     } else {
       for(int r1=2*rp; r1 < q; r1 += 2*RP) {
         const int r2=r1+1 < q ? r1+1 : -1;
+        // w_N^{sigma r} = w_N^{t r} w_N^{TPS k r}; w_N^{(sigma - M) r} adds
+        // the factor conj w_N^{M r}
+        const cplx<T> zero=mk((T) 0,(T) 0);
+        const cplx<T> b1=zN[t*r1], m1=conjg(zN[M*r1]);
+        const cplx<T> b2=r2 >= 0 ? zN[t*r2] : zero, m2=r2 >= 0 ? conjg(zN[M*r2]) : zero;
         cplx<T> X[V], Y[V];
 #pragma unroll
         for(int k=0; k < V; ++k) {
           const int sigma=t+TPS*k;
+          const cplx<T> w1=b1*zN[TPS*k*r1];
+          const cplx<T> w2=r2 >= 0 ? b2*zN[TPS*k*r2] : zero;
           cplx<T> a=mk((T) 0,(T) 0), b=a;
           // k = sigma (when < H) and k = sigma - M (when > -H)
-#pragma unroll
-          for(int h=0; h < 2; ++h) {
-            const int kk=h == 0 ? sigma : sigma-M;
-            if(h == 0 ? kk < H : kk > -H) {
-              const cplx<T> fv=kk >= 0 ? fr[kk] : conjg(fr[-kk]);
-              const cplx<T> gv=kk >= 0 ? gr[kk] : conjg(gr[-kk]);
-              cplx<T> w=zN[wrapN(kk*r1,N)];
-              if(r2 >= 0) w=w+muli(zN[wrapN(kk*r2,N)],1);
-              a=a+w*fv;
-              b=b+w*gv;
-            }
+          if(sigma < H) {
+            const cplx<T> w=w1+muli(w2,1);
+            a=w*fr[sigma];
+            b=w*gr[sigma];
+          }
+          if(M-sigma < H) {
+            const cplx<T> w=w1*m1+muli(w2*m2,1);
+            a=a+w*conjg(fr[M-sigma]);
+            b=b+w*conjg(gr[M-sigma]);
           }
           X[k]=a;
           Y[k]=b;
         }
-        fftr<T,M,V,1>(X,xb,tw,t);
-        fftr<T,M,V,1>(Y,xb,tw,t);
+        fftr<T,M,V,1,true>(X,xb,tw,t);
+        fftr<T,M,V,1,true>(Y,xb,tw,t);
 #pragma unroll
         for(int k=0; k < V; ++k) X[k]=mk(X[k].x*Y[k].x,X[k].y*Y[k].y);
-        fftr<T,M,V,-1>(X,xb,tw,t);
-        // mirror elements Z(M - sigma) through LDS, real then imaginary parts
+        fftr<T,M,V,-1,true>(X,xb,tw,t);
+        // mirror elements Z(M - sigma) through LDS
         cplx<T> Zm[V];
 #pragma unroll
-        for(int part=0; part < 2; ++part) {
-#pragma unroll
-          for(int k=0; k < V; ++k)
-            xb[padr(t+TPS*k)]=part == 0 ? X[k].x : X[k].y;
-          rsync<TPS>();
-#pragma unroll
-          for(int k=0; k < V; ++k) {
-            const T val=xb[padr((M-(t+TPS*k)) & (M-1))];
-            if(part == 0) Zm[k].x=val;
-            else Zm[k].y=val;
-          }
-          rsync<TPS>();
+        for(int k=0; k < V; ++k) {
+          const int e=xidx<T,M>(t+TPS*k);
+          xb[e]=X[k].x;
+          xb[P::PAD+e]=X[k].y;
         }
+        rsync<TPS>();
+#pragma unroll
+        for(int k=0; k < V; ++k) {
+          const int e=xidx<T,M>((M-(t+TPS*k)) & (M-1));
+          Zm[k]=mk(xb[e],xb[P::PAD+e]);
+        }
+        rsync<TPS>();
 #pragma unroll
         for(int k=0; k < V; ++k) {
           const int sigma=t+TPS*k;
           if(sigma < H) {
             const cplx<T> z=X[k], zc=conjg(Zm[k]);
             const cplx<T> y1=(T) 0.5*(z+zc);
-            cplx<T> v=mulc(y1,zN[wrapN(sigma*r1,N)]);
+            cplx<T> v=mulc(y1,b1*zN[TPS*k*r1]);
             if(r2 >= 0) {
               const cplx<T> y2=muli((T) 0.5*(z-zc),-1);
-              v=v+mulc(y2,zN[wrapN(sigma*r2,N)]);
+              v=v+mulc(y2,b2*zN[TPS*k*r2]);
             }
             acc[k]=acc[k]+v;
           }
