@@ -139,7 +139,8 @@ template<typename T> struct BatchV { static constexpr int V=sizeof(T) == 8 ? 8 :
 // destination (thread-private words: thread t owns j = t + TPS k + M b).
 // When TPS > 64 the host launches one slot per block, so the block barriers
 // inside fftr() see the same calls from every thread.  Dynamic LDS: M
-// complex twiddles + slots * RPlan<M,V>::PAD scalar words.
+// complex twiddles + slots * 2 RPlan<M,V>::PAD scalar words (one-round
+// exchanges).
 template<typename T, int M>
 __global__ void __launch_bounds__(512)
 batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
@@ -156,23 +157,39 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
   __syncthreads();
   const int slots=blockDim.x/TPS;
   const int slot=threadIdx.x/TPS, t=threadIdx.x-slot*TPS;
-  T *xb=reinterpret_cast<T *>(tw+M)+slot*P::PAD;
+  T *xb=reinterpret_cast<T *>(tw+M)+slot*2*P::PAD;
   const int64_t G=(d.q+D-1)/D;
   const int64_t units=count*G;
   for(int64_t ub=(int64_t) blockIdx.x*slots; ub < units; ub += (int64_t) gridDim.x*slots) {
     const int64_t unit=ub+slot;
     if(unit >= units) break;            // only when slots > 1 (TPS <= 64)
-    const int64_t ii=unit/G, grp=unit-ii*G;
+    int64_t ii, grp;
+    if(G > 1 && slots == 1) {
+      // Blocks are dealt to the 8 XCDs round robin (block b on XCD b mod 8)
+      // and the host makes the grid a multiple of 8, so unit u runs on XCD
+      // u mod 8.  Units c 8G + g 8 + w (g < G) are the G residue groups of
+      // item 8c + w: they run together on one XCD and share its L2 for
+      // their reads of the item.  The last, partial chunk of rem < 8 items
+      // maps u' = g rem + w.
+      const int64_t chunk=8*G, c=unit/chunk, full=count/8;
+      const int64_t u=unit-c*chunk;
+      const int64_t W=c < full ? 8 : count-8*full;
+      ii=8*c+u % W;
+      grp=u/W;
+    } else {
+      ii=unit/G;
+      grp=unit-ii*G;
+    }
     const BatchItem it=items[ii];
-    const int rbeg=(int) (grp*D), rend=(int) min(d.q,(int64_t) rbeg+D);
     const T sc=G == 1 ? scale : (T) 1;
     cplx<T> *dst=G == 1 ? out+it.off : part+grp*ptotal+it.poff;
     const cplx<T> *fi=f+it.off, *gi=g+it.off;
     const int L=(int) it.L;
+    const int qi=(int) d.q;
+    const int rbeg=(int) (grp*D), rend=(int) min(d.q,(int64_t) rbeg+D);
     for(int r=rbeg; r < rend; ++r) {
       // w_N^{(sigma + M b) r} = w_N^{sigma r} w_q^{b r}: one per-thread factor
       // and one factor shared by the slot per block b
-      const int qi=(int) d.q;
       cplx<T> X[V], Y[V];
 #pragma unroll
       for(int k=0; k < V; ++k) X[k]=Y[k]=mk((T) 0,(T) 0);
@@ -197,12 +214,12 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
           Y[k]=w*Y[k];
         }
       }
-      fftr<T,M,V,1>(X,xb,tw,t);
-      fftr<T,M,V,1>(Y,xb,tw,t);
+      fftr<T,M,V,1,true>(X,xb,tw,t);
+      fftr<T,M,V,1,true>(Y,xb,tw,t);
 #pragma unroll
       for(int k=0; k < V; ++k)
         X[k]=mult == 1 ? X[k]*Y[k] : (mult == 3 ? mulc(X[k],Y[k]) : X[k]);
-      fftr<T,M,V,-1>(X,xb,tw,t);
+      fftr<T,M,V,-1,true>(X,xb,tw,t);
       if(r) {
 #pragma unroll
         for(int k=0; k < V; ++k) X[k]=mulc(X[k],zN[(t+TPS*k)*r]);

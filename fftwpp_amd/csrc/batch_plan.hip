@@ -190,6 +190,9 @@ Tile make_tile(const fftwpp_amd_batch &h, const Cls &c, size_t TR,
   size_t slots=tps ? std::max<size_t>(1,std::max(threads,tps)/tps) : 1;
   if(tps > 64) slots=1;
   t.nblocks=std::min(ceilquot(units,slots),(size_t) (8*h.cus));
+  // several residue groups per item on one block each: a multiple of 8
+  // blocks keeps an item's groups on one XCD (the kernel's unit mapping)
+  if(tps && slots == 1 && groups(c,t) > 1) t.nblocks=ceilquot(t.nblocks,8)*8;
   return t;
 }
 

@@ -59,14 +59,20 @@ __device__ __forceinline__ int padr(int e) { return e+(e >> 4); }
 // swizzle of the low bits by higher bits that makes every exchange of the
 // transform conflict-free on gfx950's LDS (64-bit words: writes served 16
 // lanes per pass over 32 banks, reads 32 lanes over 64 banks; 32-bit words:
-// 32 lanes over 32 banks) for M >= 128 (FP64) or M >= 256 (FP32); e + e/16
-// below that.  Indices stay below M + M/16.
-template<typename T, int M>
+// 32 lanes over 32 banks) for M >= 128 (FP64, V = 8), M >= 256 (FP32, V = 8)
+// or M >= 512 (FP32, V = 16); e + e/16 below that.  Indices stay below
+// M + M/16.
+template<typename T, int M, int V>
 __device__ __forceinline__ int xidx(int e)
 {
-  if constexpr(sizeof(T) == 8 && M >= 128) return e ^ ((e >> 4) & 7) ^ (((e >> 6) & 1) << 3);
-  else if constexpr(sizeof(T) == 4 && M >= 256) return e ^ ((e >> 5) & 7) ^ (((e >> 6) & 3) << 3);
-  else return padr(e);
+  if constexpr(sizeof(T) == 8 && V == 8 && M >= 128)
+    return e ^ ((e >> 4) & 7) ^ (((e >> 6) & 1) << 3);
+  else if constexpr(sizeof(T) == 4 && V == 8 && M >= 256)
+    return e ^ ((e >> 5) & 7) ^ (((e >> 6) & 3) << 3);
+  else if constexpr(sizeof(T) == 4 && V == 16 && M >= 512)
+    return e ^ ((e >> 4) & 7) ^ (((e >> 7) & 3) << 3);
+  else
+    return padr(e);
 }
 
 // Twiddle table (M - 1 complex words) from zN of a dimension with N = M q.
@@ -149,7 +155,7 @@ __device__ __forceinline__ void fftr_stage(cplx<T> *v, T *xb, const cplx<T> *tw,
       const int base=(j-kp)*R+kp;
 #pragma unroll
       for(int u=0; u < R; ++u) {
-        const int e=xidx<T,M>(base+u*Ns);
+        const int e=xidx<T,M,V>(base+u*Ns);
         xb[e]=v[b+u*NB].x;
         xb[P::PAD+e]=v[b+u*NB].y;
       }
@@ -157,7 +163,7 @@ __device__ __forceinline__ void fftr_stage(cplx<T> *v, T *xb, const cplx<T> *tw,
     rsync<TPS,BLK>();
 #pragma unroll
     for(int k=0; k < V; ++k) {
-      const int e=xidx<T,M>(t+TPS*k);
+      const int e=xidx<T,M,V>(t+TPS*k);
       v[k]=mk(xb[e],xb[P::PAD+e]);
     }
     rsync<TPS,BLK>();

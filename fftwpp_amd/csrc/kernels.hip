@@ -694,7 +694,7 @@ static void batch_reg_shape(size_t m, size_t elt, size_t threads,
   if(n > 512) n=512/tps*tps;
   const size_t pad=m+m/16;
   *nt=n;
-  *lds=m*elt+(n/tps)*pad*(elt/2);
+  *lds=m*elt+(n/tps)*2*pad*(elt/2);
 }
 
 template<typename T, int M>

@@ -48,7 +48,7 @@ __device__ __forceinline__ void rowreg_combine(cplx<T> *acc, T *xbase, T *xb,
     if(rp > 0) {
 #pragma unroll
       for(int k=0; k < V; ++k) {
-        const int e=xidx<T,M>(t+TPS*k);
+        const int e=xidx<T,M,V>(t+TPS*k);
         xb[e]=acc[k].x;
         xb[P::PAD+e]=acc[k].y;
       }
@@ -57,7 +57,7 @@ __device__ __forceinline__ void rowreg_combine(cplx<T> *acc, T *xbase, T *xb,
     if(rp == 0) {
 #pragma unroll
       for(int k=0; k < V; ++k) {
-        const int e=xidx<T,M>(t+TPS*k);
+        const int e=xidx<T,M,V>(t+TPS*k);
 #pragma unroll
         for(int o=1; o < RP; ++o)
           acc[k]=acc[k]+mk(xbase[2*o*P::PAD+e],xbase[(2*o+1)*P::PAD+e]);
@@ -189,14 +189,14 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
         cplx<T> Zm[V];
 #pragma unroll
         for(int k=0; k < V; ++k) {
-          const int e=xidx<T,M>(t+TPS*k);
+          const int e=xidx<T,M,V>(t+TPS*k);
           xb[e]=X[k].x;
           xb[P::PAD+e]=X[k].y;
         }
         rsync<TPS>();
 #pragma unroll
         for(int k=0; k < V; ++k) {
-          const int e=xidx<T,M>((M-(t+TPS*k)) & (M-1));
+          const int e=xidx<T,M,V>((M-(t+TPS*k)) & (M-1));
           Zm[k]=mk(xb[e],xb[P::PAD+e]);
         }
         rsync<TPS>();
