@@ -63,6 +63,18 @@ struct DimArgs {
   const void *zN; // zN[j] = e^{2 pi i j/N}, j < N, in the kernel's precision
 };
 
+// Column tile of block x of a grid row of gx blocks.  Blocks are dealt to
+// the 8 XCDs round robin by linear id, so block x runs on XCD x mod 8 (when
+// gx is a multiple of 8); runs of gx/8 consecutive tiles are given to one
+// XCD, so that neighbouring tiles, which share cache lines when a tile row is
+// narrower than a line, share that XCD's L2 instead of fetching the line
+// twice.  A bijection of [0, gx).
+__device__ __forceinline__ int64_t xcd_tile(int64_t x, int64_t gx)
+{
+  const int64_t per=gx/8;
+  return x < 8*per ? (x % 8)*per+x/8 : x;
+}
+
 template<typename T>
 __device__ __forceinline__ cplx<T> ztab(const DimArgs &d, int64_t j)
 {

@@ -109,7 +109,7 @@ __global__ void fwd_many_kernel(DimArgs d, const cplx<T> *f, int64_t S,
   const int m=(int) d.m;
   const int ld=m+1;
   cplx<T> *B=A+(int64_t) TC*ld;
-  const int64_t c0=(int64_t) blockIdx.x*TC;
+  const int64_t c0=xcd_tile(blockIdx.x,gridDim.x)*TC;
   const int64_t b=blockIdx.y;
   const int nc=(int) min((int64_t) TC,C-c0);
   const cplx<T> *fb=f+b*bsi+c0;
@@ -147,7 +147,7 @@ __global__ void bwd_many_kernel(DimArgs d, const cplx<T> *F, int64_t SF,
   const int m=(int) d.m;
   const int ld=m+1;
   cplx<T> *B=A+(int64_t) TC*ld;
-  const int64_t c0=(int64_t) blockIdx.x*TC;
+  const int64_t c0=xcd_tile(blockIdx.x,gridDim.x)*TC;
   const int64_t b=blockIdx.y;
   const int nc=(int) min((int64_t) TC,C-c0);
   const cplx<T> *Fb=F+b*bsi+c0;

@@ -144,7 +144,7 @@ colfwd256_kernel(const cplx<T> *f, int64_t S, int64_t C, int64_t bsi,
   __syncthreads();
   const int c=threadIdx.x % TC, j=threadIdx.x/TC;
   T *row=reinterpret_cast<T *>(tw16+256)+c*Col256<T>::CS;
-  const int64_t c0=(int64_t) blockIdx.x*TC;
+  const int64_t c0=xcd_tile(blockIdx.x,gridDim.x)*TC;
   const bool ok=c0+c < C;
   const cplx<T> *fb=f+(int64_t) blockIdx.y*bsi+c0;
   cplx<T> *Fb=F+(int64_t) blockIdx.y*bso+c0;
@@ -167,7 +167,7 @@ colbwd256_kernel(const cplx<T> *F, int64_t SF, int64_t bsi, cplx<T> *f,
   __syncthreads();
   const int c=threadIdx.x % TC, j=threadIdx.x/TC;
   T *row=reinterpret_cast<T *>(tw16+256)+c*Col256<T>::CS;
-  const int64_t c0=(int64_t) blockIdx.x*TC;
+  const int64_t c0=xcd_tile(blockIdx.x,gridDim.x)*TC;
   const bool ok=c0+c < C;
   const cplx<T> *Fb=F+(int64_t) blockIdx.y*bsi+c0;
   cplx<T> *fb=f+(int64_t) blockIdx.y*bso+c0;

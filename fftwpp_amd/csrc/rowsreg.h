@@ -264,7 +264,7 @@ colfwd_reg_kernel(DimArgs d, const cplx<T> *f, int64_t S, int64_t C,
   __syncthreads();
   const int c=threadIdx.x % TC, t=threadIdx.x/TC;
   T *xb=reinterpret_cast<T *>(tw+M)+c*CR::CS;
-  const int64_t c0=(int64_t) blockIdx.x*TC;
+  const int64_t c0=xcd_tile(blockIdx.x,gridDim.x)*TC;
   const bool ok=c0+c < C;
   const cplx<T> *fb=f+(int64_t) blockIdx.y*bsi+c0+c;
   cplx<T> *Fb=F+(int64_t) blockIdx.y*bso+c0+c;
@@ -318,7 +318,7 @@ colbwd_reg_kernel(DimArgs d, const cplx<T> *F, int64_t SF, int64_t bsi,
   __syncthreads();
   const int c=threadIdx.x % TC, t=threadIdx.x/TC;
   T *xb=reinterpret_cast<T *>(tw+M)+c*CR::CS;
-  const int64_t c0=(int64_t) blockIdx.x*TC;
+  const int64_t c0=xcd_tile(blockIdx.x,gridDim.x)*TC;
   const bool ok=c0+c < C;
   const cplx<T> *Fb=F+(int64_t) blockIdx.y*bsi+c0+c;
   cplx<T> *fb=f+(int64_t) blockIdx.y*bso+c0+c;
