@@ -115,13 +115,19 @@ __device__ __forceinline__ void rsync()
     __syncthreads();
 }
 
+// Word of element e in a one-round exchange area (the default index map).
+template<typename T, int M, int V> struct RowXI {
+  static __device__ __forceinline__ int at(int e) { return xidx<T,M,V>(e); }
+};
+
 // Stage s of the transform (see the top of this file); recurses into s+1.
 // CX: xb holds 2 PAD words, real parts in the first PAD and imaginary parts in
 // the second, and each exchange is one round (two synchronisations instead
-// of four).
-template<typename T, int M, int V, int S, int s, bool BLK, bool CX=false>
+// of four); element e sits at word XI::at(e) ^ xm of each half.
+template<typename T, int M, int V, int S, int s, bool BLK, bool CX=false,
+         class XI=RowXI<T,M,V>>
 __device__ __forceinline__ void fftr_stage(cplx<T> *v, T *xb, const cplx<T> *tw,
-                                           int t)
+                                           int t, int xm=0)
 {
   typedef RPlan<M,V> P;
   constexpr int TPS=P::TPS;
@@ -155,7 +161,7 @@ __device__ __forceinline__ void fftr_stage(cplx<T> *v, T *xb, const cplx<T> *tw,
       const int base=(j-kp)*R+kp;
 #pragma unroll
       for(int u=0; u < R; ++u) {
-        const int e=xidx<T,M,V>(base+u*Ns);
+        const int e=XI::at(base+u*Ns) ^ xm;
         xb[e]=v[b+u*NB].x;
         xb[P::PAD+e]=v[b+u*NB].y;
       }
@@ -163,11 +169,11 @@ __device__ __forceinline__ void fftr_stage(cplx<T> *v, T *xb, const cplx<T> *tw,
     rsync<TPS,BLK>();
 #pragma unroll
     for(int k=0; k < V; ++k) {
-      const int e=xidx<T,M,V>(t+TPS*k);
+      const int e=XI::at(t+TPS*k) ^ xm;
       v[k]=mk(xb[e],xb[P::PAD+e]);
     }
     rsync<TPS,BLK>();
-    fftr_stage<T,M,V,S,s+1,BLK,CX>(v,xb,tw,t);
+    fftr_stage<T,M,V,S,s+1,BLK,CX,XI>(v,xb,tw,t,xm);
   } else if constexpr(s+1 < P::NST) {
     // exchange: output u of butterfly b goes to (j/Ns) Ns R + k' + u Ns
 #pragma unroll
@@ -211,6 +217,14 @@ __device__ __forceinline__ void fftr_blk(cplx<T> *v, T *xb, const cplx<T> *tw,
                                          int t)
 {
   fftr_stage<T,M,V,S,0,true,CX>(v,xb,tw,t);
+}
+
+// fftr_blk with one-round exchanges through the index map XI, XORed by xm.
+template<typename T, int M, int V, int S, class XI>
+__device__ __forceinline__ void fftr_blkx(cplx<T> *v, T *xb, const cplx<T> *tw,
+                                          int t, int xm)
+{
+  fftr_stage<T,M,V,S,0,true,true,XI>(v,xb,tw,t,xm);
 }
 
 } // namespace fftwpp_amd

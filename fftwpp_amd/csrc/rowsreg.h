@@ -235,17 +235,38 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
 // blocks of M per column (p <= 2), complex or centred data.  Lane (c, t) =
 // (threadIdx.x % TC, threadIdx.x / TC): consecutive lanes take consecutive
 // columns, so every load and store of a wave covers TC adjacent columns; the
-// TPS threads of a column span waves and exchange with block barriers through
-// the column's LDS area (CS scalar words apart, a stride chosen to spread the
-// columns of a wave over the banks).  Dynamic LDS: M complex twiddles +
-// TC * CS scalar words.
+// TPS threads of a column span waves and exchange with block barriers, one
+// round per stage, through the column's LDS area of CS = 2 PAD + X words
+// (real parts, then imaginary parts PAD words on).  Element e of column c
+// sits at word base(e) ^ (c BETA mod 32), base being the identity or the
+// row kernels' swizzle (both keep words below M): the (base, X, BETA) of each
+// size were found
+// by an exhaustive search under the gfx950 LDS bank model (64-bit words:
+// writes 16 lanes per pass over 32 banks, reads 32 lanes over 64 banks;
+// 32-bit words: 32 lanes over 32 banks) for the lane layout (c, t) above,
+// and give conflict-free exchanges (FP32 M = 256: 128 extra passes in 1536).
+// Dynamic LDS: M complex twiddles + TC * CS scalar words.
 // ------------------------------------------------------------------------
 template<typename T, int M> struct ColReg {
   static constexpr int V=8;
   static constexpr int TPS=M/V;
   static constexpr int TC=TPS >= 512 ? 1 : (512/TPS < 64 ? 512/TPS : 64);
-  static constexpr int CS=RPlan<M,V>::PAD+(sizeof(T) == 8 ? 4 : 8);
+  static constexpr int PAD=RPlan<M,V>::PAD;
+  static constexpr bool F64=sizeof(T) == 8;
+  // base: 0 identity, 2 swizzle (xidx)
+  static constexpr int BASE=M >= 1024 ? 2 : 0;
+  static constexpr int X=M == 16 ? 1 : 0;
+  static constexpr int BETA=M == 16 || M == 4096 ? 0 : M == 256 ? (F64 ? 17 : 2) : M == 512 ? 9 :
+    M == 1024 ? 10 : M == 2048 ? (F64 ? 28 : 20) : 1;
+  static constexpr int CS=2*PAD+X;
   static constexpr int NT=TPS*TC;
+  static __device__ __forceinline__ int at(int e)
+  {
+    if constexpr(BASE == 2) return xidx<T,M,V>(e);
+    else return e;
+  }
+  // the column's XOR of the word index (M = 16: the words stay below 16)
+  static __device__ __forceinline__ int xm(int c) { return (c*BETA) & (M < 32 ? M-1 : 31); }
 };
 
 // F[b][(r-r0) M + s][c] = sum_j f[b][j][c] w_N^{(j+k0)(q s + r)}, j < L.
@@ -264,6 +285,7 @@ colfwd_reg_kernel(DimArgs d, const cplx<T> *f, int64_t S, int64_t C,
   __syncthreads();
   const int c=threadIdx.x % TC, t=threadIdx.x/TC;
   T *xb=reinterpret_cast<T *>(tw+M)+c*CR::CS;
+  const int xm=CR::xm(c);
   const int64_t c0=xcd_tile(blockIdx.x,gridDim.x)*TC;
   const bool ok=c0+c < C;
   const cplx<T> *fb=f+(int64_t) blockIdx.y*bsi+c0+c;
@@ -292,7 +314,7 @@ colfwd_reg_kernel(DimArgs d, const cplx<T> *f, int64_t S, int64_t C,
         a=a+(r ? raw[i][k]*zN[wrapN((j0+M*i+k0)*r,N)] : raw[i][k]);
       X[k]=a;
     }
-    fftr_blk<T,M,V,1>(X,xb,tw,t);
+    fftr_blkx<T,M,V,1,CR>(X,xb,tw,t,xm);
     if(ok) {
       cplx<T> *Fr=Fb+rr*M*SF;
 #pragma unroll
@@ -318,6 +340,7 @@ colbwd_reg_kernel(DimArgs d, const cplx<T> *F, int64_t SF, int64_t bsi,
   __syncthreads();
   const int c=threadIdx.x % TC, t=threadIdx.x/TC;
   T *xb=reinterpret_cast<T *>(tw+M)+c*CR::CS;
+  const int xm=CR::xm(c);
   const int64_t c0=xcd_tile(blockIdx.x,gridDim.x)*TC;
   const bool ok=c0+c < C;
   const cplx<T> *Fb=F+(int64_t) blockIdx.y*bsi+c0+c;
@@ -334,7 +357,7 @@ colbwd_reg_kernel(DimArgs d, const cplx<T> *F, int64_t SF, int64_t bsi,
     cplx<T> X[V];
 #pragma unroll
     for(int k=0; k < V; ++k) X[k]=ok ? Fr[(int64_t) (t+TPS*k)*SF] : mk((T) 0,(T) 0);
-    fftr_blk<T,M,V,-1>(X,xb,tw,t);
+    fftr_blkx<T,M,V,-1,CR>(X,xb,tw,t,xm);
 #pragma unroll
     for(int k=0; k < V; ++k) {
       const int j0=(t+TPS*k-k0) & (M-1);
