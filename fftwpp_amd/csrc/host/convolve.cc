@@ -371,6 +371,18 @@ static Choice choose(size_t L, size_t M, Application &app, size_t C, size_t S,
       }
     }
   }
+  // Complex rows (C = 1) longer than 2048 take m = 4096: the fused register
+  // row kernel covers L <= 4096, and longer rows run on the batched register
+  // kernel with one block per residue (the LDS kernels for other m run a
+  // long row in one block, or unfused).
+  if(C == 1 && kind == 0 && L > 2048) {
+    size_t p, n, q;
+    fftBase::parameters(L,M,4096,centered,p,n,q);
+    if(fftPad::valid(4096,p,q,n,1,S)) {
+      best_m=4096;
+      best_N=4096*q;
+    }
+  }
   // Complex and centred rows (C = 1) run fused with their residues in
   // sequence.  At m = 1024 the register row kernel is faster at m = 2048 when
   // that lowers q, although the padded length may grow: measured 10% for a
@@ -685,8 +697,28 @@ void Convolution::execDevice(void *out, const void *f0, const void *f1, bool nor
   }
   if(fft->kind() == FFTWPP_AMD_KIND_HERMITIAN)
     throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"Hermitian sub-FFT size beyond the LDS limit");
-  // unfused: all residues forward, multiply, backward
   size_t E=eltsize(dtype);
+  if(fft->kind() == FFTWPP_AMD_KIND_COMPLEX && fftwpp_amd_batch_rows_tps(fft->dev,dtype) &&
+     code != FFTWPP_AMD_MULT_REAL_BINARY) {
+    // a row longer than a power-of-two m <= 4096: the batched register
+    // kernel as a batch of one item, one block per residue (D = 1), the
+    // residue sums added by its second launch
+    void *item=work.get(3,32);
+    if(!itemReady) {
+      const int64_t h[3]={0,(int64_t) n,0};
+      check(fftwpp_amd_memcpy(item,h,sizeof(h),stream),"item table");
+      check(fftwpp_amd_synchronize(stream),"item table");
+      itemReady=true;
+    }
+    void *part=work.get(2,fft->q*n*E);
+    const size_t tps=fftwpp_amd_batch_rows_tps(fft->dev,dtype);
+    check(fftwpp_amd_batch_rows(fft->dev,dtype,code,item,1,f0,f1,out,1,
+                                tps < 64 ? 64 : tps,fft->q,0,1,part,n,s,stream),
+          "batched row");
+    if(timeline) timeline->mark("batched row");
+    return;
+  }
+  // unfused: all residues forward, multiply, backward
   size_t words=fft->q*fft->m;
   void *T0=work.get(0,words*E);
   void *T1=work.get(1,words*E);

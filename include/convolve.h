@@ -331,6 +331,7 @@ public:
 protected:
   int multCode;
   DeviceWork work;
+  bool itemReady=false;  // the one-item table of the batched row path is on the device
   void run(Complex **f, size_t offset, bool normalize);
 };
 

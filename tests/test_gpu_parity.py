@@ -417,22 +417,44 @@ def test_register_fft256_paths(gpu, oracle, L, M):
     assert oracle.rel_l2(h3, want) < TOL64
 
 
-@pytest.mark.parametrize("L", [4097, 16384])
-def test_conv1d_long_rows_unfused(gpu, oracle, L):
-    """Lengths whose outputs do not fit beside the sub-FFTs in LDS take the
-    unfused residue path (forward, multiply, backward)."""
+@pytest.mark.parametrize("L,m", [(4097, None), (6000, None), (12289, None), (16384, None),
+                                 (5000, 3000), (9000, 2048)])
+def test_conv1d_long_rows(gpu, oracle, L, m):
+    """Rows longer than one power-of-two block of m <= 4096 run on the batched
+    register kernel as a batch of one item (one block per residue); a forced
+    non-power-of-two m whose outputs do not fit beside the sub-FFTs in LDS
+    takes the unfused residue path (forward, multiply, backward).  Two calls
+    on one plan, the second on other data."""
     import torch
     from fftwpp_amd.torch_conv import DeviceConvolution
     rng = np.random.default_rng(L)
-    f, g = rnd(rng, L), rnd(rng, L)
-    want = oracle.explicit_conv(f, g, 1 << (2 * L - 1).bit_length())
     for dtype, tol in ((torch.complex128, TOL64), (torch.complex64, 1e-5)):
-        plan = DeviceConvolution((L,), dtype=dtype, M=[2 * L])
-        out = torch.empty(L, dtype=dtype, device=gpu)
-        plan.convolve_out(out, torch.tensor(f, dtype=dtype, device=gpu),
-                          torch.tensor(g, dtype=dtype, device=gpu))
-        torch.cuda.synchronize()
-        assert oracle.rel_l2(out.cpu().numpy(), want) < tol
+        plan = DeviceConvolution((L,), dtype=dtype, M=[2 * L], m=None if m is None else [m])
+        if m is None:
+            assert plan.geometry()[1][0] == 4096
+        for _ in range(2):
+            f, g = rnd(rng, L), rnd(rng, L)
+            want = oracle.explicit_conv(f, g, 1 << (2 * L - 1).bit_length())
+            out = torch.empty(L, dtype=dtype, device=gpu)
+            plan.convolve_out(out, torch.tensor(f, dtype=dtype, device=gpu),
+                              torch.tensor(g, dtype=dtype, device=gpu))
+            torch.cuda.synchronize()
+            assert oracle.rel_l2(out.cpu().numpy(), want) < tol
+
+
+def test_adjoint_long_row(gpu, oracle):
+    """The adjoint (correlations) of a row longer than 4096 on the batched path."""
+    import torch
+    from fftwpp_amd.torch_conv import DeviceConvolution
+    rng = np.random.default_rng(5001)
+    L = 5001
+    a, b, w = rnd(rng, L), rnd(rng, L), rnd(rng, L)
+    plan = DeviceConvolution((L,))
+    T = lambda x: torch.tensor(x, device=gpu)
+    ga, gb = plan.adjoint(T(w), T(a), T(b))
+    torch.cuda.synchronize()
+    assert oracle.rel_l2(ga.cpu().numpy(), _correlate(w, b)) < TOL64
+    assert oracle.rel_l2(gb.cpu().numpy(), _correlate(w, a)) < TOL64
 
 
 def test_cfftw_device_entry_points(gpu, oracle):
