@@ -465,6 +465,29 @@ static int allow_lds(K kernel, size_t bytes)
   return rc;
 }
 
+// Blocks of `kernel` that stay resident on the whole device at once
+// (occupancy per CU times the CU count of the current device), at least 1:
+// the grid of a persistent kernel.
+// Cached per (kernel, device).
+template<typename K>
+static size_t resident_blocks(K kernel, int threads, size_t lds)
+{
+  static std::mutex mu;
+  static std::vector<std::pair<std::pair<const void *,int>,size_t>> known;
+  int dev=0;
+  if(hipGetDevice(&dev) != hipSuccess) return 1024;
+  const std::pair<const void *,int> key((const void *) kernel,dev);
+  std::lock_guard<std::mutex> lock(mu);
+  for(const auto &e : known) if(e.first == key) return e.second;
+  int cus=0, per=0;
+  if(hipDeviceGetAttribute(&cus,hipDeviceAttributeMultiprocessorCount,dev) != hipSuccess ||
+     hipOccupancyMaxActiveBlocksPerMultiprocessor(&per,(const void *) kernel,threads,lds) != hipSuccess)
+    return 1024;
+  const size_t n=std::max<size_t>(1,(size_t) cus*(size_t) per);
+  known.push_back({key,n});
+  return n;
+}
+
 // The register-resident m=256 kernels apply to non-centred complex
 // dimensions with p = 1.
 static bool fast256(const fftwpp_amd_dim *d, size_t s1=0, size_t s2=0)
@@ -830,22 +853,26 @@ static int conv_rows_t(const fftwpp_amd_dim *d, int mult, const void *f,
   // q = 2: both residues of a row in one wave (each row read once)
   if(fast256(d,rs,os) && d->q == 2) {
     constexpr int WPB=4;
-    dim3 grid((unsigned) ((rows+2*WPB-1)/(2*WPB)));
+    const size_t need=(rows+2*WPB-1)/(2*WPB);
     const cplx<T> *zt=(const cplx<T> *) a.zN;
     size_t lds=row256_lds<T>(4*WPB,2);
     int rc;
+    // persistent grid: at most the blocks that are resident at once
     if(mult == FFTWPP_AMD_MULT_CORRELATION) {
       if((rc=allow_lds(rowconv256p_kernel<T,3,WPB>,lds))) return rc;
+      dim3 grid((unsigned) std::min(need,resident_blocks(rowconv256p_kernel<T,3,WPB>,64*WPB,lds)));
       hipLaunchKernelGGL((rowconv256p_kernel<T,3,WPB>),grid,dim3(64*WPB),lds,st,
                          (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
                          (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt);
     } else if(mult == FFTWPP_AMD_MULT_NONE) {
       if((rc=allow_lds(rowconv256p_kernel<T,0,WPB>,lds))) return rc;
+      dim3 grid((unsigned) std::min(need,resident_blocks(rowconv256p_kernel<T,0,WPB>,64*WPB,lds)));
       hipLaunchKernelGGL((rowconv256p_kernel<T,0,WPB>),grid,dim3(64*WPB),lds,st,
                          (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
                          (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt);
     } else {
       if((rc=allow_lds(rowconv256p_kernel<T,1,WPB>,lds))) return rc;
+      dim3 grid((unsigned) std::min(need,resident_blocks(rowconv256p_kernel<T,1,WPB>,64*WPB,lds)));
       hipLaunchKernelGGL((rowconv256p_kernel<T,1,WPB>),grid,dim3(64*WPB),lds,st,
                          (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
                          (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt);

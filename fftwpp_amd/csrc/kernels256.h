@@ -290,8 +290,10 @@ __device__ __forceinline__ double pair16_sum(double v)
 // residue groups of a row load the same elements in the same instruction, so
 // f and g are read from memory once; each group transforms its residue and
 // the contributions are summed across the group pair in registers.  Group r
-// then stores the outputs j + 16 k with k in [8 r, 8 r + 8).  Dynamic LDS:
-// row256_lds(4 WPB, 2).
+// then stores the outputs j + 16 k with k in [8 r, 8 r + 8).  The grid is
+// persistent: each block fills its twiddle tables once and its waves, which
+// exchange only within themselves, loop over row pairs independently.
+// Dynamic LDS: row256_lds(4 WPB, 2).
 template<typename T, int MULT, int WPB>
 __global__ void __launch_bounds__(64*WPB,2)
 rowconv256p_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows,
@@ -308,41 +310,45 @@ rowconv256p_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows,
   const int j=threadIdx.x & 15;
   const int grp=threadIdx.x >> 4;          // 4 groups per wave
   const int r=grp & 1;
-  const int64_t row=(int64_t) blockIdx.x*(2*WPB)+(grp >> 1);
-  const bool ok=row < rows;
+  const int wave=threadIdx.x >> 6;
   const cplx<T> *rt=rtab+256*r;
-  const cplx<T> *fr=f+(ok ? row : 0)*rs;
-  const cplx<T> *gr=g+(ok ? row : 0)*rs;
   constexpr int RS=17;                     // row stride of the exchange areas
   T *xrow=lds+grp*16*RS;
-  cplx<T> a[16], b[16];
+  const int64_t pairs=(rows+1)/2;
+  for(int64_t pr=(int64_t) blockIdx.x*WPB+wave; pr < pairs; pr += (int64_t) gridDim.x*WPB) {
+    const int64_t row=2*pr+((grp & 3) >> 1);
+    const bool ok=row < rows;
+    const cplx<T> *fr=f+(ok ? row : 0)*rs;
+    const cplx<T> *gr=g+(ok ? row : 0)*rs;
+    cplx<T> a[16], b[16];
 #pragma unroll
-  for(int k=0; k < 16; ++k) {
-    bool in=ok && j+16*k < L;
-    cplx<T> x=in ? (fr+16*k)[j] : zero<T>();
-    cplx<T> y=in ? (gr+16*k)[j] : zero<T>();
-    cplx<T> w=(rt+16*k)[j];                 // w_512^{r (j + 16 k)}
-    a[k]=x*w;
-    b[k]=y*w;
-  }
-  fft256s<T,1,1,RS>(a,xrow,j,0,tw16);
-  fft256s<T,1,1,RS>(b,xrow,j,0,tw16);
+    for(int k=0; k < 16; ++k) {
+      bool in=ok && j+16*k < L;
+      cplx<T> x=in ? (fr+16*k)[j] : zero<T>();
+      cplx<T> y=in ? (gr+16*k)[j] : zero<T>();
+      cplx<T> w=(rt+16*k)[j];                 // w_512^{r (j + 16 k)}
+      a[k]=x*w;
+      b[k]=y*w;
+    }
+    fft256s<T,1,1,RS>(a,xrow,j,0,tw16);
+    fft256s<T,1,1,RS>(b,xrow,j,0,tw16);
 #pragma unroll
-  for(int k=0; k < 16; ++k) {
-    if(MULT == 1) a[k]=a[k]*b[k];
-    else if(MULT == 3) a[k]=mulc(a[k],b[k]);
-  }
-  fft256s<T,-1,1,RS>(a,xrow,j,0,tw16);
+    for(int k=0; k < 16; ++k) {
+      if(MULT == 1) a[k]=a[k]*b[k];
+      else if(MULT == 3) a[k]=mulc(a[k],b[k]);
+    }
+    fft256s<T,-1,1,RS>(a,xrow,j,0,tw16);
 #pragma unroll
-  for(int k=0; k < 16; ++k) {
-    cplx<T> y=mulc(a[k],(rt+16*k)[j]);
-    a[k]=mk(pair16_sum(y.x),pair16_sum(y.y));
-  }
-  if(ok) {
-    cplx<T> *orow=out+row*os;
+    for(int k=0; k < 16; ++k) {
+      cplx<T> y=mulc(a[k],(rt+16*k)[j]);
+      a[k]=mk(pair16_sum(y.x),pair16_sum(y.y));
+    }
+    if(ok) {
+      cplx<T> *orow=out+row*os;
 #pragma unroll
-    for(int k=0; k < 16; ++k)
-      if((k >> 3) == r && j+16*k < L) (orow+16*k)[j]=scale*a[k];
+      for(int k=0; k < 16; ++k)
+        if((k >> 3) == r && j+16*k < L) (orow+16*k)[j]=scale*a[k];
+    }
   }
 }
 
