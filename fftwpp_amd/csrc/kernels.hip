@@ -806,7 +806,8 @@ static int rowreg_launch_rp(const DimArgs &a, const void *f, const void *g,
   if(rc) return rc;
   const size_t rpb=RP > 1 ? 1 : slots;
   size_t nb=(rows+rpb-1)/rpb;
-  if(nb > 65536) nb=65536;
+  // persistent grid: the twiddle table is filled once per resident block
+  nb=std::min(nb,resident_blocks(rowreg_kernel<T,M,HERM,RP>,(int) nt,lds));
   hipLaunchKernelGGL((rowreg_kernel<T,M,HERM,RP>),dim3((unsigned) nb),dim3((unsigned) nt),lds,st,
                      a,(const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
                      (cplx<T> *) out,(int64_t) os,(T) scale,mult);
