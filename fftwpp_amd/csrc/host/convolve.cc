@@ -353,10 +353,12 @@ static Choice choose(size_t L, size_t M, Application &app, size_t C, size_t S,
   // The register-resident kernels take power-of-two m up to 4096 (rows when
   // one block of m covers the row: L <= m, or H <= m for Hermitian data);
   // they run several times faster than the LDS engine, so when the smallest
-  // padded length needs another m, a covering power of two is preferred while
-  // its padded length stays within twice the smallest one.
-  if(best_m != 0 && (best_m & (best_m-1)) != 0) {
-    size_t cover=kind == 2 ? (L+1)/2 : L;
+  // padded length needs another m, or a power of two that splits the data
+  // into p > 1 blocks (a length of 67 padded to 136 with m = 4, p = 17), a
+  // covering power of two is preferred while its padded length stays within
+  // twice the smallest one.
+  const size_t cover=kind == 2 ? (L+1)/2 : L;
+  if(best_m != 0 && ((best_m & (best_m-1)) != 0 || best_m < cover)) {
     size_t m=utils::ceilpow2(std::max<size_t>(cover,16));
     if(m <= std::min<size_t>(lim,4096)) {
       size_t p, n, q;
