@@ -360,7 +360,7 @@ static Choice choose(size_t L, size_t M, Application &app, size_t C, size_t S,
   const size_t cover=kind == 2 ? (L+1)/2 : L;
   if(best_m != 0 && ((best_m & (best_m-1)) != 0 || best_m < cover)) {
     size_t m=utils::ceilpow2(std::max<size_t>(cover,16));
-    if(m <= std::min<size_t>(lim,4096)) {
+    if(m <= 4096) {
       size_t p, n, q;
       fftBase::parameters(L,M,m,centered,p,n,q);
       bool ok;

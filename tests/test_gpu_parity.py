@@ -442,6 +442,26 @@ def test_conv1d_long_rows(gpu, oracle, L, m):
             assert oracle.rel_l2(out.cpu().numpy(), want) < tol
 
 
+@pytest.mark.parametrize("kind,L", [("centered", 5001), ("centered", 9000), ("hermitian", 6001),
+                                    ("hermitian", 8191)])
+def test_long_centred_and_hermitian_rows(gpu, oracle, kind, L):
+    """Default plans of long centred rows (unfused residue path) and Hermitian
+    rows up to H = 4096 (register row kernel at m = 4096)."""
+    import torch
+    rng = np.random.default_rng(L + 1)
+    if kind == "centered":
+        f, g = rnd(rng, L), rnd(rng, L)
+        want = oracle.explicit_conv(f, g, 2 * L, centered=True)
+        h, _ = run(gpu, "centered", f, g)
+    else:
+        H = (L + 1) // 2
+        f, g = rnd(rng, H), rnd(rng, H)
+        f[0], g[0] = f[0].real, g[0].real
+        want = oracle.explicit_hconv(f, g, oracle.default_M(L, True))
+        h, plan = run(gpu, "hermitian", f, g, lengths=[L])
+    assert oracle.rel_l2(h, want) < TOL64
+
+
 def test_adjoint_long_row(gpu, oracle):
     """The adjoint (correlations) of a row longer than 4096 on the batched path."""
     import torch
