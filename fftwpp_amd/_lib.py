@@ -25,6 +25,8 @@ SIGNATURES = {
     "fftwpp_amd_forward_many": (I, [P, I, P, Z, Z, Z, Z, P, Z, Z, Z, Z, P]),
     "fftwpp_amd_backward_many": (I, [P, I, P, Z, Z, P, Z, Z, Z, Z, Z, Z, I, D, P]),
     "fftwpp_amd_conv_rows": (I, [P, I, I, P, P, Z, Z, P, Z, D, P]),
+    "fftwpp_amd_hconv_rows_long_part": (Z, [P, Z]),
+    "fftwpp_amd_hconv_rows_long": (I, [P, I, P, P, Z, Z, P, Z, P, D, P]),
     "fftwpp_amd_batch_rows_lds": (Z, [P, I, Z, Z]),
     "fftwpp_amd_batch_rows_tps": (Z, [P, I]),
     "fftwpp_amd_batch_rows": (I, [P, I, I, P, Z, P, P, P, Z, Z, Z, Z, Z, P, Z, D, P]),

@@ -9,8 +9,10 @@
 #include <iostream>
 #include <sstream>
 
-// kernel-layer query (kernels.hip): does the fused row kernel take a row?
+// kernel-layer queries (kernels.hip): does a fused row kernel take a row;
+// which engine takes the rows (stage 1) of a dimension (2: register FFT).
 extern "C" int fftwpp_amd_internal_rows_fit(const fftwpp_amd_dim *d, int dtype);
+extern "C" int fftwpp_amd_internal_engine(const fftwpp_amd_dim *d, int stage);
 
 namespace fftwpp {
 
@@ -402,6 +404,16 @@ static Choice choose(size_t L, size_t M, Application &app, size_t C, size_t S,
       best_N=2048*q2;
     }
   }
+  // Hermitian rows (C = 1) longer than 2 x 4096 run on the long Hermitian row
+  // kernel (residue pairs in parallel), which takes power-of-two m and any p:
+  // m = 4096.  (The reference's D = 2 code paths need even p,
+  // convolve.h:721-723; the pairs (r, r + 1) of this kernel do not.)
+  if(C == 1 && kind == 2 && (L+1)/2 > 4096) {
+    size_t p, n, q;
+    fftBase::parameters(L,M,4096,centered,p,n,q);
+    best_m=4096;
+    best_N=4096*q;
+  }
   if(best_m == 0) {
     // explicit padding as the last resort (convolve.cc:431-434)
     c.m=nextfftsize(M);
@@ -681,6 +693,39 @@ void Convolution::convolveDevice(Complex *f0, const Complex *f1,
   execDevice(f0,f0,f1,normalize,FFTWPP_AMD_F64,stream);
 }
 
+// Whether the fused row stage takes the rows of fft: a fused row kernel, or
+// the long Hermitian row path (Hermitian rows longer than one block of a
+// power-of-two m <= 4096).
+static bool rowsSupported(fftBase *fft, int dtype)
+{
+  return fftwpp_amd_internal_rows_fit(fft->dev,dtype) ||
+    fftwpp_amd_hconv_rows_long_part(fft->dev,1) != 0;
+}
+
+// The fused row stage of dimension fft: the register row kernels when they
+// cover the row, else the long Hermitian path for Hermitian rows of a
+// power-of-two m (residue passes in parallel, partial sums in work slot
+// `slot`), else the LDS row kernel.
+static void fusedRows(fftBase *fft, DeviceWork &work, size_t slot, int dtype,
+                      int code, const void *f, const void *g, size_t rows,
+                      size_t rs, void *out, size_t os, double s, void *stream,
+                      const char *what)
+{
+  const bool reg=fftwpp_amd_internal_engine(fft->dev,1) > 0;
+  const bool longh=fftwpp_amd_hconv_rows_long_part(fft->dev,1) != 0;
+  if(reg || (!longh && fftwpp_amd_internal_rows_fit(fft->dev,dtype))) {
+    check(fftwpp_amd_conv_rows(fft->dev,dtype,code,f,g,rows,rs,out,os,s,stream),what);
+    return;
+  }
+  const size_t words=fftwpp_amd_hconv_rows_long_part(fft->dev,rows);
+  if(words == 0)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner sub-FFT size beyond the fused row kernels");
+  if(code != FFTWPP_AMD_MULT_REAL_BINARY)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"Hermitian rows take realMultBinary");
+  void *part=work.get(slot,words*(dtype == FFTWPP_AMD_F32 ? 8 : 16));
+  check(fftwpp_amd_hconv_rows_long(fft->dev,dtype,f,g,rows,rs,out,os,part,s,stream),what);
+}
+
 void Convolution::execDevice(void *out, const void *f0, const void *f1, bool normalize,
                              int dtype, void *stream, int mult)
 {
@@ -691,14 +736,17 @@ void Convolution::execDevice(void *out, const void *f0, const void *f1, bool nor
   double s=normalize ? scale : 1.0;
   size_t n=fft->inputLength();
   if(timeline) timeline->begin(stream);
+  if(fft->kind() == FFTWPP_AMD_KIND_HERMITIAN) {
+    fusedRows(fft,work,4,dtype,code,f0,f1,1,n,out,n,s,stream,"Hermitian rows");
+    if(timeline) timeline->mark("rows");
+    return;
+  }
   if(fftwpp_amd_internal_rows_fit(fft->dev,dtype)) {
     check(fftwpp_amd_conv_rows(fft->dev,dtype,code,f0,f1,1,n,out,n,s,
                                stream),"conv_rows");
     if(timeline) timeline->mark("rows");
     return;
   }
-  if(fft->kind() == FFTWPP_AMD_KIND_HERMITIAN)
-    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"Hermitian sub-FFT size beyond the LDS limit");
   size_t E=eltsize(dtype);
   if(fft->kind() == FFTWPP_AMD_KIND_COMPLEX && fftwpp_amd_batch_rows_tps(fft->dev,dtype) &&
      code != FFTWPP_AMD_MULT_REAL_BINARY) {
@@ -832,7 +880,7 @@ void Convolution2::execDevice(void *out, const void *f0, const void *f1, bool no
 {
   int code=mult >= 0 ? mult : multCode;
   requireBinary(A,B,code);
-  if(!fftwpp_amd_internal_rows_fit(ffty->dev,dtype))
+  if(!rowsSupported(ffty,dtype))
     throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner sub-FFT size beyond the fused row kernels");
   double s=normalize ? scale : 1.0;
   size_t E=eltsize(dtype);
@@ -845,7 +893,7 @@ void Convolution2::execDevice(void *out, const void *f0, const void *f1, bool no
   if(timeline) timeline->mark("x forward f");
   check(fftwpp_amd_forward_many(fftx->dev,dtype,f1,Sx,ly,1,0,F1,ly,0,0,qx,stream),"x forward");
   if(timeline) timeline->mark("x forward g");
-  check(fftwpp_amd_conv_rows(ffty->dev,dtype,code,F0,F1,rows,ly,F0,ly,1.0,stream),"y rows");
+  fusedRows(ffty,work,2,dtype,code,F0,F1,rows,ly,F0,ly,1.0,stream,"y rows");
   if(timeline) timeline->mark("y rows");
   check(fftwpp_amd_backward_many(fftx->dev,dtype,F0,ly,0,out,Sx,ly,1,0,0,qx,0,s,stream),"x backward");
   if(timeline) timeline->mark("x backward");
@@ -895,7 +943,7 @@ void Convolution3::execDevice(void *out, const void *f0, const void *f1, bool no
 {
   int code=mult >= 0 ? mult : multCode;
   requireBinary(A,B,code);
-  if(!fftwpp_amd_internal_rows_fit(fftz->dev,dtype))
+  if(!rowsSupported(fftz,dtype))
     throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner sub-FFT size beyond the fused row kernels");
   double s=normalize ? scale : 1.0;
   size_t E=eltsize(dtype);
@@ -921,7 +969,7 @@ void Convolution3::execDevice(void *out, const void *f0, const void *f1, bool no
     check(fftwpp_amd_forward_many(ffty->dev,dtype,X[a],lz,lz,planes,plane,Y[a],lz,yplane,0,qy,stream),"y forward");
     if(timeline) timeline->mark(a == 0 ? "y forward f" : "y forward g");
   }
-  check(fftwpp_amd_conv_rows(fftz->dev,dtype,code,Y0,Y1,planes*qy*my,lz,Y0,lz,1.0,stream),"z rows");
+  fusedRows(fftz,work,4,dtype,code,Y0,Y1,planes*qy*my,lz,Y0,lz,1.0,stream,"z rows");
   if(timeline) timeline->mark("z rows");
   check(fftwpp_amd_backward_many(ffty->dev,dtype,Y0,lz,yplane,X0,lz,lz,planes,plane,0,qy,0,1.0,stream),"y backward");
   if(timeline) timeline->mark("y backward");

@@ -946,6 +946,49 @@ static int conv_rows_t(const fftwpp_amd_dim *d, int mult, const void *f,
   return launch_check("conv_rows_kernel");
 }
 
+template<typename T, int M>
+static int hrows_long_launch(const fftwpp_amd_dim *d, const void *f, const void *g,
+                             size_t rows, size_t rs, void *out, size_t os,
+                             void *part, double scale, hipStream_t st)
+{
+  const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
+  constexpr int TPS=M/RowV::V;
+  const size_t nt=TPS > 64 ? TPS : 256;
+  const size_t lds=M*sizeof(cplx<T>)+(nt/TPS)*2*RPlan<M,RowV::V>::PAD*sizeof(T);
+  int rc=allow_lds(hrows_long_kernel<T,M>,lds);
+  if(rc) return rc;
+  const size_t passes=(d->q+1)/2;
+  const size_t units=rows*passes;
+  const size_t nb=std::min((units+nt/TPS-1)/(nt/TPS),(size_t) 65536);
+  hipLaunchKernelGGL((hrows_long_kernel<T,M>),dim3((unsigned) nb),dim3((unsigned) nt),lds,st,
+                     a,(const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
+                     (cplx<T> *) part);
+  if((rc=launch_check("hrows_long_kernel"))) return rc;
+  hipLaunchKernelGGL(hrows_reduce_kernel<T>,dim3((unsigned) std::min(rows,(size_t) 65536)),
+                     dim3(256),0,st,(const cplx<T> *) part,(int64_t) rows,(int) a.H,
+                     (int) passes,(cplx<T> *) out,(int64_t) os,(T) scale);
+  return launch_check("hrows_reduce_kernel");
+}
+
+template<typename T>
+static int hrows_long_t(const fftwpp_amd_dim *d, const void *f, const void *g,
+                        size_t rows, size_t rs, void *out, size_t os, void *part,
+                        double scale, hipStream_t st)
+{
+  switch(d->m) {
+  case 16: return hrows_long_launch<T,16>(d,f,g,rows,rs,out,os,part,scale,st);
+  case 32: return hrows_long_launch<T,32>(d,f,g,rows,rs,out,os,part,scale,st);
+  case 64: return hrows_long_launch<T,64>(d,f,g,rows,rs,out,os,part,scale,st);
+  case 128: return hrows_long_launch<T,128>(d,f,g,rows,rs,out,os,part,scale,st);
+  case 256: return hrows_long_launch<T,256>(d,f,g,rows,rs,out,os,part,scale,st);
+  case 512: return hrows_long_launch<T,512>(d,f,g,rows,rs,out,os,part,scale,st);
+  case 1024: return hrows_long_launch<T,1024>(d,f,g,rows,rs,out,os,part,scale,st);
+  case 2048: return hrows_long_launch<T,2048>(d,f,g,rows,rs,out,os,part,scale,st);
+  case 4096: return hrows_long_launch<T,4096>(d,f,g,rows,rs,out,os,part,scale,st);
+  default: return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"long Hermitian rows need a power-of-two m in [16, 4096]");
+  }
+}
+
 extern "C" {
 
 const char *fftwpp_amd_last_error(void) { return last_error; }
@@ -1092,6 +1135,29 @@ int fftwpp_amd_conv_rows(const fftwpp_amd_dim *d, int dtype, int mult,
   return dtype == FFTWPP_AMD_F32 ?
     conv_rows_t<float>(d,mult,f,g,rows,rowstride,out,outstride,scale,st) :
     conv_rows_t<double>(d,mult,f,g,rows,rowstride,out,outstride,scale,st);
+}
+
+size_t fftwpp_amd_hconv_rows_long_part(const fftwpp_amd_dim *d, size_t rows)
+{
+  if(!d || d->kind != FFTWPP_AMD_KIND_HERMITIAN) return 0;
+  const size_t m=d->m;
+  if(m < 16 || m > 4096 || (m & (m-1)) || m*d->q >= ((size_t) 1 << 30)) return 0;
+  return (d->q+1)/2*rows*((d->L+1)/2);
+}
+
+int fftwpp_amd_hconv_rows_long(const fftwpp_amd_dim *d, int dtype, const void *f,
+                               const void *g, size_t rows, size_t rowstride,
+                               void *out, size_t outstride, void *part,
+                               double scale, void *stream)
+{
+  if(!d || !f || !g || !out || !part) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  if(rows == 0) return FFTWPP_AMD_OK;
+  if(!fftwpp_amd_hconv_rows_long_part(d,1))
+    return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"long Hermitian rows need a Hermitian dimension with power-of-two m in [16, 4096]");
+  hipStream_t st=(hipStream_t) stream;
+  return dtype == FFTWPP_AMD_F32 ?
+    hrows_long_t<float>(d,f,g,rows,rowstride,out,outstride,part,scale,st) :
+    hrows_long_t<double>(d,f,g,rows,rowstride,out,outstride,part,scale,st);
 }
 
 size_t fftwpp_amd_batch_rows_lds(const fftwpp_amd_dim *d, int dtype,

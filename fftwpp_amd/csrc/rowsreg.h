@@ -229,6 +229,130 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
 }
 
 
+// Hermitian rows longer than one block (H > M): the residue-pair passes of a
+// row (fftPadHermitian's D = 2 pairs, convolve.cc:4517-5012) run as
+// independent work units (row, pass), each on one slot of TPS threads.  A
+// unit folds every block b of the row (wavenumbers k = sigma + b M, |k| < H,
+// f(-k) = conj f(k)) into the packed pair W1 + i W2, transforms f and g,
+// multiplies as realMultBinary, inverts, unpacks with the mirror element
+// Z(M - sigma) and writes its contribution to every stored mode k < H of the
+// row into its slice of `part` (slice p = rows x H words); hrows_reduce_kernel
+// adds the slices in pass order (deterministic).  Twiddles
+// w_N^{(sigma + b M) r} = w_N^{sigma r} w_q^{b r} with w_q^{b r} = zN[(b r mod q) M].
+// Dynamic LDS: M complex twiddles + slots * 2 PAD scalar words.
+template<typename T, int M>
+__global__ void __launch_bounds__(512)
+hrows_long_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
+                  int64_t rs, cplx<T> *part)
+{
+  constexpr int V=RowV::V;
+  typedef RPlan<M,V> P;
+  constexpr int TPS=P::TPS;
+  extern __shared__ char smem_raw[];
+  cplx<T> *tw=reinterpret_cast<cplx<T> *>(smem_raw);
+  const cplx<T> *zN=(const cplx<T> *) d.zN;
+  fill_twr<T,M,V>(tw,zN,(int) d.q,threadIdx.x,blockDim.x);
+  __syncthreads();
+  const int slots=blockDim.x/TPS;
+  const int slot=threadIdx.x/TPS, t=threadIdx.x-slot*TPS;
+  T *xb=reinterpret_cast<T *>(tw+M)+slot*2*P::PAD;
+  const int q=(int) d.q, H=(int) d.H;
+  const int passes=(q+1)/2;
+  const int nb=(H-1)/M+1;                 // blocks b in [0, nb) hold the modes k < H
+  const int64_t units=rows*passes;
+  const cplx<T> zero=mk((T) 0,(T) 0);
+  for(int64_t ub=(int64_t) blockIdx.x*slots; ub < units; ub += (int64_t) gridDim.x*slots) {
+    const int64_t unit=ub+slot;
+    if(unit >= units) break;              // only when slots > 1 (TPS <= 64)
+    const int64_t row=unit/passes;
+    const int pass=(int) (unit-row*passes);
+    const int r1=2*pass, r2=r1+1 < q ? r1+1 : -1;
+    const cplx<T> *fr=f+row*rs, *gr=g+row*rs;
+    cplx<T> *pr=part+((int64_t) pass*rows+row)*H;
+    cplx<T> X[V], Y[V];
+#pragma unroll
+    for(int k=0; k < V; ++k) X[k]=Y[k]=zero;
+    // b r mod q from b = -nb on, advanced by r per block
+    int e1=(int) ((q-((int64_t) nb*r1) % q) % q);
+    int e2=r2 >= 0 ? (int) ((q-((int64_t) nb*r2) % q) % q) : 0;
+    for(int b=-nb; b < nb; ++b) {
+      const cplx<T> c1=zN[e1*M], c2=r2 >= 0 ? zN[e2*M] : zero;
+#pragma unroll
+      for(int k=0; k < V; ++k) {
+        const int sigma=t+TPS*k;
+        const int kk=sigma+b*M;
+        if(kk > -H && kk < H) {
+          cplx<T> w=zN[sigma*r1]*c1;
+          if(r2 >= 0) w=w+muli(zN[sigma*r2]*c2,1);
+          const cplx<T> fv=kk >= 0 ? fr[kk] : conjg(fr[-kk]);
+          const cplx<T> gv=kk >= 0 ? gr[kk] : conjg(gr[-kk]);
+          X[k]=X[k]+w*fv;
+          Y[k]=Y[k]+w*gv;
+        }
+      }
+      e1 += r1;
+      if(e1 >= q) e1 -= q;
+      if(r2 >= 0) {
+        e2 += r2;
+        if(e2 >= q) e2 -= q;
+      }
+    }
+    fftr<T,M,V,1,true>(X,xb,tw,t);
+    fftr<T,M,V,1,true>(Y,xb,tw,t);
+#pragma unroll
+    for(int k=0; k < V; ++k) X[k]=mk(X[k].x*Y[k].x,X[k].y*Y[k].y);
+    fftr<T,M,V,-1,true>(X,xb,tw,t);
+    // mirror elements Z(M - sigma) through LDS
+#pragma unroll
+    for(int k=0; k < V; ++k) {
+      const int e=xidx<T,M,V>(t+TPS*k);
+      xb[e]=X[k].x;
+      xb[P::PAD+e]=X[k].y;
+    }
+    rsync<TPS>();
+#pragma unroll
+    for(int k=0; k < V; ++k) {
+      const int e=xidx<T,M,V>((M-(t+TPS*k)) & (M-1));
+      const cplx<T> z=X[k], zc=conjg(mk(xb[e],xb[P::PAD+e]));
+      // this thread's unpacked pair, with the conjugate twiddles of sigma
+      Y[k]=mulc((T) 0.5*(z+zc),zN[(t+TPS*k)*r1]);
+      X[k]=r2 >= 0 ? mulc(muli((T) 0.5*(z-zc),-1),zN[(t+TPS*k)*r2]) : zero;
+    }
+    rsync<TPS>();
+    e1=0;
+    e2=0;
+    for(int b=0; b < nb; ++b) {
+      const cplx<T> c1=zN[e1*M], c2=r2 >= 0 ? zN[e2*M] : zero;
+#pragma unroll
+      for(int k=0; k < V; ++k) {
+        const int kk=t+TPS*k+b*M;
+        if(kk < H) pr[kk]=mulc(Y[k],c1)+(r2 >= 0 ? mulc(X[k],c2) : zero);
+      }
+      e1 += r1;
+      if(e1 >= q) e1 -= q;
+      if(r2 >= 0) {
+        e2 += r2;
+        if(e2 >= q) e2 -= q;
+      }
+    }
+  }
+}
+
+// out[row][k] = scale * sum over passes of part[pass][row][k], k < H, in
+// pass order; one block per row (grid-stride).
+template<typename T>
+__global__ void __launch_bounds__(256)
+hrows_reduce_kernel(const cplx<T> *part, int64_t rows, int H, int passes,
+                    cplx<T> *out, int64_t os, T scale)
+{
+  for(int64_t row=blockIdx.x; row < rows; row += gridDim.x)
+    for(int k=threadIdx.x; k < H; k += blockDim.x) {
+      cplx<T> sum=part[row*H+k];
+      for(int p=1; p < passes; ++p) sum=sum+part[((int64_t) p*rows+row)*H+k];
+      out[row*os+k]=scale*sum;
+    }
+}
+
 // ------------------------------------------------------------------------
 // Column residue transforms on the register FFT (fftPad forward/backward of
 // strided columns, convolve.cc:771-1965) for power-of-two M and at most P2 = 2

@@ -102,6 +102,20 @@ int fftwpp_amd_conv_rows(const fftwpp_amd_dim *d, int dtype, int mult,
                          size_t rowstride, void *out, size_t outstride,
                          double scale, void *stream);
 
+/* Hermitian rows longer than one block of m (H = ceil(L/2) > m): the
+ * fused row convolution of fftwpp_amd_conv_rows with realMultBinary, for
+ * power-of-two m in [16, 4096].  Each residue pair (fftPadHermitian's D = 2
+ * passes, convolve.cc:4517-5012) of each row runs as an independent work
+ * unit writing its contribution into `part` (fftwpp_amd_hconv_rows_long_part
+ * words, 0 when d is not such a dimension); a second launch adds the passes
+ * in order into out (out may equal f).  Replaces the residue loop of
+ * Convolution::convolveRaw (convolve.cc:7513-7575) for these rows. */
+size_t fftwpp_amd_hconv_rows_long_part(const fftwpp_amd_dim *d, size_t rows);
+int fftwpp_amd_hconv_rows_long(const fftwpp_amd_dim *d, int dtype, const void *f,
+                               const void *g, size_t rows, size_t rowstride,
+                               void *out, size_t outstride, void *part,
+                               double scale, void *stream);
+
 /* Fused convolutions of `count` independent items that share the geometry
  * (m, q) of d.  items is a device array of {int64 off, int64 L, int64 poff}
  * triples: item i convolves f[off..off+L) with g[off..off+L) into

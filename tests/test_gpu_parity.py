@@ -462,6 +462,33 @@ def test_long_centred_and_hermitian_rows(gpu, oracle, kind, L):
     assert oracle.rel_l2(h, want) < TOL64
 
 
+@pytest.mark.parametrize("lengths,m", [([8193], None), ([10001], None), ([20001], None),
+                                       ([3001], [256]), ([15, 10001], None),
+                                       ([5, 7, 9001], None)])
+def test_long_hermitian_rows(gpu, oracle, lengths, m):
+    """Hermitian rows longer than one block of a power-of-two m: the residue
+    pairs of each row run as parallel work units whose partial sums a second
+    launch adds (1D, and the inner dimension of 2D/3D), FP64 and FP32."""
+    import torch
+    rng = np.random.default_rng(sum(lengths))
+    if len(lengths) == 1:
+        H = (lengths[0] + 1) // 2
+        f, g = rnd(rng, H), rnd(rng, H)
+        f[0], g[0] = f[0].real, g[0].real
+    elif len(lengths) == 2:
+        f, g = oracle.inputs_h2(lengths[0], (lengths[1] + 1) // 2, seed=lengths[1])
+    else:
+        Hz = (lengths[2] + 1) // 2
+        f = oracle.ramph3(lengths[0], lengths[1], Hz, 0).astype(complex) / Hz
+        g = oracle.ramph3(lengths[0], lengths[1], Hz, 1).astype(complex) / Hz
+    want = oracle.explicit_hconv(f, g, tuple(oracle.default_M(L, True) for L in lengths))
+    for dtype, tol in ((torch.complex128, TOL64), (torch.complex64, TOL32)):
+        h, plan = run(gpu, "hermitian", f, g, dtype=dtype, m=m, lengths=lengths)
+        mm = plan.geometry()[1][-1]
+        assert mm & (mm - 1) == 0 and (lengths[-1] + 1) // 2 > mm
+        assert oracle.rel_l2(h, want) < tol, dtype
+
+
 def test_adjoint_long_row(gpu, oracle):
     """The adjoint (correlations) of a row longer than 4096 on the batched path."""
     import torch
