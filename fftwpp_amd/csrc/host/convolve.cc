@@ -3,6 +3,7 @@
 // C ABI of include/fftwpp_amd.h.
 #include "convolve.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -620,10 +621,12 @@ void *DeviceWork::get(size_t slot, size_t bytes)
     bufs.resize(slot+1,NULL);
     sizes.resize(slot+1,0);
   }
+  if(slot >= tags.size()) tags.resize(slot+1,0);
   if(sizes[slot] < bytes) {
     fftwpp_amd_free(bufs[slot]);
     bufs[slot]=NULL;
     sizes[slot]=0;
+    tags[slot]=0;
     check(fftwpp_amd_malloc(&bufs[slot],bytes),"device work buffer");
     sizes[slot]=bytes;
   }
@@ -693,13 +696,14 @@ void Convolution::convolveDevice(Complex *f0, const Complex *f1,
   execDevice(f0,f0,f1,normalize,FFTWPP_AMD_F64,stream);
 }
 
-// Whether the fused row stage takes the rows of fft: a fused row kernel, or
-// the long Hermitian row path (Hermitian rows longer than one block of a
-// power-of-two m <= 4096).
+// Whether the fused row stage takes the rows of fft: a fused row kernel, the
+// long Hermitian row path or the batched register kernel (Hermitian or
+// complex rows longer than one block of a power-of-two m <= 4096).
 static bool rowsSupported(fftBase *fft, int dtype)
 {
   return fftwpp_amd_internal_rows_fit(fft->dev,dtype) ||
-    fftwpp_amd_hconv_rows_long_part(fft->dev,1) != 0;
+    fftwpp_amd_hconv_rows_long_part(fft->dev,1) != 0 ||
+    (fft->kind() == FFTWPP_AMD_KIND_COMPLEX && fftwpp_amd_batch_rows_tps(fft->dev,dtype) != 0);
 }
 
 // The fused row stage of dimension fft: the register row kernels when they
@@ -713,8 +717,40 @@ static void fusedRows(fftBase *fft, DeviceWork &work, size_t slot, int dtype,
 {
   const bool reg=fftwpp_amd_internal_engine(fft->dev,1) > 0;
   const bool longh=fftwpp_amd_hconv_rows_long_part(fft->dev,1) != 0;
-  if(reg || (!longh && fftwpp_amd_internal_rows_fit(fft->dev,dtype))) {
+  const size_t tps=fft->kind() == FFTWPP_AMD_KIND_COMPLEX && rs == os &&
+    code != FFTWPP_AMD_MULT_REAL_BINARY ? fftwpp_amd_batch_rows_tps(fft->dev,dtype) : 0;
+  if(reg || (!longh && !tps && fftwpp_amd_internal_rows_fit(fft->dev,dtype))) {
     check(fftwpp_amd_conv_rows(fft->dev,dtype,code,f,g,rows,rs,out,os,s,stream),what);
+    return;
+  }
+  if(tps) {
+    // complex rows longer than one block of a power-of-two m: the batched
+    // register kernel with the rows as items (row i at i rs).  The residues
+    // always go to partial sums that its second launch adds into out (at
+    // least two groups when q > 1), since out may alias f or g and a unit
+    // that wrote its outputs would re-read them in its next residue's fold:
+    // one residue per unit when rows are few, two groups otherwise.
+    const size_t L=fft->L, q=fft->q;
+    const size_t D=rows >= 512 ? (q+1)/2 : 1;
+    void *items=work.get(slot+1,rows*3*sizeof(int64_t));
+    const size_t tag=rows*1000003u+rs*7919u+L;
+    if(work.tags[slot+1] != tag) {
+      std::vector<int64_t> h(3*rows);
+      for(size_t i=0; i < rows; ++i) {
+        h[3*i]=(int64_t) (i*rs);
+        h[3*i+1]=(int64_t) L;
+        h[3*i+2]=(int64_t) (i*L);
+      }
+      check(fftwpp_amd_memcpy(items,h.data(),h.size()*sizeof(int64_t),stream),"item table");
+      check(fftwpp_amd_synchronize(stream),"item table");
+      work.tags[slot+1]=tag;
+    }
+    const size_t G=(q+D-1)/D;
+    void *part=G > 1 ? work.get(slot,G*rows*L*(dtype == FFTWPP_AMD_F32 ? 8 : 16)) : NULL;
+    const size_t units=rows*G;
+    check(fftwpp_amd_batch_rows(fft->dev,dtype,code,items,rows,f,g,out,1,
+                                tps < 64 ? 64 : tps,std::min<size_t>(units,2048),0,D,
+                                part,rows*L,s,stream),what);
     return;
   }
   const size_t words=fftwpp_amd_hconv_rows_long_part(fft->dev,rows);
@@ -750,21 +786,8 @@ void Convolution::execDevice(void *out, const void *f0, const void *f1, bool nor
   size_t E=eltsize(dtype);
   if(fft->kind() == FFTWPP_AMD_KIND_COMPLEX && fftwpp_amd_batch_rows_tps(fft->dev,dtype) &&
      code != FFTWPP_AMD_MULT_REAL_BINARY) {
-    // a row longer than a power-of-two m <= 4096: the batched register
-    // kernel as a batch of one item, one block per residue (D = 1), the
-    // residue sums added by its second launch
-    void *item=work.get(3,32);
-    if(!itemReady) {
-      const int64_t h[3]={0,(int64_t) n,0};
-      check(fftwpp_amd_memcpy(item,h,sizeof(h),stream),"item table");
-      check(fftwpp_amd_synchronize(stream),"item table");
-      itemReady=true;
-    }
-    void *part=work.get(2,fft->q*n*E);
-    const size_t tps=fftwpp_amd_batch_rows_tps(fft->dev,dtype);
-    check(fftwpp_amd_batch_rows(fft->dev,dtype,code,item,1,f0,f1,out,1,
-                                tps < 64 ? 64 : tps,fft->q,0,1,part,n,s,stream),
-          "batched row");
+    // a row longer than a power-of-two m <= 4096: the batched register kernel
+    fusedRows(fft,work,2,dtype,code,f0,f1,1,n,out,n,s,stream,"batched row");
     if(timeline) timeline->mark("batched row");
     return;
   }

@@ -287,6 +287,7 @@ class DeviceWork {
 public:
   std::vector<void *> bufs;
   std::vector<size_t> sizes;
+  std::vector<size_t> tags;  // what a slot holds (0: nothing known), e.g. an item table
   void *get(size_t slot, size_t bytes);
   ~DeviceWork();
 };
@@ -331,7 +332,6 @@ public:
 protected:
   int multCode;
   DeviceWork work;
-  bool itemReady=false;  // the one-item table of the batched row path is on the device
   void run(Complex **f, size_t offset, bool normalize);
 };
 

@@ -489,6 +489,21 @@ def test_long_hermitian_rows(gpu, oracle, lengths, m):
         assert oracle.rel_l2(h, want) < tol, dtype
 
 
+@pytest.mark.parametrize("shape", [(8, 5000), (3, 4, 6001), (600, 4500)])
+def test_long_complex_inner_rows(gpu, oracle, shape):
+    """Complex 2D/3D convolutions whose inner rows are longer than 4096 run the
+    inner stage on the batched register kernel with the rows as items (one
+    block per residue for few rows, one block per row for many)."""
+    import torch
+    rng = np.random.default_rng(sum(shape))
+    f, g = rnd(rng, *shape), rnd(rng, *shape)
+    want = oracle.explicit_conv(f, g, tuple(2 * L for L in shape))
+    for dtype, tol in ((torch.complex128, TOL64), (torch.complex64, TOL32)):
+        h, plan = run(gpu, "complex", f, g, dtype=dtype)
+        assert plan.geometry()[1][-1] == 4096
+        assert oracle.rel_l2(h, want) < tol, dtype
+
+
 def test_adjoint_long_row(gpu, oracle):
     """The adjoint (correlations) of a row longer than 4096 on the batched path."""
     import torch
