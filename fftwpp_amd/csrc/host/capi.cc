@@ -19,10 +19,12 @@ struct HybridConvolution {
   Application *app;
   fftPad *fft;
   Convolution *convolve;
-  HybridConvolution(size_t L, size_t M=0, size_t m=0, multiplier mult=multBinary) {
+  // real: real data (fftPadReal, as tests/hybridconvr.cc builds it)
+  HybridConvolution(size_t L, size_t M=0, size_t m=0, multiplier mult=multBinary,
+                    bool real=false) {
     if(M == 0) M=2*L-1;
     app=new Application(2,1,mult,fftw::maxthreads,false,m);
-    fft=new fftPad(L,M,*app);
+    fft=real ? new fftPadReal(L,M,*app) : new fftPad(L,M,*app);
     convolve=new Convolution(fft);
   }
   ~HybridConvolution() { delete convolve; delete fft; delete app; }
@@ -62,12 +64,14 @@ struct HybridConvolution2 {
   Application *appx, *appy;
   fftPad *fftx, *ffty;
   Convolution2 *convolve2;
+  // real: real data, the outer dimension an fftPadReal (tests/hybridconvr2.cc)
   HybridConvolution2(size_t Lx, size_t Ly, size_t Mx=0, size_t My=0,
-                     size_t mx=0, size_t my=0, multiplier mult=multBinary) {
+                     size_t mx=0, size_t my=0, multiplier mult=multBinary,
+                     bool real=false) {
     if(Mx == 0) Mx=2*Lx-1;
     if(My == 0) My=2*Ly-1;
     appx=new Application(2,1,multNone,fftw::maxthreads,false,mx);
-    fftx=new fftPad(Lx,Mx,*appx,Ly);
+    fftx=real ? new fftPadReal(Lx,Mx,*appx,Ly) : new fftPad(Lx,Mx,*appx,Ly);
     appy=new Application(2,1,mult,*appx,my);
     ffty=new fftPad(Ly,My,*appy);
     convolve2=new Convolution2(fftx,ffty);
@@ -102,14 +106,15 @@ struct HybridConvolution3 {
   Application *appx, *appy, *appz;
   fftPad *fftx, *ffty, *fftz;
   Convolution3 *convolve3;
+  // real: real data, the outer dimension an fftPadReal (tests/hybridconvr3.cc)
   HybridConvolution3(size_t Lx, size_t Ly, size_t Lz, size_t Mx=0,
                      size_t My=0, size_t Mz=0, size_t mx=0, size_t my=0,
-                     size_t mz=0, multiplier mult=multBinary) {
+                     size_t mz=0, multiplier mult=multBinary, bool real=false) {
     if(Mx == 0) Mx=2*Lx-1;
     if(My == 0) My=2*Ly-1;
     if(Mz == 0) Mz=2*Lz-1;
     appx=new Application(2,1,multNone,fftw::maxthreads,false,mx);
-    fftx=new fftPad(Lx,Mx,*appx,Ly*Lz);
+    fftx=real ? new fftPadReal(Lx,Mx,*appx,Ly*Lz) : new fftPad(Lx,Mx,*appx,Ly*Lz);
     appy=new Application(2,1,multNone,*appx,my);
     ffty=new fftPad(Ly,My,*appy,Lz);
     appz=new Application(2,1,mult,*appy,mz);
@@ -495,6 +500,23 @@ int fftwpp_amd_conv_create(fftwpp_amd_conv **h, int dim, int hermitian,
       c->k1=new CenteredConvolution(L[0],Mz[0],mz[0]);
       c->words=L[0];
       c->dims[0]=c->k1->fft;
+    } else if(hermitian == 3) {
+      // real data: the complex compositions with an fftPadReal outer dimension
+      const bool r=true;
+      if(dim == 1) {
+        c->c1=new HybridConvolution(L[0],Mz[0],mz[0],multBinary,r);
+        c->dims[0]=c->c1->fft;
+        c->words=L[0];
+      } else if(dim == 2) {
+        c->c2=new HybridConvolution2(L[0],L[1],Mz[0],Mz[1],mz[0],mz[1],multBinary,r);
+        c->dims[0]=c->c2->fftx; c->dims[1]=c->c2->ffty;
+        c->words=L[0]*L[1];
+      } else {
+        c->c3=new HybridConvolution3(L[0],L[1],L[2],Mz[0],Mz[1],Mz[2],mz[0],mz[1],mz[2],
+                                     multBinary,r);
+        c->dims[0]=c->c3->fftx; c->dims[1]=c->c3->ffty; c->dims[2]=c->c3->fftz;
+        c->words=L[0]*L[1]*L[2];
+      }
     } else if(dim == 1) {
       if(hermitian) {
         c->h1=new HybridConvolutionHermitian(L[0],Mz[0],mz[0]);
@@ -626,6 +648,10 @@ int fftwpp_amd_conv_adjoint(fftwpp_amd_conv *c, const void *w, const void *a,
       return FFTWPP_AMD_ERR_ARG;
     }
   if(c->hermitian == 1) return hermitian_adjoint(c,w,a,b,grad_a,grad_b,stream,bytes);
+  if(c->hermitian == 3) {
+    fftwpp_amd_internal_set_error("fftwpp_amd_conv_adjoint: no adjoint for real convolutions");
+    return FFTWPP_AMD_ERR_UNSUPPORTED;
+  }
   try {
     if(grad_a) {
       if(!b) return FFTWPP_AMD_ERR_ARG;

@@ -641,25 +641,44 @@ DeviceWork::~DeviceWork()
 static size_t eltsize(int dtype) { return dtype == FFTWPP_AMD_F32 ? 8 : 16; }
 
 // Stage host arrays through device buffers; run `exec` on device pointers.
+// Elements are esize bytes: complex words, or real words for real data
+// (the arrays of fftPadReal convolutions are double arrays).
 template<class Exec>
 static void staged(DeviceWork &work, Complex **f, size_t offset, size_t n,
-                   size_t nin, Exec exec)
+                   size_t nin, size_t esize, Exec exec)
 {
-  bool dev0=fftwpp_amd_is_device_pointer(f[0]);
-  bool dev1=fftwpp_amd_is_device_pointer(f[1]);
+  char *a=(char *) f[0]+offset*esize, *b=(char *) f[1]+offset*esize;
+  bool dev0=fftwpp_amd_is_device_pointer(a);
+  bool dev1=fftwpp_amd_is_device_pointer(b);
   if(dev0 && dev1) {
-    exec(f[0]+offset,f[1]+offset);
+    exec(a,b);
     check(fftwpp_amd_synchronize(NULL),"synchronize");
     return;
   }
-  size_t bytes=n*sizeof(Complex);
-  Complex *d0=(Complex *) work.get(100,bytes);
-  Complex *d1=(Complex *) work.get(101,bytes);
-  check(fftwpp_amd_memcpy(d0,f[0]+offset,bytes,NULL),"copy in");
-  if(nin > 1) check(fftwpp_amd_memcpy(d1,f[1]+offset,bytes,NULL),"copy in");
+  size_t bytes=n*esize;
+  void *d0=work.get(100,bytes);
+  void *d1=work.get(101,bytes);
+  check(fftwpp_amd_memcpy(d0,a,bytes,NULL),"copy in");
+  if(nin > 1) check(fftwpp_amd_memcpy(d1,b,bytes,NULL),"copy in");
   exec(d0,d1);
-  check(fftwpp_amd_memcpy(f[0]+offset,d0,bytes,NULL),"copy out");
+  check(fftwpp_amd_memcpy(a,d0,bytes,NULL),"copy out");
   check(fftwpp_amd_synchronize(NULL),"synchronize");
+}
+
+// A binary convolution of real arrays f0, f1 (n real words each) into out:
+// z = f0 + i f1 in work slot 6, the complex self-convolution `self` of z in
+// place, out = Im z / 2.
+template<class Self>
+static void realConvolution(DeviceWork &work, size_t n, int dtype, int code,
+                            void *out, const void *f0, const void *f1,
+                            void *stream, Self self)
+{
+  if(code != FFTWPP_AMD_MULT_BINARY)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"real convolutions take multBinary");
+  void *z=work.get(6,n*(dtype == FFTWPP_AMD_F32 ? 8 : 16));
+  check(fftwpp_amd_real_pack(dtype,f0,f1,z,n,stream),"real pack");
+  self(z);
+  check(fftwpp_amd_real_unpack(dtype,z,out,n,0.5,stream),"real unpack");
 }
 
 static void requireBinary(size_t A, size_t B, int code)
@@ -767,6 +786,17 @@ void Convolution::execDevice(void *out, const void *f0, const void *f1, bool nor
 {
   int code=mult >= 0 ? mult : multCode;
   requireBinary(A,B,code);
+  if(fft->realInput())
+    realConvolution(work,arraySize(),dtype,code,out,f0,f1,stream,[&](void *z) {
+      execComplex(z,z,z,normalize,dtype,stream,code);
+    });
+  else
+    execComplex(out,f0,f1,normalize,dtype,stream,code);
+}
+
+void Convolution::execComplex(void *out, const void *f0, const void *f1, bool normalize,
+                              int dtype, void *stream, int code)
+{
   if(fft->C != 1)
     throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"1D Convolution with C > 1");
   double s=normalize ? scale : 1.0;
@@ -804,9 +834,8 @@ void Convolution::execDevice(void *out, const void *f0, const void *f1, bool nor
 
 void Convolution::run(Complex **f, size_t offset, bool normalize)
 {
-  staged(work,f,offset,arraySize(),2,[&](Complex *a, Complex *b) {
-    execDevice(a,a,b,normalize,FFTWPP_AMD_F64,NULL);
-  });
+  staged(work,f,offset,arraySize(),2,fft->realInput() ? sizeof(double) : sizeof(Complex),
+         [&](void *a, void *b) { execDevice(a,a,b,normalize,FFTWPP_AMD_F64,NULL); });
 }
 
 // ------------------------------------------------------------------------
@@ -903,6 +932,17 @@ void Convolution2::execDevice(void *out, const void *f0, const void *f1, bool no
 {
   int code=mult >= 0 ? mult : multCode;
   requireBinary(A,B,code);
+  if(fftx->realInput())
+    realConvolution(work,arraySize(),dtype,code,out,f0,f1,stream,[&](void *z) {
+      execComplex(z,z,z,normalize,dtype,stream,code);
+    });
+  else
+    execComplex(out,f0,f1,normalize,dtype,stream,code);
+}
+
+void Convolution2::execComplex(void *out, const void *f0, const void *f1, bool normalize,
+                               int dtype, void *stream, int code)
+{
   if(!rowsSupported(ffty,dtype))
     throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner sub-FFT size beyond the fused row kernels");
   double s=normalize ? scale : 1.0;
@@ -924,9 +964,8 @@ void Convolution2::execDevice(void *out, const void *f0, const void *f1, bool no
 
 void Convolution2::run(Complex **f, size_t offset, bool normalize)
 {
-  staged(work,f,offset,arraySize(),2,[&](Complex *a, Complex *b) {
-    execDevice(a,a,b,normalize,FFTWPP_AMD_F64,NULL);
-  });
+  staged(work,f,offset,arraySize(),2,fftx->realInput() ? sizeof(double) : sizeof(Complex),
+         [&](void *a, void *b) { execDevice(a,a,b,normalize,FFTWPP_AMD_F64,NULL); });
 }
 
 // ------------------------------------------------------------------------
@@ -966,6 +1005,17 @@ void Convolution3::execDevice(void *out, const void *f0, const void *f1, bool no
 {
   int code=mult >= 0 ? mult : multCode;
   requireBinary(A,B,code);
+  if(fftx->realInput())
+    realConvolution(work,arraySize(),dtype,code,out,f0,f1,stream,[&](void *z) {
+      execComplex(z,z,z,normalize,dtype,stream,code);
+    });
+  else
+    execComplex(out,f0,f1,normalize,dtype,stream,code);
+}
+
+void Convolution3::execComplex(void *out, const void *f0, const void *f1, bool normalize,
+                               int dtype, void *stream, int code)
+{
   if(!rowsSupported(fftz,dtype))
     throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner sub-FFT size beyond the fused row kernels");
   double s=normalize ? scale : 1.0;
@@ -1005,9 +1055,8 @@ void Convolution3::execDevice(void *out, const void *f0, const void *f1, bool no
 
 void Convolution3::run(Complex **f, size_t offset, bool normalize)
 {
-  staged(work,f,offset,arraySize(),2,[&](Complex *a, Complex *b) {
-    execDevice(a,a,b,normalize,FFTWPP_AMD_F64,NULL);
-  });
+  staged(work,f,offset,arraySize(),2,fftx->realInput() ? sizeof(double) : sizeof(Complex),
+         [&](void *a, void *b) { execDevice(a,a,b,normalize,FFTWPP_AMD_F64,NULL); });
 }
 
 } // namespace fftwpp

@@ -312,6 +312,22 @@ __global__ void multiply_kernel(int mult, cplx<T> *F0, const cplx<T> *F1,
   }
 }
 
+// Real convolutions: z = f0 + i f1, and out = s Im z (the halved imaginary
+// part of the self-convolution of z is f0 * f1).
+template<typename T>
+__global__ void real_pack_kernel(const T *f0, const T *f1, cplx<T> *z, int64_t n)
+{
+  int64_t i=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  if(i < n) z[i]=mk(f0[i],f1[i]);
+}
+
+template<typename T>
+__global__ void real_unpack_kernel(const cplx<T> *z, T *out, int64_t n, T s)
+{
+  int64_t i=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  if(i < n) out[i]=s*z[i].y;
+}
+
 template<typename T>
 __global__ void scale_kernel(cplx<T> *f, int64_t n, T s)
 {
@@ -1205,6 +1221,38 @@ int fftwpp_amd_batch_rows(const fftwpp_amd_dim *d, int dtype, int mult,
   b.D=D < d->q ? D : d->q; b.ptotal=ptotal; b.ld=0; b.lds=0; b.scale=scale;
   return dtype == FFTWPP_AMD_F32 ? batch_rows_t<float>(d,mult,b,st) :
     batch_rows_t<double>(d,mult,b,st);
+}
+
+int fftwpp_amd_real_pack(int dtype, const void *f0, const void *f1, void *z,
+                         size_t n, void *stream)
+{
+  if(n == 0) return FFTWPP_AMD_OK;
+  if(!f0 || !f1 || !z) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  const unsigned nb=(unsigned) ((n+255)/256);
+  hipStream_t st=(hipStream_t) stream;
+  if(dtype == FFTWPP_AMD_F32)
+    hipLaunchKernelGGL(real_pack_kernel<float>,dim3(nb),dim3(256),0,st,
+                       (const float *) f0,(const float *) f1,(cplx<float> *) z,(int64_t) n);
+  else
+    hipLaunchKernelGGL(real_pack_kernel<double>,dim3(nb),dim3(256),0,st,
+                       (const double *) f0,(const double *) f1,(cplx<double> *) z,(int64_t) n);
+  return launch_check("real_pack_kernel");
+}
+
+int fftwpp_amd_real_unpack(int dtype, const void *z, void *out, size_t n,
+                           double scale, void *stream)
+{
+  if(n == 0) return FFTWPP_AMD_OK;
+  if(!z || !out) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  const unsigned nb=(unsigned) ((n+255)/256);
+  hipStream_t st=(hipStream_t) stream;
+  if(dtype == FFTWPP_AMD_F32)
+    hipLaunchKernelGGL(real_unpack_kernel<float>,dim3(nb),dim3(256),0,st,
+                       (const cplx<float> *) z,(float *) out,(int64_t) n,(float) scale);
+  else
+    hipLaunchKernelGGL(real_unpack_kernel<double>,dim3(nb),dim3(256),0,st,
+                       (const cplx<double> *) z,(double *) out,(int64_t) n,scale);
+  return launch_check("real_unpack_kernel");
 }
 
 int fftwpp_amd_multiply(int dtype, int mult, void *F0, const void *F1,

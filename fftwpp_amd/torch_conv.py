@@ -12,10 +12,16 @@ import torch
 
 from ._lib import F32, F64, check, lib, sizes
 
-KINDS = {"complex": 0, "hermitian": 1, "centered": 2}
+KINDS = {"complex": 0, "hermitian": 1, "centered": 2, "real": 3}
 
 
-def _dtype_code(dtype):
+def _dtype_code(dtype, kind="complex"):
+    if kind == "real":
+        if dtype == torch.float64:
+            return F64
+        if dtype == torch.float32:
+            return F32
+        raise TypeError("real convolutions take torch.float64 or torch.float32")
     if dtype == torch.complex128:
         return F64
     if dtype == torch.complex64:
@@ -34,8 +40,10 @@ class DeviceConvolution:
     (Lx,Ly,Lz) (HybridConvolution{,2,3}); kind="hermitian": (H,), (Lx,Hy) or
     (Lx,Ly,Hz) with odd full length 2H-1 on the last axis unless
     ``lengths`` gives the full lengths (HybridConvolutionHermitian{,2,3});
-    kind="centered": (L,) centred 1D.  M and m optionally give the minimum
-    padded sizes and forced sub-FFT sizes per dimension.
+    kind="centered": (L,) centred 1D; kind="real": (L,), (Lx,Ly) or
+    (Lx,Ly,Lz) real data (fftPadReal outer dimension, tests/hybridconvr*.cc;
+    dtype torch.float64 or torch.float32).  M and m optionally give the
+    minimum padded sizes and forced sub-FFT sizes per dimension.
     """
 
     def __init__(self, shape, kind="complex", dtype=torch.complex128,
@@ -55,7 +63,7 @@ class DeviceConvolution:
         check(lib().fftwpp_amd_conv_create(
             ctypes.byref(h), self.dim, KINDS[kind], sizes(self.lengths),
             sizes(M) if M is not None else None,
-            sizes(m) if m is not None else None, _dtype_code(dtype)))
+            sizes(m) if m is not None else None, _dtype_code(dtype, kind)))
         self._h = h
 
     def __del__(self):

@@ -190,6 +190,8 @@ public:
 
   // Device transform kind of this dimension.
   virtual int kind() const { return FFTWPP_AMD_KIND_COMPLEX; }
+  // Real input data (fftPadReal): the convolution drivers take double arrays.
+  virtual bool realInput() const { return false; }
 
   // Residue block r of the padded transform of DEVICE data f (stride S,
   // count C) into F, laid out as the reference's routines lay it out:
@@ -238,6 +240,22 @@ public:
 
   bool conjugates() { return D > 1 && (p == 1 || p % 2 == 0); }
   int kind() const { return FFTWPP_AMD_KIND_CENTERED; }
+};
+
+// Real data (fftPadReal, convolve.h:805-980 of the reference): the outer (or
+// only) dimension of a real convolution, with the reference's constructors.
+// On the device a binary convolution of real f and g is the imaginary part of
+// the complex self-convolution of z = f + i g, halved ((f + i g) * (f + i g)
+// = f*f - g*g + 2 i f*g), so the dimension is planned and transformed as a
+// complex fftPad and the drivers pack and unpack the real arrays.
+class fftPadReal : public fftPad {
+public:
+  fftPadReal(size_t L, size_t M, Application &app, size_t C=1, size_t S=0,
+             bool Explicit=false) : fftPad(L,M,app,C,S,Explicit) {}
+  fftPadReal(size_t L, size_t M, Application &app, size_t C, size_t S,
+             size_t m, size_t D, bool inplace) : fftPad(L,M,app,C,S,m,D,inplace) {}
+  bool realInput() const { return true; }
+  size_t wordSize() { return 1; }
 };
 
 class fftPadHermitian : public fftBase {
@@ -333,6 +351,9 @@ protected:
   int multCode;
   DeviceWork work;
   void run(Complex **f, size_t offset, bool normalize);
+  // The complex device convolution (execDevice on complex arrays).
+  void execComplex(void *out, const void *f0, const void *f1, bool normalize,
+                   int dtype, void *stream, int code);
 };
 
 void HermitianSymmetrize(Complex *f);
@@ -383,6 +404,9 @@ protected:
   size_t ly;  // stored length of a y row (Ly or ceil(Ly/2))
   DeviceWork work;
   void run(Complex **f, size_t offset, bool normalize);
+  // The complex device convolution (execDevice on complex arrays).
+  void execComplex(void *out, const void *f0, const void *f1, bool normalize,
+                   int dtype, void *stream, int code);
 };
 
 class Convolution3 : public ThreadBase {
@@ -428,6 +452,9 @@ protected:
   size_t lz;  // stored length of a z row (Lz or ceil(Lz/2))
   DeviceWork work;
   void run(Complex **f, size_t offset, bool normalize);
+  // The complex device convolution (execDevice on complex arrays).
+  void execComplex(void *out, const void *f0, const void *f1, bool normalize,
+                   int dtype, void *stream, int code);
 };
 
 } // namespace fftwpp

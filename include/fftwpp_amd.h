@@ -142,6 +142,15 @@ int fftwpp_amd_batch_rows(const fftwpp_amd_dim *d, int dtype, int mult,
                           size_t nblocks, size_t lacc, size_t D, void *part,
                           size_t ptotal, double scale, void *stream);
 
+/* Real convolutions (fftPadReal, convolve.h:805-980; tests/hybridconvr*.cc):
+ * z[i] = f0[i] + i f1[i] for n real words of f0 and f1, and out[i] =
+ * scale * Im z[i]; between the two, a complex self-convolution of z gives
+ * 2 i f0 * f1 in its imaginary part. */
+int fftwpp_amd_real_pack(int dtype, const void *f0, const void *f1, void *z,
+                         size_t n, void *stream);
+int fftwpp_amd_real_unpack(int dtype, const void *z, void *out, size_t n,
+                           double scale, void *stream);
+
 /* The multipliers on device arrays (convolve.cc:26-110): n counts complex
  * words for multBinary/multcorrelation and real words for realMultBinary. */
 int fftwpp_amd_multiply(int dtype, int mult, void *F0, const void *F1,
@@ -199,6 +208,9 @@ typedef struct fftwpp_amd_conv fftwpp_amd_conv;
  * hermitian = 1: centred Hermitian (HybridConvolutionHermitian{,2,3},
  *   HybridConvolution.h:33-52,80-111,149-182); L[] are the full lengths and
  *   the last stored dimension holds ceil(L/2) modes.
+ * hermitian = 3: real data (fftPadReal as the outer or only dimension,
+ *   tests/hybridconvr{,2,3}.cc), arrays of L[0] x ... real words (float64
+ *   or float32); multBinary only, no adjoint.
  * M[d] = 0 selects the reference default (2L-1, or 3ceil(L/2)-2(L mod 2));
  * m[d] = 0 lets the library choose the sub-FFT size (autotuned plan).
  * The result is normalised and written over `a`. */
