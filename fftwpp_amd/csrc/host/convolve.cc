@@ -621,12 +621,12 @@ void *DeviceWork::get(size_t slot, size_t bytes)
     bufs.resize(slot+1,NULL);
     sizes.resize(slot+1,0);
   }
-  if(slot >= tags.size()) tags.resize(slot+1,0);
+  if(slot >= keys.size()) keys.resize(slot+1);
   if(sizes[slot] < bytes) {
     fftwpp_amd_free(bufs[slot]);
     bufs[slot]=NULL;
     sizes[slot]=0;
-    tags[slot]=0;
+    keys[slot].clear();
     check(fftwpp_amd_malloc(&bufs[slot],bytes),"device work buffer");
     sizes[slot]=bytes;
   }
@@ -752,8 +752,8 @@ static void fusedRows(fftBase *fft, DeviceWork &work, size_t slot, int dtype,
     const size_t L=fft->L, q=fft->q;
     const size_t D=rows >= 512 ? (q+1)/2 : 1;
     void *items=work.get(slot+1,rows*3*sizeof(int64_t));
-    const size_t tag=rows*1000003u+rs*7919u+L;
-    if(work.tags[slot+1] != tag) {
+    const std::vector<size_t> key={rows,rs,L};
+    if(work.keys[slot+1] != key) {
       std::vector<int64_t> h(3*rows);
       for(size_t i=0; i < rows; ++i) {
         h[3*i]=(int64_t) (i*rs);
@@ -762,7 +762,7 @@ static void fusedRows(fftBase *fft, DeviceWork &work, size_t slot, int dtype,
       }
       check(fftwpp_amd_memcpy(items,h.data(),h.size()*sizeof(int64_t),stream),"item table");
       check(fftwpp_amd_synchronize(stream),"item table");
-      work.tags[slot+1]=tag;
+      work.keys[slot+1]=key;
     }
     const size_t G=(q+D-1)/D;
     void *part=G > 1 ? work.get(slot,G*rows*L*(dtype == FFTWPP_AMD_F32 ? 8 : 16)) : NULL;

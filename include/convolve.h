@@ -305,7 +305,7 @@ class DeviceWork {
 public:
   std::vector<void *> bufs;
   std::vector<size_t> sizes;
-  std::vector<size_t> tags;  // what a slot holds (0: nothing known), e.g. an item table
+  std::vector<std::vector<size_t>> keys;  // what a slot holds (empty: nothing known), e.g. an item table
   void *get(size_t slot, size_t bytes);
   ~DeviceWork();
 };
