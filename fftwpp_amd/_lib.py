@@ -58,6 +58,8 @@ SIGNATURES = {
     "fftwpp_amd_conv_execute_timed": (I, [P, P, P, P, P, ctypes.POINTER(ctypes.c_float), I,
                                           ctypes.POINTER(I)]),
     "fftwpp_amd_conv_stage_name": (ctypes.c_char_p, [P, I]),
+    "fftwpp_amd_conv_create_general": (I, [ctypes.POINTER(P), I, I, ZP, ZP, ZP, I, Z, Z, P, P]),
+    "fftwpp_amd_conv_execute_general": (I, [P, ctypes.POINTER(P), P]),
     # batches of 1D convolutions
     "fftwpp_amd_batch_create": (I, [ctypes.POINTER(P), Z, ZP, ZP, I, I, I]),
     "fftwpp_amd_batch_destroy": (I, [P]),

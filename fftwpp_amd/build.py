@@ -51,7 +51,8 @@ def build(verbose=False):
         _run([cxx] + CXX_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs, verbose)
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-Wl,-soname,libfftwpp_amd.so",
+          "-o", tmp] + objs, verbose)
     os.replace(tmp, LIB)
     return LIB
 

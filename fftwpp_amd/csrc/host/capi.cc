@@ -4,6 +4,7 @@
 #include "convolve.h"
 #include "cfftw++.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <iostream>
@@ -371,6 +372,13 @@ struct fftwpp_amd_conv {
   void *adj[3]={nullptr,nullptr,nullptr};   // Hermitian adjoint temporaries
 
   fftBase *dims[3];
+  // user-multiplier plans (conv_create_general): one Application and one
+  // complex or centred transform per dimension
+  Application *gapp[3]={nullptr,nullptr,nullptr};
+  fftBase *gdims[3]={nullptr,nullptr,nullptr};
+  size_t A=2, B=1;
+  fftwpp_amd_multiplier umult=nullptr;
+  void *uuser=nullptr;
   fftwpp::Timeline timeline;
   void setTimeline(fftwpp::Timeline *t) {
     if(c1) c1->convolve->timeline=t;
@@ -390,6 +398,7 @@ struct fftwpp_amd_conv {
     else if(h2) h2->convolve2->execDevice(out,a,b,norm,dtype,stream,mult);
     else if(c3) c3->convolve3->execDevice(out,a,b,norm,dtype,stream,mult);
     else if(h3) h3->convolve3->execDevice(out,a,b,norm,dtype,stream,mult);
+    else throw fftwpp::Error(FFTWPP_AMD_ERR_UNSUPPORTED,"a user-multiplier plan runs through fftwpp_amd_conv_execute_general");
   }
 };
 
@@ -564,6 +573,7 @@ int fftwpp_amd_conv_destroy(fftwpp_amd_conv *c)
   if(!c) return FFTWPP_AMD_OK;
   delete c->c1; delete c->h1; delete c->k1; delete c->c2; delete c->h2; delete c->c3;
   delete c->h3;
+  for(int d=0; d < 3; ++d) { delete c->gdims[d]; delete c->gapp[d]; }
   for(void *p : c->adj)
     if(p) fftwpp_amd_free(p);
   delete c;
@@ -667,6 +677,69 @@ int fftwpp_amd_conv_adjoint(fftwpp_amd_conv *c, const void *w, const void *a,
     }
   } catch(const std::exception &e) {
     return conv_status("fftwpp_amd_conv_adjoint",e);
+  }
+  return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_conv_create_general(fftwpp_amd_conv **h, int dim, int centred,
+                                   const size_t *L, const size_t *M,
+                                   const size_t *m, int dtype, size_t A,
+                                   size_t B, fftwpp_amd_multiplier mult,
+                                   void *user)
+{
+  if(!h || !L || !mult) return FFTWPP_AMD_ERR_ARG;
+  *h=NULL;
+  if(dim < 1 || dim > 3) return FFTWPP_AMD_ERR_ARG;
+  if(dtype != FFTWPP_AMD_F64 && dtype != FFTWPP_AMD_F32) return FFTWPP_AMD_ERR_ARG;
+  if(A < 1 || B < 1 || A > 8 || B > 8) return FFTWPP_AMD_ERR_ARG;
+  for(int d=0; d < dim; ++d)
+    if(L[d] == 0) return FFTWPP_AMD_ERR_ARG;
+  fftwpp_amd_conv *c=new fftwpp_amd_conv();
+  c->dim=dim; c->hermitian=centred ? 2 : 0; c->dtype=dtype;
+  c->A=A; c->B=B; c->umult=mult; c->uuser=user;
+  c->words=1;
+  try {
+    for(int d=0; d < dim; ++d) {
+      size_t Md=M && M[d] ? M[d] : centred ? hdefault(L[d]) : 2*L[d]-1;
+      c->gapp[d]=new Application(A,B,multNone,fftw::maxthreads,false,m ? m[d] : 0);
+      if(centred)
+        c->gdims[d]=new fftPadCentered(L[d],Md,*c->gapp[d]);
+      else
+        c->gdims[d]=new fftPad(L[d],Md,*c->gapp[d]);
+      c->dims[d]=c->gdims[d];
+      c->words *= L[d];
+    }
+  } catch(const std::exception &e) {
+    fftwpp_amd_conv_destroy(c);
+    return conv_status("fftwpp_amd_conv_create_general",e);
+  }
+  *h=c;
+  return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_conv_execute_general(fftwpp_amd_conv *c, void *const *f,
+                                    void *stream)
+{
+  if(!c || !f) return FFTWPP_AMD_ERR_ARG;
+  if(!c->umult) {
+    fftwpp_amd_internal_set_error("fftwpp_amd_conv_execute_general: the plan has no user multiplier");
+    return FFTWPP_AMD_ERR_ARG;
+  }
+  const size_t na=std::max(c->A,c->B);
+  for(size_t a=0; a < na; ++a)
+    if(!f[a] || !fftwpp_amd_is_device_pointer(f[a])) {
+      fftwpp_amd_internal_set_error("fftwpp_amd_conv_execute_general: f[a] must be device pointers");
+      return FFTWPP_AMD_ERR_ARG;
+    }
+  double N=1.0;
+  for(int d=0; d < c->dim; ++d) N *= (double) (c->gdims[d]->m*c->gdims[d]->q);
+  try {
+    fftwpp::generalConvolution(c->gdims,c->dim,c->A,c->B,c->work,c->dtype,f,1.0/N,stream,
+                               [&](void **F, size_t n, const fftwpp_amd_layout &layout) {
+                                 c->umult(F,n,&layout,stream,c->uuser);
+                               });
+  } catch(const std::exception &e) {
+    return conv_status("fftwpp_amd_conv_execute_general",e);
   }
   return FFTWPP_AMD_OK;
 }

@@ -684,9 +684,122 @@ static void realConvolution(DeviceWork &work, size_t n, int dtype, int code,
 static void requireBinary(size_t A, size_t B, int code)
 {
   if(code < 0)
-    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"user-defined host multipliers are not supported; use multBinary, realMultBinary or multcorrelation");
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"a user multiplier runs through convolve() or fftwpp_amd_conv_execute_general");
   if(A != 2 || B != 1)
     throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"only binary convolutions (A=2, B=1) run on the device");
+}
+
+// ------------------------------------------------------------------------
+// User multipliers (the multiplier/Indices contract, convolve.h:48-82 and
+// Convolution::operate, convolve.h:1120-1135): every dimension expanded to
+// all its residues, the multiplier called once on the padded arrays.
+// ------------------------------------------------------------------------
+// forward_many/backward_many take one grid row per batch entry: batches are
+// split to stay within the grid's y extent.
+static const size_t maxBatch=65535;
+
+void generalConvolution(fftBase *const *dims, size_t nd, size_t A, size_t B,
+                        DeviceWork &work, int dtype, void *const *f,
+                        double scale, void *stream, const DeviceMultiplier &mult)
+{
+  if(nd < 1 || nd > 3) throw Error(FFTWPP_AMD_ERR_ARG,"one to three dimensions");
+  if(A < 1 || B < 1 || A > 8 || B > 8) throw Error(FFTWPP_AMD_ERR_ARG,"1 <= A, B <= 8 arrays");
+  size_t Ld[3], Nd[3];
+  size_t total=1;
+  for(size_t d=0; d < nd; ++d) {
+    if(dims[d]->kind() == FFTWPP_AMD_KIND_HERMITIAN || dims[d]->realInput())
+      throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"user multipliers take complex or centred dimensions");
+    Ld[d]=dims[d]->L;
+    Nd[d]=dims[d]->m*dims[d]->q;
+    total *= Nd[d];
+  }
+  // Stage d turns outer(d) x L_d x inner(d) words into outer(d) x N_d x
+  // inner(d): outer(d) = N_0...N_{d-1}, inner(d) = L_{d+1}...L_{nd-1}.
+  auto outer=[&](size_t d) { size_t w=1; for(size_t e=0; e < d; ++e) w *= Nd[e]; return w; };
+  auto inner=[&](size_t d) { size_t w=1; for(size_t e=d+1; e < nd; ++e) w *= Ld[e]; return w; };
+  const size_t E=eltsize(dtype);
+  const size_t na=std::max(A,B);
+  std::vector<void *> F(na);
+  for(size_t a=0; a < na; ++a) F[a]=work.get(20+a,total*E);
+  // P[d]: the output of forward stage d < nd-1 (and input of backward d+1)
+  char *P[2]={NULL,NULL};
+  for(size_t d=0; d+1 < nd; ++d)
+    P[d]=(char *) work.get(18+d,outer(d+1)*inner(d)*E);
+
+  for(size_t a=0; a < A; ++a)
+    for(size_t d=0; d < nd; ++d) {
+      const size_t out=outer(d), in=inner(d);
+      const char *src=d == 0 ? (const char *) f[a] : P[d-1];
+      char *dst=d+1 == nd ? (char *) F[a] : P[d];
+      for(size_t b0=0; b0 < out; b0 += maxBatch)
+        check(fftwpp_amd_forward_many(dims[d]->dev,dtype,src+b0*Ld[d]*in*E,in,in,
+                                      std::min(maxBatch,out-b0),Ld[d]*in,
+                                      dst+b0*Nd[d]*in*E,in,Nd[d]*in,0,dims[d]->q,
+                                      stream),"user multiplier: forward");
+    }
+
+  fftwpp_amd_layout layout;
+  memset(&layout,0,sizeof(layout));
+  layout.dims=(int) nd;
+  layout.dtype=dtype;
+  layout.A=A;
+  layout.B=B;
+  for(size_t d=0; d < nd; ++d) {
+    layout.N[d]=Nd[d];
+    layout.m[d]=dims[d]->m;
+    layout.q[d]=dims[d]->q;
+    size_t st=1;
+    for(size_t e=d+1; e < nd; ++e) st *= Nd[e];
+    layout.stride[d]=st;
+  }
+  mult(F.data(),total,layout);
+
+  for(size_t b=0; b < B; ++b)
+    for(size_t d=nd; d-- > 0;) {
+      const size_t out=outer(d), in=inner(d);
+      const char *src=d+1 == nd ? (const char *) F[b] : P[d];
+      char *dst=d == 0 ? (char *) f[b] : P[d-1];
+      for(size_t b0=0; b0 < out; b0 += maxBatch)
+        check(fftwpp_amd_backward_many(dims[d]->dev,dtype,src+b0*Nd[d]*in*E,in,Nd[d]*in,
+                                       dst+b0*Ld[d]*in*E,in,in,std::min(maxBatch,out-b0),
+                                       Ld[d]*in,0,dims[d]->q,0,d == 0 ? scale : 1.0,
+                                       stream),"user multiplier: backward");
+    }
+}
+
+// The C++ face of a user multiplier: f[0..max(A,B)-1] host or device arrays
+// of `words` complex words from `offset`; host arrays are staged through
+// device buffers.  The multiplier sees the padded device arrays and finds
+// their layout and stream in `indices`.
+static void userConvolution(fftBase *const *dims, size_t nd, size_t A, size_t B,
+                            multiplier *mult, Indices &indices, size_t threads,
+                            DeviceWork &work, Complex **f, size_t offset,
+                            size_t words, double scale)
+{
+  const size_t na=std::max(A,B);
+  const size_t bytes=words*sizeof(Complex);
+  std::vector<void *> d(na);
+  std::vector<bool> host(na);
+  for(size_t a=0; a < na; ++a) {
+    Complex *p=f[a]+offset;
+    host[a]=!fftwpp_amd_is_device_pointer(p);
+    d[a]=host[a] ? work.get(40+a,bytes) : (void *) p;
+    if(host[a] && a < A)
+      check(fftwpp_amd_memcpy(d[a],p,bytes,NULL),"copy in");
+  }
+  generalConvolution(dims,nd,A,B,work,FFTWPP_AMD_F64,d.data(),scale,NULL,
+                     [&](void **F, size_t n, const fftwpp_amd_layout &layout) {
+                       indices.layout=&layout;
+                       indices.stream=NULL;
+                       indices.r=0;
+                       indices.offset=0;
+                       mult((Complex **) F,n,&indices,threads);
+                       indices.layout=NULL;
+                     });
+  for(size_t b=0; b < B; ++b)
+    if(host[b])
+      check(fftwpp_amd_memcpy(f[b]+offset,d[b],bytes,NULL),"copy out");
+  check(fftwpp_amd_synchronize(NULL),"synchronize");
 }
 
 // ------------------------------------------------------------------------
@@ -834,6 +947,14 @@ void Convolution::execComplex(void *out, const void *f0, const void *f1, bool no
 
 void Convolution::run(Complex **f, size_t offset, bool normalize)
 {
+  if(multCode < 0) {
+    fftBase *dims[1]={fft};
+    if(fft->C != 1)
+      throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"1D Convolution with C > 1");
+    userConvolution(dims,1,A,B,mult,indices,threads,work,f,offset,arraySize(),
+                    normalize ? scale : 1.0);
+    return;
+  }
   staged(work,f,offset,arraySize(),2,fft->realInput() ? sizeof(double) : sizeof(Complex),
          [&](void *a, void *b) { execDevice(a,a,b,normalize,FFTWPP_AMD_F64,NULL); });
 }
@@ -913,6 +1034,8 @@ Convolution2::Convolution2(fftBase *fftx, fftBase *ffty, Complex **,
   ly=ffty->inputLength();
   scale=1.0/normalization();
   multCode=multiplierCode(mult);
+  indices.copy(NULL,1);
+  indices.fft=ffty;
   if(ly != Ly) throw Error(FFTWPP_AMD_ERR_ARG,"fftx->C must equal the stored length of the y transform");
   if(ffty->C != 1) throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner transform with C > 1");
 }
@@ -964,6 +1087,14 @@ void Convolution2::execComplex(void *out, const void *f0, const void *f1, bool n
 
 void Convolution2::run(Complex **f, size_t offset, bool normalize)
 {
+  if(multCode < 0) {
+    fftBase *dims[2]={fftx,ffty};
+    if(Sx != Ly)
+      throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"user multipliers take contiguous arrays");
+    userConvolution(dims,2,A,B,mult,indices,threads,work,f,offset,arraySize(),
+                    normalize ? scale : 1.0);
+    return;
+  }
   staged(work,f,offset,arraySize(),2,fftx->realInput() ? sizeof(double) : sizeof(Complex),
          [&](void *a, void *b) { execDevice(a,a,b,normalize,FFTWPP_AMD_F64,NULL); });
 }
@@ -984,6 +1115,8 @@ Convolution3::Convolution3(fftBase *fftx, fftBase *ffty, fftBase *fftz,
   lz=fftz->inputLength();
   scale=1.0/normalization();
   multCode=multiplierCode(mult);
+  indices.copy(NULL,2);
+  indices.fft=fftz;
   if(lz != Lz) throw Error(FFTWPP_AMD_ERR_ARG,"ffty->C must equal the stored length of the z transform");
   if(Sx < Ly*Sy) throw Error(FFTWPP_AMD_ERR_ARG,"Sx cannot be less than Ly*Sy");
   if(fftx->C != (contiguous() ? Ly*Sy : Lz)) throw Error(FFTWPP_AMD_ERR_ARG,"fftx->C is invalid");
@@ -1055,6 +1188,14 @@ void Convolution3::execComplex(void *out, const void *f0, const void *f1, bool n
 
 void Convolution3::run(Complex **f, size_t offset, bool normalize)
 {
+  if(multCode < 0) {
+    fftBase *dims[3]={fftx,ffty,fftz};
+    if(Sy != Lz || Sx != Ly*Lz)
+      throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"user multipliers take contiguous arrays");
+    userConvolution(dims,3,A,B,mult,indices,threads,work,f,offset,arraySize(),
+                    normalize ? scale : 1.0);
+    return;
+  }
   staged(work,f,offset,arraySize(),2,fftx->realInput() ? sizeof(double) : sizeof(Complex),
          [&](void *a, void *b) { execDevice(a,a,b,normalize,FFTWPP_AMD_F64,NULL); });
 }

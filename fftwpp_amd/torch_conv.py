@@ -174,3 +174,62 @@ def convolve(a, b, plan):
     Hermitian plan the inputs are symmetrized first (h = S a (*) S b), so the
     function and its gradient are defined for any stored arrays."""
     return _ConvolveFn.apply(a, b, plan)
+
+
+class UserConvolution:
+    """Plan of a dealiased convolution of A device tensors into B with a user
+    device multiplier (fftwpp_amd_conv_create_general; Application(A,B,mult)
+    of convolve.h:84-121 with a multiplier that runs on the device).
+
+    ``mult`` is the address of an ``fftwpp_amd_multiplier`` callback compiled
+    into a HIP library (e.g. ``ctypes.cast(cdll.my_mult, ctypes.c_void_p)``);
+    it is called once per convolution with the whole padded transforms.
+    ``shape`` is (L,), (Lx,Ly) or (Lx,Ly,Lz) of contiguous complex data;
+    ``centered`` makes every dimension centred.  ``user`` is passed through.
+    """
+
+    def __init__(self, shape, A, B, mult, dtype=torch.complex128, centered=False,
+                 M=None, m=None, user=None):
+        if isinstance(shape, int):
+            shape = (shape,)
+        self.shape = tuple(int(s) for s in shape)
+        self.dim = len(self.shape)
+        self.A, self.B = int(A), int(B)
+        self.dtype = dtype
+        self._mult = ctypes.c_void_p(mult.value if isinstance(mult, ctypes.c_void_p) else int(mult))
+        h = ctypes.c_void_p()
+        check(lib().fftwpp_amd_conv_create_general(
+            ctypes.byref(h), self.dim, int(bool(centered)), sizes(self.shape),
+            sizes(M) if M is not None else None, sizes(m) if m is not None else None,
+            _dtype_code(dtype), self.A, self.B, self._mult, user))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().fftwpp_amd_conv_destroy(h)
+            except Exception:  # interpreter shutdown: module globals already gone
+                pass
+            self._h = None
+
+    def geometry(self):
+        """(N, m, q) per dimension: padded length, sub-FFT size, residues."""
+        N, m, q = sizes([0] * 3), sizes([0] * 3), sizes([0] * 3)
+        check(lib().fftwpp_amd_conv_geometry(self._h, N, m, q))
+        d = self.dim
+        return tuple(N[:d]), tuple(m[:d]), tuple(q[:d])
+
+    def __call__(self, *arrays):
+        """In place on max(A,B) device tensors: inputs arrays[:A], the
+        normalised products written to arrays[:B], which are returned."""
+        n = max(self.A, self.B)
+        if len(arrays) != n:
+            raise ValueError(f"expected {n} tensors")
+        for t in arrays:
+            if not t.is_cuda or t.dtype != self.dtype or tuple(t.shape) != self.shape \
+                    or not t.is_contiguous():
+                raise ValueError(f"expected contiguous {self.dtype} GPU tensors of shape {self.shape}")
+        ptrs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in arrays])
+        check(lib().fftwpp_amd_conv_execute_general(self._h, ptrs, _stream()))
+        return arrays[:self.B]

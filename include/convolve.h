@@ -14,8 +14,12 @@
  *   - errors throw fftwpp::Error (carrying a status code) instead of exit();
  *   - the parameter optimiser picks (m, D, I) with the GPU heuristic/autotune
  *     cache instead of timing CPU convolutions (convolve.cc:177-470);
- *   - multipliers are the four library multipliers, run on the device; a
- *     user-written host multiplier is rejected with FFTWPP_AMD_ERR_UNSUPPORTED;
+ *   - the four library multipliers run fused on the device; a user-written
+ *     multiplier is a DEVICE multiplier: it is called once per convolution
+ *     with device arrays F[a] holding the whole padded transforms (not once
+ *     per residue block), reads their layout from indices->layout and
+ *     enqueues its work on indices->stream (A inputs, B outputs, complex
+ *     and centred dimensions of contiguous data);
  *   - caller-provided F/W/V work arrays are accepted and not used (the device
  *     keeps its own work buffers).
  */
@@ -24,6 +28,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -71,8 +76,13 @@ public:
   size_t size, maxsize;
   size_t r;
   size_t offset;
+  // Set for a user multiplier, which runs on the device: the layout of the
+  // padded arrays it receives and the stream (hipStream_t) for its work.
+  const fftwpp_amd_layout *layout;
+  void *stream;
 
-  Indices() : fft(NULL), index(NULL), size(0), maxsize(0), r(0), offset(0) {}
+  Indices() : fft(NULL), index(NULL), size(0), maxsize(0), r(0), offset(0),
+              layout(NULL), stream(NULL) {}
   void copy(Indices *indices, size_t size0);
   ~Indices();
 };
@@ -309,6 +319,18 @@ public:
   void *get(size_t slot, size_t bytes);
   ~DeviceWork();
 };
+
+// A convolution with a user multiplier on contiguous complex device data of
+// nd dimensions (dims outermost first, complex or centred): every input
+// f[a], a < A, is expanded to all residues of every dimension, `mult` is
+// called once on the padded arrays (layout as fftwpp_amd_layout describes),
+// and the B products are transformed back into f[0..B-1], multiplied by
+// `scale`.  f holds max(A,B) arrays.
+typedef std::function<void(void **F, size_t n, const fftwpp_amd_layout &layout)>
+  DeviceMultiplier;
+void generalConvolution(fftBase *const *dims, size_t nd, size_t A, size_t B,
+                        DeviceWork &work, int dtype, void *const *f,
+                        double scale, void *stream, const DeviceMultiplier &mult);
 
 class Convolution : public ThreadBase {
 public:

@@ -249,6 +249,56 @@ int fftwpp_amd_conv_geometry(const fftwpp_amd_conv *h, size_t *N,
                              size_t *m, size_t *q);
 
 /* ------------------------------------------------------------------------
+ * User multipliers: A inputs, B outputs and a DEVICE multiplier, the
+ * multiplier/Indices contract of convolve.h:48-82 (Convolution::operate,
+ * convolve.h:1120-1135; padded index Indices::fft->index(r,i),
+ * convolve.h:297-326) on the GPU.
+ *
+ * The reference calls a host multiplier once per residue block.  Here every
+ * dimension's q residues are expanded and the multiplier is called ONCE per
+ * convolution on the whole padded arrays, which are device arrays:
+ *   F[a], a < max(A,B), holds n = N[0] ... N[dims-1] complex words (complex
+ *   double for FFTWPP_AMD_F64, complex float for FFTWPP_AMD_F32).  F[0..A-1]
+ *   hold the transformed inputs; the multiplier leaves the B products in
+ *   F[0..B-1] (pointwise products on the padded grid).
+ *   The word at offset sum_d j[d]*stride[d] (j[d] < N[d] = m[d]*q[d]) is the
+ *   value at padded grid point x[d] = FFTWPP_AMD_GRID_POINT(m[d],q[d],j[d]),
+ *   i.e. sum_k f_k exp(2 pi i k x / N) per dimension, the index q s + r of
+ *   the reference's residue r at sub-FFT position s.  For centred data k
+ *   runs over -floor(L/2) ... ceil(L/2)-1.
+ * The multiplier enqueues its work on `stream` and returns; it must not free
+ * F or synchronise other streams.  `user` is passed through. */
+typedef struct fftwpp_amd_layout {
+  int dims;                  /* 1..3 */
+  int dtype;                 /* FFTWPP_AMD_F64 or FFTWPP_AMD_F32 */
+  size_t A, B;               /* inputs, outputs */
+  size_t N[3], m[3], q[3];   /* padded length, sub-FFT size, residues */
+  size_t stride[3];          /* word stride of position j[d] */
+} fftwpp_amd_layout;
+
+#define FFTWPP_AMD_GRID_POINT(m, q, j) ((q) * ((j) % (m)) + (j) / (m))
+
+typedef void (*fftwpp_amd_multiplier)(void **F, size_t n,
+                                      const fftwpp_amd_layout *layout,
+                                      void *stream, void *user);
+
+/* A dealiased convolution of A inputs into B outputs with a user device
+ * multiplier (Application(A,B,mult) with fftPad or fftPadCentered in every
+ * dimension, convolve.h:84-121,510-699).  Data are L[0] x ... contiguous
+ * complex arrays; centred = 1 makes every dimension centred (default M the
+ * 3/2 rule, else 2L-1).  1 <= A, B <= 8. */
+int fftwpp_amd_conv_create_general(fftwpp_amd_conv **h, int dim, int centred,
+                                   const size_t *L, const size_t *M,
+                                   const size_t *m, int dtype, size_t A,
+                                   size_t B, fftwpp_amd_multiplier mult,
+                                   void *user);
+
+/* Device pointers f[0..max(A,B)-1]: inputs f[0..A-1], normalised results
+ * written to f[0..B-1] (Convolution::convolve, convolve.cc:7577-7599). */
+int fftwpp_amd_conv_execute_general(fftwpp_amd_conv *h, void *const *f,
+                                    void *stream);
+
+/* ------------------------------------------------------------------------
  * Batches of independent 1D complex convolutions of uneven lengths.  No
  * reference counterpart: FFTW++ convolves one set of arrays per Convolution
  * object and call (convolve.h:982-1166); a batch of n items replaces n such

@@ -272,6 +272,36 @@ def explicit_conv(f, g, N, centered=False):
     return H[ix]
 
 
+def explicit_general(arrays, N, op, centered=False):
+    """A convolution with a user multiplier by explicit padding (Convolution
+    with A inputs and B outputs, the multiplier applied pointwise to the padded
+    transforms, convolve.h:1120-1135).  Each input is zero-padded (optionally
+    centred) to N and transformed with sign +1, F[x] = sum_k f_k
+    exp(2 pi i k x / N) per dimension (the reference's forward sign,
+    convolve.h:297-326 gives the padded index x of each transformed value);
+    ``op(Fs, x)`` maps the list of A transforms and the tuple of grid-point
+    arrays x[d] (broadcastable against F) to the list of B products, each
+    transformed back with sign -1, divided by prod(N) and truncated to the
+    input shape."""
+    arrays = [np.asarray(a, dtype=np.complex128) for a in arrays]
+    d = arrays[0].ndim
+    Ns = (N,) * d if np.isscalar(N) else tuple(N)
+    cs = (centered,) * d if isinstance(centered, bool) else tuple(centered)
+    idx = []
+    for ax in range(d):
+        L = arrays[0].shape[ax]
+        idx.append(np.mod(np.arange(L) + _offsets(L, cs[ax]), Ns[ax]))
+    ix = np.ix_(*idx)
+    total = float(np.prod(Ns))
+    Fs = []
+    for a in arrays:
+        P = np.zeros(Ns, dtype=np.complex128)
+        P[ix] = a
+        Fs.append(np.fft.ifftn(P) * total)
+    x = np.ix_(*[np.arange(n) for n in Ns])
+    return [np.fft.fftn(G)[ix] / total for G in op(Fs, x)]
+
+
 def expand_hermitian(f, compact_axes_even):
     """Return (full, kranges) for a centred Hermitian array whose last axis
     stores the non-negative modes.  Leading axes are centred (index i <-> k =

@@ -234,6 +234,65 @@ def test_explicit_independent_of_padding(oracle):
         assert oracle.rel_l2(oracle.explicit_conv(f, g, N), a) < TOL
 
 
+def _cyclic_brute(f, g, N):
+    # h[n] = sum over i + j = n (mod N) of f_i g_j, by definition
+    h = np.zeros(N, dtype=complex)
+    for i, fi in enumerate(f):
+        for j, gj in enumerate(g):
+            h[(i + j) % N] += fi * gj
+    return h
+
+
+def test_explicit_general_products(oracle):
+    # user multipliers by explicit padding: binary, ternary, square and
+    # two-output products against direct sums and definitions
+    rng = np.random.default_rng(31)
+    L = 23
+    f, g, h = rnd(rng, L), rnd(rng, L), rnd(rng, L)
+    mul2 = lambda F, x: [F[0] * F[1]]
+    assert oracle.rel_l2(oracle.explicit_general([f, g], 2 * L, mul2)[0],
+                         oracle.direct_conv(f, g)) < TOL
+    ternary = lambda F, x: [F[0] * F[1] * F[2]]
+    want3 = oracle.direct_conv(oracle.direct_conv(f, g), h)
+    for N in [3 * L - 2, 80]:
+        assert oracle.rel_l2(oracle.explicit_general([f, g, h], N, ternary)[0], want3) < TOL
+    square = lambda F, x: [F[0] * F[0]]
+    assert oracle.rel_l2(oracle.explicit_general([f], 50, square)[0],
+                         oracle.direct_conv(f, f)) < TOL
+    pair = lambda F, x: [F[0] * F[1], F[0] - F[1]]
+    c, d = oracle.explicit_general([f, g], 48, pair)
+    assert oracle.rel_l2(c, oracle.direct_conv(f, g)) < TOL
+    assert oracle.rel_l2(d, f - g) < TOL
+    # centred data against the centred direct sums
+    assert oracle.rel_l2(oracle.explicit_general([f, g], 36, mul2, centered=True)[0],
+                         oracle.direct_convC(f, g)) < TOL
+
+
+def test_explicit_general_grid_point_sign(oracle):
+    # a weight exp(2 pi i x/N) at padded grid point x shifts the circular
+    # convolution up by one under the forward sign +1: out_k = h_{k-1 mod N}
+    rng = np.random.default_rng(32)
+    L, N = 9, 20
+    f, g = rnd(rng, L), rnd(rng, L)
+    h = _cyclic_brute(f, g, N)
+    up = lambda F, x: [F[0] * F[1] * np.exp(2j * np.pi * x[0] / N)]
+    got = oracle.explicit_general([f, g], N, up)[0]
+    assert oracle.rel_l2(got, h[(np.arange(L) - 1) % N]) < TOL
+    # 2D: a weight along the last axis shifts that axis only
+    Lx, Ly, Nx, Ny = 4, 5, 8, 11
+    f2, g2 = rnd(rng, Lx, Ly), rnd(rng, Lx, Ly)
+    H = np.zeros((Nx, Ny), dtype=complex)
+    for i in range(Lx):
+        for j in range(Ly):
+            for a in range(Lx):
+                for b in range(Ly):
+                    H[(i + a) % Nx, (j + b) % Ny] += f2[i, j] * g2[a, b]
+    up2 = lambda F, x: [F[0] * F[1] * np.exp(2j * np.pi * x[1] / Ny)]
+    got2 = oracle.explicit_general([f2, g2], (Nx, Ny), up2)[0]
+    want2 = H[np.ix_(np.arange(Lx), (np.arange(Ly) - 1) % Ny)]
+    assert oracle.rel_l2(got2, want2) < TOL
+
+
 def test_pad_forward_definition(oracle):
     # residue r of the padded transform: F_{qs+r} = sum_k f_k e^{2pi i k(qs+r)/N}
     rng = np.random.default_rng(9)
