@@ -58,6 +58,10 @@ SIGNATURES = {
     "fftwpp_amd_conv_execute_timed": (I, [P, P, P, P, P, ctypes.POINTER(ctypes.c_float), I,
                                           ctypes.POINTER(I)]),
     "fftwpp_amd_conv_stage_name": (ctypes.c_char_p, [P, I]),
+    "fftwpp_amd_fft_create": (I, [ctypes.POINTER(P), I, I, ZP, I, I, Z, Z, Z, Z, Z]),
+    "fftwpp_amd_fft_destroy": (I, [P]),
+    "fftwpp_amd_fft_execute": (I, [P, P, P, D, P]),
+    "fftwpp_amd_fft_normalization": (Z, [P]),
     "fftwpp_amd_conv_create_general": (I, [ctypes.POINTER(P), I, I, ZP, ZP, ZP, I, Z, Z, P, P]),
     "fftwpp_amd_conv_execute_general": (I, [P, ctypes.POINTER(P), P]),
     # batches of 1D convolutions

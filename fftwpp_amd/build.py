@@ -22,7 +22,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I", INCLUDE]
 
 HIP_SOURCES = ["kernels.hip", "batch_plan.hip"]
-HOST_SOURCES = ["host/convolve.cc", "host/capi.cc"]
+HOST_SOURCES = ["host/convolve.cc", "host/capi.cc", "host/fft.cc"]
 
 
 def _run(cmd, verbose):

@@ -1,0 +1,224 @@
+// fft.cc -- standalone FFTs (the transform classes of fftw++.h:556-1444) on
+// the residue transforms of the kernel layer.
+//
+// A DFT of length n = m q with sign +1 is every residue of the padded
+// transform of a dimension with L = N = n: residue r, position s holds
+// sum_k f_k zeta_n^{k(q s + r)}, so one forward residue pass per r, written
+// with output stride q and offset r, leaves the transform in natural order.
+// Sign -1 is the backward transform of the same dimension read the same
+// way: residue r takes the inputs at indices q s + r, and the passes
+// accumulate into the output.  m is the largest divisor of n that the
+// sub-FFT layer takes (at most 4096), so powers of two up to 4096 are one
+// register pass and up to 8192 two; other lengths cost q passes.
+#include "fftwpp_amd.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+extern "C" void fftwpp_amd_internal_set_error(const char *msg);
+extern "C" int fftwpp_amd_internal_fft_map(int dtype, int op, const void *x, size_t xs,
+                                           size_t xd, void *y, size_t ys, size_t yd,
+                                           size_t n, size_t nb, double scale, void *stream);
+
+struct fftwpp_amd_fft {
+  int kind, dims, sign, dtype;
+  size_t n[3], q[3];
+  size_t howmany, istride, ostride, idist, odist;
+  fftwpp_amd_dim *d[3];
+  void *tmp[2];
+  size_t tmpBytes;
+};
+
+namespace {
+
+const size_t maxSub=4096;     // largest sub-FFT size m
+const size_t maxPasses=4096;  // largest residue count q = n/m
+const size_t maxBatch=65535;  // grid rows per launch
+
+int error(int rc, const char *msg)
+{
+  fftwpp_amd_internal_set_error(msg);
+  return rc;
+}
+
+size_t subsize(size_t n)
+{
+  if(n <= maxSub) return n;
+  for(size_t m=maxSub; m > 1; --m)
+    if(n % m == 0) return m;
+  return 1;
+}
+
+size_t elt(int dtype) { return dtype == FFTWPP_AMD_F32 ? 8 : 16; }
+
+// One axis: element (b, j, c) of x at b*xb + j*xS + c (j < n, c < C,
+// b < nb), of y at b*yb + j*yS + c.
+int axis(const fftwpp_amd_fft *h, int a, int sign, const char *x, size_t xS,
+         size_t C, size_t nb, size_t xb, char *y, size_t yS, size_t yb,
+         void *stream)
+{
+  const size_t q=h->q[a], E=elt(h->dtype);
+  for(size_t b0=0; b0 < nb; b0 += maxBatch) {
+    const size_t cnt=std::min(maxBatch,nb-b0);
+    const char *xx=x+b0*xb*E;
+    char *yy=y+b0*yb*E;
+    for(size_t r=0; r < q; ++r) {
+      int rc=sign > 0 ?
+        fftwpp_amd_forward_many(h->d[a],h->dtype,xx,xS,C,cnt,xb,yy+r*yS*E,q*yS,yb,r,1,stream) :
+        fftwpp_amd_backward_many(h->d[a],h->dtype,xx+r*xS*E,q*xS,xb,yy,yS,C,cnt,yb,r,1,
+                                 r > 0,1.0,stream);
+      if(rc) return rc;
+    }
+  }
+  return FFTWPP_AMD_OK;
+}
+
+int temporaries(fftwpp_amd_fft *h, size_t words)
+{
+  const size_t bytes=words*elt(h->dtype);
+  if(bytes <= h->tmpBytes) return FFTWPP_AMD_OK;
+  for(void *&p : h->tmp) {
+    if(p) fftwpp_amd_free(p);
+    p=NULL;
+  }
+  h->tmpBytes=0;
+  for(void *&p : h->tmp) {
+    int rc=fftwpp_amd_malloc(&p,bytes);
+    if(rc) return rc;
+  }
+  h->tmpBytes=bytes;
+  return FFTWPP_AMD_OK;
+}
+
+size_t total(const fftwpp_amd_fft *h)
+{
+  size_t t=1;
+  for(int a=0; a < h->dims; ++a) t *= h->n[a];
+  return t;
+}
+
+} // namespace
+
+extern "C" {
+
+int fftwpp_amd_fft_create(fftwpp_amd_fft **out, int kind, int dims, const size_t *n,
+                          int sign, int dtype, size_t howmany, size_t istride,
+                          size_t ostride, size_t idist, size_t odist)
+{
+  if(!out || !n) return error(FFTWPP_AMD_ERR_ARG,"fftwpp_amd_fft_create: null argument");
+  *out=NULL;
+  if(kind < FFTWPP_AMD_FFT_C2C || kind > FFTWPP_AMD_FFT_C2R || dims < 1 || dims > 3 ||
+     (dtype != FFTWPP_AMD_F64 && dtype != FFTWPP_AMD_F32) || (sign != 1 && sign != -1))
+    return error(FFTWPP_AMD_ERR_ARG,"fftwpp_amd_fft_create: bad kind, dims, sign or dtype");
+  if(kind == FFTWPP_AMD_FFT_R2C) sign=-1;
+  if(kind == FFTWPP_AMD_FFT_C2R) sign=1;
+  if(dims > 1 && (kind != FFTWPP_AMD_FFT_C2C || howmany > 1))
+    return error(FFTWPP_AMD_ERR_UNSUPPORTED,"fftwpp_amd_fft_create: multidimensional transforms are single complex transforms");
+  if(howmany == 0) howmany=1;
+  for(int a=0; a < dims; ++a)
+    if(n[a] == 0) return error(FFTWPP_AMD_ERR_ARG,"fftwpp_amd_fft_create: zero length");
+  const size_t hw=kind == FFTWPP_AMD_FFT_C2C ? n[0] : n[0]/2+1;
+  const size_t inlen=kind == FFTWPP_AMD_FFT_C2R ? hw : n[0];
+  const size_t outlen=kind == FFTWPP_AMD_FFT_R2C ? hw : n[0];
+  if(dims == 1) {
+    if(istride == 0) istride=1;
+    if(ostride == 0) ostride=1;
+    if(idist == 0) idist=inlen;
+    if(odist == 0) odist=outlen;
+  } else {
+    istride=ostride=1;
+    idist=odist=0;
+  }
+  fftwpp_amd_fft *h=new(std::nothrow) fftwpp_amd_fft();
+  if(!h) return error(FFTWPP_AMD_ERR_HIP,"fftwpp_amd_fft_create: out of memory");
+  h->kind=kind; h->dims=dims; h->sign=sign; h->dtype=dtype;
+  h->howmany=howmany; h->istride=istride; h->ostride=ostride; h->idist=idist; h->odist=odist;
+  for(int a=0; a < dims; ++a) {
+    h->n[a]=n[a];
+    const size_t m=subsize(n[a]);
+    h->q[a]=n[a]/m;
+    if(h->q[a] > maxPasses) {
+      fftwpp_amd_fft_destroy(h);
+      return error(FFTWPP_AMD_ERR_UNSUPPORTED,"fftwpp_amd_fft_create: length without a divisor in [n/4096, 4096]");
+    }
+    int rc=fftwpp_amd_dim_create(&h->d[a],n[a],m,h->q[a],FFTWPP_AMD_KIND_COMPLEX);
+    if(rc) { fftwpp_amd_fft_destroy(h); return rc; }
+  }
+  *out=h;
+  return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_fft_destroy(fftwpp_amd_fft *h)
+{
+  if(!h) return FFTWPP_AMD_OK;
+  for(int a=0; a < 3; ++a)
+    if(h->d[a]) fftwpp_amd_dim_destroy(h->d[a]);
+  for(void *p : h->tmp)
+    if(p) fftwpp_amd_free(p);
+  delete h;
+  return FFTWPP_AMD_OK;
+}
+
+size_t fftwpp_amd_fft_normalization(const fftwpp_amd_fft *h)
+{
+  return h ? total(h) : 0;
+}
+
+int fftwpp_amd_fft_execute(fftwpp_amd_fft *h, const void *in, void *out, double scale,
+                           void *stream)
+{
+  if(!h || !in || !out) return error(FFTWPP_AMD_ERR_ARG,"fftwpp_amd_fft_execute: null argument");
+  if(!fftwpp_amd_is_device_pointer(in) || !fftwpp_amd_is_device_pointer(out))
+    return error(FFTWPP_AMD_ERR_ARG,"fftwpp_amd_fft_execute: in and out must be device pointers");
+  const int dt=h->dtype;
+  const size_t n=h->n[0], H=h->howmany;
+  const char *x=(const char *) in;
+  char *y=(char *) out;
+  int rc;
+  if(h->kind != FFTWPP_AMD_FFT_C2C) {
+    // through complex sequences of length n in two temporaries
+    if((rc=temporaries(h,n*H))) return rc;
+    char *t0=(char *) h->tmp[0], *t1=(char *) h->tmp[1];
+    const bool r2c=h->kind == FFTWPP_AMD_FFT_R2C;
+    if((rc=fftwpp_amd_internal_fft_map(dt,r2c ? 1 : 3,x,h->istride,h->idist,t0,1,n,n,H,1.0,stream)))
+      return rc;
+    if((rc=axis(h,0,h->sign,t0,1,1,H,n,t1,1,n,stream))) return rc;
+    return fftwpp_amd_internal_fft_map(dt,r2c ? 0 : 2,t1,1,n,y,h->ostride,h->odist,
+                                       r2c ? n/2+1 : n,H,scale,stream);
+  }
+  if(h->dims == 1) {
+    const bool alias=x == y;
+    if(alias) {
+      if((rc=temporaries(h,n*H))) return rc;
+      if((rc=fftwpp_amd_internal_fft_map(dt,0,x,h->istride,h->idist,h->tmp[0],1,n,n,H,1.0,stream)))
+        return rc;
+      rc=axis(h,0,h->sign,(const char *) h->tmp[0],1,1,H,n,y,h->ostride,h->odist,stream);
+    } else if(h->idist == 1 && h->odist == 1 && h->istride >= H && h->ostride >= H)
+      rc=axis(h,0,h->sign,x,h->istride,H,1,0,y,h->ostride,0,stream);   // interleaved columns
+    else
+      rc=axis(h,0,h->sign,x,h->istride,1,H,h->idist,y,h->ostride,h->odist,stream);
+    if(rc) return rc;
+    if(scale != 1.0)
+      rc=fftwpp_amd_internal_fft_map(dt,0,y,h->ostride,h->odist,y,h->ostride,h->odist,n,H,scale,stream);
+    return rc;
+  }
+  const size_t all=total(h);
+  if((rc=temporaries(h,all))) return rc;
+  const char *cur=x;
+  for(int a=0; a < h->dims; ++a) {
+    size_t outer=1, inner=1;
+    for(int e=0; e < a; ++e) outer *= h->n[e];
+    for(int e=a+1; e < h->dims; ++e) inner *= h->n[e];
+    char *dst=a == h->dims-1 ? y : (char *) h->tmp[a % 2];
+    if((rc=axis(h,a,h->sign,cur,inner,inner,outer,h->n[a]*inner,dst,inner,h->n[a]*inner,stream)))
+      return rc;
+    cur=dst;
+  }
+  if(scale != 1.0)
+    rc=fftwpp_amd_internal_fft_map(dt,0,y,1,all,y,1,all,all,1,scale,stream);
+  return rc;
+}
+
+} // extern "C"

@@ -328,6 +328,35 @@ __global__ void real_unpack_kernel(const cplx<T> *z, T *out, int64_t n, T s)
   if(i < n) out[i]=s*z[i].y;
 }
 
+// Element maps of the standalone FFTs, y[b yd + j ys] = s op(x[b xd + j xs])
+// for j < n, b < nb, strides in elements of each side's type.  op 0: complex
+// copy; 1: real to complex (imaginary part 0); 2: complex to its real part;
+// 3: the n-point spectrum of a Hermitian sequence from its n/2+1 stored modes
+// (y_j = conj x_{n-j} for j > n/2).
+template<typename T>
+__global__ void fft_map_kernel(int op, const void *x, int64_t xs, int64_t xd,
+                               void *y, int64_t ys, int64_t yd, int64_t n,
+                               int64_t nb, T s)
+{
+  int64_t i=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  if(i >= n*nb) return;
+  int64_t b=i/n, j=i-b*n;
+  if(op == 1) {
+    T v=reinterpret_cast<const T *>(x)[b*xd+j*xs];
+    reinterpret_cast<cplx<T> *>(y)[b*yd+j*ys]=mk(s*v,(T) 0);
+    return;
+  }
+  const cplx<T> *xc=reinterpret_cast<const cplx<T> *>(x)+b*xd;
+  if(op == 2) {
+    reinterpret_cast<T *>(y)[b*yd+j*ys]=s*xc[j*xs].x;
+    return;
+  }
+  cplx<T> v;
+  if(op == 3 && 2*j > n) v=conjg(xc[(n-j)*xs]);
+  else v=xc[j*xs];
+  reinterpret_cast<cplx<T> *>(y)[b*yd+j*ys]=s*v;
+}
+
 template<typename T>
 __global__ void scale_kernel(cplx<T> *f, int64_t n, T s)
 {
@@ -1253,6 +1282,29 @@ int fftwpp_amd_real_unpack(int dtype, const void *z, void *out, size_t n,
     hipLaunchKernelGGL(real_unpack_kernel<double>,dim3(nb),dim3(256),0,st,
                        (const cplx<double> *) z,(double *) out,(int64_t) n,scale);
   return launch_check("real_unpack_kernel");
+}
+
+// The element maps of fft_map_kernel (standalone FFTs; internal to the
+// library, not declared in the public header).
+int fftwpp_amd_internal_fft_map(int dtype, int op, const void *x, size_t xs,
+                                size_t xd, void *y, size_t ys, size_t yd,
+                                size_t n, size_t nb, double scale, void *stream)
+{
+  if(n == 0 || nb == 0) return FFTWPP_AMD_OK;
+  if(!x || !y || op < 0 || op > 3) return fail(FFTWPP_AMD_ERR_ARG,"bad element map");
+  const size_t total=n*nb;
+  if((total+255)/256 > 0xffffffffu) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"element map too large");
+  const unsigned blocks=(unsigned) ((total+255)/256);
+  hipStream_t st=(hipStream_t) stream;
+  if(dtype == FFTWPP_AMD_F32)
+    hipLaunchKernelGGL(fft_map_kernel<float>,dim3(blocks),dim3(256),0,st,op,x,(int64_t) xs,
+                       (int64_t) xd,y,(int64_t) ys,(int64_t) yd,(int64_t) n,(int64_t) nb,
+                       (float) scale);
+  else
+    hipLaunchKernelGGL(fft_map_kernel<double>,dim3(blocks),dim3(256),0,st,op,x,(int64_t) xs,
+                       (int64_t) xd,y,(int64_t) ys,(int64_t) yd,(int64_t) n,(int64_t) nb,
+                       scale);
+  return launch_check("fft_map_kernel");
 }
 
 int fftwpp_amd_multiply(int dtype, int mult, void *F0, const void *F1,

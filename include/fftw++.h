@@ -1,0 +1,183 @@
+/* fftw++.h -- the transform classes of FFTW++'s fftw++.h (fftw++.h:556-1444:
+ * fft1d, mfft1d, fft2d, fft3d, rcfft1d, crfft1d, mrcfft1d, mcrfft1d) on the
+ * MI355X, over the standalone FFT entry points of fftwpp_amd.h.
+ *
+ * Same constructors and fft / fftNormalized methods; the transforms run on
+ * the device.  Arrays may be device pointers (the work is
+ * enqueued on the default stream and waited for) or host pointers (staged
+ * through device buffers).  A NULL output means in place.  Differences:
+ * the `threads` arguments and the in/out arrays passed to constructors are
+ * accepted and unused (they chose FFTW plans); multidimensional transforms
+ * are complex (rcfft2d/3d and the Transpose class are not provided).
+ */
+#ifndef FFTWPP_AMD_FFTWPP_H
+#define FFTWPP_AMD_FFTWPP_H
+
+#include <algorithm>
+
+#include "convolve.h"
+
+namespace fftwpp {
+
+// Shared by the transform classes (the role of class fftw, fftw++.h:119-434).
+class fftwPlan {
+protected:
+  fftwpp_amd_fft *h=NULL;
+  size_t inWords=0, outWords=0;   // array extents in elements
+  size_t inSize=0, outSize=0;     // element bytes (sizeof(Complex) or sizeof(double))
+
+  void create(int kind, int dims, const size_t *n, int sign, size_t howmany,
+              size_t istride, size_t ostride, size_t idist, size_t odist) {
+    int rc=fftwpp_amd_fft_create(&h,kind,dims,n,sign,FFTWPP_AMD_F64,howmany,istride,
+                                 ostride,idist,odist);
+    if(rc) throw Error(rc,fftwpp_amd_last_error());
+    inSize=kind == FFTWPP_AMD_FFT_R2C ? sizeof(double) : sizeof(Complex);
+    outSize=kind == FFTWPP_AMD_FFT_C2R ? sizeof(double) : sizeof(Complex);
+    const size_t len=n[0], half=n[0]/2+1;
+    size_t inl=kind == FFTWPP_AMD_FFT_C2R ? half : len;
+    size_t outl=kind == FFTWPP_AMD_FFT_R2C ? half : len;
+    if(dims > 1) {
+      inWords=outWords=fftwpp_amd_fft_normalization(h);
+      return;
+    }
+    if(istride == 0) istride=1;
+    if(ostride == 0) ostride=1;
+    if(idist == 0) idist=inl;
+    if(odist == 0) odist=outl;
+    if(howmany == 0) howmany=1;
+    inWords=(howmany-1)*idist+(inl-1)*istride+1;
+    outWords=(howmany-1)*odist+(outl-1)*ostride+1;
+  }
+
+  void run(const void *in, void *out, double scale) {
+    if(!out) out=(void *) in;
+    const bool dev=fftwpp_amd_is_device_pointer(in) && fftwpp_amd_is_device_pointer(out);
+    int rc;
+    if(dev) {
+      rc=fftwpp_amd_fft_execute(h,in,out,scale,NULL);
+      if(!rc) rc=fftwpp_amd_synchronize(NULL);
+    } else {
+      // host arrays: one device buffer per distinct array
+      const size_t ib=inWords*inSize, ob=outWords*outSize;
+      const bool inplace=in == out;
+      void *din=NULL, *dout=NULL;
+      rc=fftwpp_amd_malloc(&din,inplace ? std::max(ib,ob) : ib);
+      if(!rc && !inplace) rc=fftwpp_amd_malloc(&dout,ob);
+      if(inplace) dout=din;
+      if(!rc) rc=fftwpp_amd_memcpy(din,in,ib,NULL);
+      if(!rc) rc=fftwpp_amd_fft_execute(h,din,dout,scale,NULL);
+      if(!rc) rc=fftwpp_amd_memcpy(out,dout,ob,NULL);
+      if(!rc) rc=fftwpp_amd_synchronize(NULL);
+      if(din) fftwpp_amd_free(din);
+      if(dout && !inplace) fftwpp_amd_free(dout);
+    }
+    if(rc) throw Error(rc,fftwpp_amd_last_error());
+  }
+
+public:
+  fftwPlan() {}
+  fftwPlan(const fftwPlan &)=delete;
+  fftwPlan &operator=(const fftwPlan &)=delete;
+  virtual ~fftwPlan() { fftwpp_amd_fft_destroy(h); }
+
+  // The product of the transform lengths (what fftNormalized divides by).
+  size_t normalization() const { return fftwpp_amd_fft_normalization(h); }
+  double Normalization() const { return 1.0/(double) normalization(); }
+
+  void fft(Complex *in, Complex *out=NULL) { run(in,out,1.0); }
+  void fft(double *in, Complex *out=NULL) { run(in,out,1.0); }
+  void fft(Complex *in, double *out) { run(in,out,1.0); }
+  void fftNormalized(Complex *in, Complex *out=NULL) { run(in,out,Normalization()); }
+  void fftNormalized(double *in, Complex *out=NULL) { run(in,out,Normalization()); }
+  void fftNormalized(Complex *in, double *out) { run(in,out,Normalization()); }
+};
+
+// fftw++.h:556-574: one complex transform of length nx.
+class fft1d : public fftwPlan {
+public:
+  fft1d(size_t nx, int sign, Complex * =NULL, Complex * =NULL,
+        size_t =fftw::maxthreads) {
+    create(FFTWPP_AMD_FFT_C2C,1,&nx,sign,1,1,1,0,0);
+  }
+};
+
+// fftw++.h:827-896: M complex transforms, element j of transform t at
+// j*stride + t*dist (defaults: contiguous transforms, dist = nx).
+class mfft1d : public fftwPlan {
+public:
+  mfft1d(size_t nx, int sign, size_t M=1, Complex * =NULL, Complex * =NULL,
+         size_t =fftw::maxthreads) {
+    create(FFTWPP_AMD_FFT_C2C,1,&nx,sign,M,1,1,nx,nx);
+  }
+  mfft1d(size_t nx, int sign, size_t M, size_t stride, size_t dist=0,
+         Complex * =NULL, Complex * =NULL, size_t =fftw::maxthreads) {
+    if(dist == 0) dist=nx;
+    create(FFTWPP_AMD_FFT_C2C,1,&nx,sign,M,stride,stride,dist,dist);
+  }
+};
+
+// fftw++.h:1199-1247, 1390-1444: complex 2D and 3D transforms of
+// contiguous row-major arrays.
+class fft2d : public fftwPlan {
+public:
+  fft2d(size_t nx, size_t ny, int sign, Complex * =NULL, Complex * =NULL,
+        size_t =fftw::maxthreads) {
+    const size_t n[2]={nx,ny};
+    create(FFTWPP_AMD_FFT_C2C,2,n,sign,1,1,1,0,0);
+  }
+};
+
+class fft3d : public fftwPlan {
+public:
+  fft3d(size_t nx, size_t ny, size_t nz, int sign, Complex * =NULL,
+        Complex * =NULL, size_t =fftw::maxthreads) {
+    const size_t n[3]={nx,ny,nz};
+    create(FFTWPP_AMD_FFT_C2C,3,n,sign,1,1,1,0,0);
+  }
+};
+
+// fftw++.h:917-975: real-to-complex (sign -1, nx/2+1 outputs) and
+// complex-to-real (sign +1) transforms.
+class rcfft1d : public fftwPlan {
+public:
+  rcfft1d(size_t nx, Complex * =NULL, size_t =fftw::maxthreads) {
+    create(FFTWPP_AMD_FFT_R2C,1,&nx,-1,1,1,1,0,0);
+  }
+  rcfft1d(size_t nx, double *, Complex * =NULL, size_t =fftw::maxthreads) {
+    create(FFTWPP_AMD_FFT_R2C,1,&nx,-1,1,1,1,0,0);
+  }
+};
+
+class crfft1d : public fftwPlan {
+public:
+  crfft1d(size_t nx, double * =NULL, size_t =fftw::maxthreads) {
+    create(FFTWPP_AMD_FFT_C2R,1,&nx,1,1,1,1,0,0);
+  }
+  crfft1d(size_t nx, Complex *, double * =NULL, size_t =fftw::maxthreads) {
+    create(FFTWPP_AMD_FFT_C2R,1,&nx,1,1,1,1,0,0);
+  }
+};
+
+// fftw++.h:1023-1072, 1122-1171: M real-to-complex / complex-to-real
+// transforms with input and output strides and distances.
+class mrcfft1d : public fftwPlan {
+public:
+  mrcfft1d(size_t nx, size_t M, size_t istride, size_t ostride, size_t idist,
+           size_t odist, double * =NULL, Complex * =NULL,
+           size_t =fftw::maxthreads) {
+    create(FFTWPP_AMD_FFT_R2C,1,&nx,-1,M,istride,ostride,idist,odist);
+  }
+};
+
+class mcrfft1d : public fftwPlan {
+public:
+  mcrfft1d(size_t nx, size_t M, size_t istride, size_t ostride, size_t idist,
+           size_t odist, Complex * =NULL, double * =NULL,
+           size_t =fftw::maxthreads) {
+    create(FFTWPP_AMD_FFT_C2R,1,&nx,1,M,istride,ostride,idist,odist);
+  }
+};
+
+} // namespace fftwpp
+
+#endif

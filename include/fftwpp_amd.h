@@ -337,6 +337,42 @@ int fftwpp_amd_batch_describe(const fftwpp_amd_batch *h, char *buf,
 int fftwpp_amd_batch_execute(fftwpp_amd_batch *h, void *out, const void *f,
                              const void *g, void *stream);
 
+/* ------------------------------------------------------------------------
+ * 3. Standalone FFTs: the transforms of the fft1d, mfft1d, fft2d, fft3d,
+ * rcfft1d, crfft1d, mrcfft1d and mcrfft1d classes (fftw++.h:556-1444) on
+ * device arrays, unnormalised unless `scale` says otherwise:
+ *   out_k = scale * sum_j in_j exp(sign 2 pi i j k / n)  (FFTW's sign).
+ * kind FFTWPP_AMD_FFT_C2C: any sign; dims 1..3 of n[0] x ... contiguous
+ *   words, or (dims = 1) `howmany` transforms with element j of transform t
+ *   at j*istride + t*idist (output: ostride, odist), as mfft1d
+ *   (fftw++.h:827-896).
+ * kind FFTWPP_AMD_FFT_R2C: sign -1, dims = 1: n real words in (istride,
+ *   idist in reals), n/2+1 complex words out (ostride, odist), as
+ *   rcfft1d/mrcfft1d (fftw++.h:917-955,1023-1072).
+ * kind FFTWPP_AMD_FFT_C2R: sign +1, dims = 1: n/2+1 complex words of a
+ *   Hermitian spectrum in, n real words out, as crfft1d/mcrfft1d
+ *   (fftw++.h:957-975,1122-1171).
+ * A length n = m q runs as q residue passes of m-point sub-FFTs (m the
+ * largest divisor of n the sub-FFT layer takes, at most 4096 here), writing
+ * natural order through strides: a length with such a divisor and small q
+ * is one or two passes; any length is supported. */
+#define FFTWPP_AMD_FFT_C2C 0
+#define FFTWPP_AMD_FFT_R2C 1
+#define FFTWPP_AMD_FFT_C2R 2
+
+typedef struct fftwpp_amd_fft fftwpp_amd_fft;
+
+int fftwpp_amd_fft_create(fftwpp_amd_fft **h, int kind, int dims,
+                          const size_t *n, int sign, int dtype,
+                          size_t howmany, size_t istride, size_t ostride,
+                          size_t idist, size_t odist);
+int fftwpp_amd_fft_destroy(fftwpp_amd_fft *h);
+/* Device pointers; in may equal out (in place). */
+int fftwpp_amd_fft_execute(fftwpp_amd_fft *h, const void *in, void *out,
+                           double scale, void *stream);
+/* n[0]*...*n[dims-1]: the factor fftNormalized divides by. */
+size_t fftwpp_amd_fft_normalization(const fftwpp_amd_fft *h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,5 +1,5 @@
 """Build tests/support/libtestmult.so: the user device multipliers of
-multipliers.hip, linked against the in-tree libfftwpp_amd.so (found next to
+multipliers.hip and the C++ FFT face of fftface.cc, linked against the in-tree libfftwpp_amd.so (found next to
 the package at run time through $ORIGIN)."""
 import os
 import subprocess
@@ -15,6 +15,7 @@ def build(verbose=False):
     from fftwpp_amd import build as b
     pkg = os.path.join(ROOT, "fftwpp_amd")
     cmd = [b.HIPCC] + b.HIP_FLAGS + ["-shared", os.path.join(HERE, "multipliers.hip"),
+                                     os.path.join(HERE, "fftface.cc"),
                                      "-L", pkg, "-lfftwpp_amd",
                                      "-Wl,-rpath,$ORIGIN/../../fftwpp_amd",
                                      "-o", LIB + ".tmp"]

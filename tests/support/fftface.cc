@@ -1,0 +1,53 @@
+// fftface.cc -- the C++ face of the standalone FFTs (include/fftw++.h) on
+// host arrays, as a user of FFTW++'s fftw++.h calls it (examples/example0.cc,
+// example0r.cc, example1.cc): exported for the tests through ctypes.
+#include <cstring>
+
+#include "fftw++.h"
+
+using namespace fftwpp;
+
+extern "C" {
+
+// cout = FFT_{-1}(cin); cback = normalised FFT_{+1}(cout) in place (= cin);
+// rout = rcfft1d(rin); rback = normalised crfft1d(rout) (= rin).
+// Returns 0, or 1 when the library threw.
+int testfft_cxx(size_t n, double *cin, double *cout, double *cback, double *rin,
+                double *rout, double *rback)
+{
+  try {
+    fft1d Forward(n,-1);
+    Forward.fft((Complex *) cin,(Complex *) cout);
+    memcpy(cback,cout,n*sizeof(Complex));
+    fft1d Backward(n,1);
+    Backward.fftNormalized((Complex *) cback);
+    rcfft1d R(n);
+    R.fft(rin,(Complex *) rout);
+    crfft1d C(n);
+    C.fftNormalized((Complex *) rout,rback);
+  } catch(const std::exception &) {
+    return 1;
+  }
+  return 0;
+}
+
+// out = fft2d / fft3d (sign -1) of in, or of out in place when in is NULL.
+int testfft_multi_cxx(int dims, const size_t *n, double *in, double *out)
+{
+  try {
+    if(dims == 2) {
+      fft2d F(n[0],n[1],-1);
+      if(in) F.fft((Complex *) in,(Complex *) out);
+      else F.fft((Complex *) out);
+    } else {
+      fft3d F(n[0],n[1],n[2],-1);
+      if(in) F.fft((Complex *) in,(Complex *) out);
+      else F.fft((Complex *) out);
+    }
+  } catch(const std::exception &) {
+    return 1;
+  }
+  return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,197 @@
+"""Standalone FFTs (the fft1d / mfft1d / fft2d / fft3d / rcfft1d / crfft1d /
+mrcfft1d / mcrfft1d transforms of fftw++.h:556-1444) on the GPU against
+numpy's pocketfft (a library routine).  FFTW's sign convention: sign -1 is
+numpy.fft.fft, sign +1 is n * numpy.fft.ifft; r2c is numpy.fft.rfft, c2r is
+n * numpy.fft.irfft.  The reference's own FFT tests only print
+(tests/fft1.cc, mfft1.cc), so the library routine is the check.  Bar:
+relative L2 error <= 1e-12 in FP64, <= 1e-5 in FP32 against FP64 numpy."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-5
+TESTLIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "support", "libtestmult.so")
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    d = np.linalg.norm((a - b).ravel())
+    nb = np.linalg.norm(b.ravel())
+    return d / nb if nb > 0 else d
+
+
+def rnd(rng, shape):
+    return rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)
+
+
+def dft(x, sign, axes=None):
+    if sign < 0:
+        return np.fft.fftn(x, axes=axes)
+    n = np.prod([x.shape[a] for a in (axes or range(x.ndim))])
+    return np.fft.ifftn(x, axes=axes) * n
+
+
+def run(gpu, plan, x, out_shape, out_dtype, inplace=False):
+    import torch
+    t = torch.tensor(x, device=gpu)
+    o = t if inplace else torch.empty(out_shape, dtype=out_dtype, device=gpu)
+    plan(t, o)
+    torch.cuda.synchronize()
+    return o.cpu().numpy()
+
+
+PRECISIONS = [("f64", TOL64), ("f32", TOL32)]
+
+
+def types(p):
+    import torch
+    return (torch.float64, torch.complex128, np.complex128, np.float64) if p == "f64" else \
+        (torch.float32, torch.complex64, np.complex64, np.float32)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 16, 64, 100, 256, 1000, 1024, 4093, 4096, 4097,
+                               5000, 8192, 12000, 65536, 1 << 20])
+def test_c2c_1d(gpu, n):
+    from fftwpp_amd.torch_fft import DeviceFFT
+    rng = np.random.default_rng(n)
+    x = rnd(rng, n)
+    for p, tol in PRECISIONS:
+        real, cplx, npc, npr = types(p)
+        for sign in (-1, 1):
+            plan = DeviceFFT(n, sign, dtype=real)
+            got = run(gpu, plan, x.astype(npc), (n,), cplx)
+            assert rel(got, dft(x, sign)) < tol, (n, p, sign)
+
+
+def test_c2c_unsupported_length(gpu):
+    from fftwpp_amd import FFTWPPError
+    from fftwpp_amd.torch_fft import DeviceFFT
+    with pytest.raises(FFTWPPError):
+        DeviceFFT(5003)        # prime beyond the largest sub-FFT: no divisor in [n/4096, 4096]
+
+
+@pytest.mark.parametrize("n,M,stride,dist", [(48, 5, 1, 48), (48, 5, 5, 1), (30, 3, 2, 70),
+                                             (1024, 64, 1, 1024), (256, 16, 16, 1)])
+def test_mfft1d_layouts(gpu, n, M, stride, dist):
+    import torch
+    from fftwpp_amd.torch_fft import DeviceFFT
+    rng = np.random.default_rng(n + M)
+    extent = (M - 1) * dist + (n - 1) * stride + 1
+    buf = rnd(rng, extent)
+    idx = np.arange(M)[:, None] * dist + np.arange(n)[None, :] * stride
+    for sign in (-1, 1):
+        plan = DeviceFFT(n, sign, dtype=torch.float64, howmany=M, istride=stride,
+                         ostride=stride, idist=dist, odist=dist)
+        t = torch.tensor(buf, device=gpu)
+        o = torch.zeros(extent, dtype=torch.complex128, device=gpu)
+        plan(t, o)
+        torch.cuda.synchronize()
+        got = o.cpu().numpy()[idx]
+        assert rel(got, dft(buf[idx], sign, axes=(1,))) < TOL64, sign
+
+
+@pytest.mark.parametrize("shape", [(64, 48), (256, 256), (7, 9, 11), (64, 64, 64), (8, 4096),
+                                   (300, 256, 5)])
+def test_c2c_multi(gpu, shape):
+    from fftwpp_amd.torch_fft import DeviceFFT
+    rng = np.random.default_rng(sum(shape))
+    x = rnd(rng, shape)
+    for p, tol in PRECISIONS:
+        real, cplx, npc, npr = types(p)
+        for sign in (-1, 1):
+            plan = DeviceFFT(shape, sign, dtype=real)
+            got = run(gpu, plan, x.astype(npc), shape, cplx)
+            assert rel(got, dft(x, sign)) < tol, (shape, p, sign)
+
+
+@pytest.mark.parametrize("shape", [(100,), (4096,), (5000,), (32, 20), (6, 7, 8)])
+def test_c2c_in_place_and_normalized(gpu, shape):
+    import torch
+    from fftwpp_amd.torch_fft import DeviceFFT
+    rng = np.random.default_rng(3 + sum(shape))
+    x = rnd(rng, shape)
+    fwd = DeviceFFT(shape, -1, dtype=torch.float64)
+    got = run(gpu, fwd, x, shape, torch.complex128, inplace=True)
+    assert rel(got, dft(x, -1)) < TOL64
+    t = torch.tensor(x, device=gpu)
+    bwd = DeviceFFT(shape, 1, dtype=torch.float64)
+    fwd(t, t)
+    bwd(t, t, scale=1.0 / bwd.normalization())
+    torch.cuda.synchronize()
+    assert rel(t.cpu().numpy(), x) < TOL64
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 9, 100, 1024, 4096, 10000, 16384])
+def test_r2c_c2r(gpu, n):
+    from fftwpp_amd.torch_fft import DeviceFFT
+    rng = np.random.default_rng(n + 11)
+    x = rng.uniform(-1, 1, n)
+    X = np.fft.rfft(x)
+    for p, tol in PRECISIONS:
+        real, cplx, npc, npr = types(p)
+        r2c = DeviceFFT(n, -1, kind="r2c", dtype=real)
+        got = run(gpu, r2c, x.astype(npr), (n // 2 + 1,), cplx)
+        assert rel(got, X) < tol, (n, p, "r2c")
+        c2r = DeviceFFT(n, 1, kind="c2r", dtype=real)
+        back = run(gpu, c2r, X.astype(npc), (n,), real)
+        assert rel(back, np.fft.irfft(X, n) * n) < tol, (n, p, "c2r")
+
+
+def test_mrcfft1d_mcrfft1d(gpu):
+    # M = 4 transforms of n = 30: reals with stride 2 and distance 70, spectra
+    # interleaved (stride 4, distance 1)
+    import torch
+    from fftwpp_amd.torch_fft import DeviceFFT
+    n, M = 30, 4
+    h = n // 2 + 1
+    rng = np.random.default_rng(5)
+    rbuf = rng.uniform(-1, 1, (M - 1) * 70 + (n - 1) * 2 + 1)
+    ridx = np.arange(M)[:, None] * 70 + np.arange(n)[None, :] * 2
+    cidx = np.arange(M)[:, None] + np.arange(h)[None, :] * M
+    r2c = DeviceFFT(n, -1, kind="r2c", dtype=torch.float64, howmany=M, istride=2, ostride=M,
+                    idist=70, odist=1)
+    cout = torch.zeros(M * h, dtype=torch.complex128, device=gpu)
+    r2c(torch.tensor(rbuf, device=gpu), cout)
+    torch.cuda.synchronize()
+    X = np.fft.rfft(rbuf[ridx], axis=1)
+    assert rel(cout.cpu().numpy()[cidx], X) < TOL64
+    c2r = DeviceFFT(n, 1, kind="c2r", dtype=torch.float64, howmany=M, istride=M, ostride=2,
+                    idist=1, odist=70)
+    rback = torch.zeros_like(torch.tensor(rbuf, device=gpu))
+    c2r(cout, rback, scale=1.0 / n)
+    torch.cuda.synchronize()
+    assert rel(rback.cpu().numpy()[ridx], rbuf[ridx]) < TOL64
+
+
+def test_cxx_face_host_arrays(gpu):
+    # fft1d / rcfft1d / crfft1d / fft2d / fft3d of include/fftw++.h on host arrays
+    lib = ctypes.CDLL(TESTLIB)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rng = np.random.default_rng(17)
+    for n in [60, 1024, 6000]:
+        cin = rnd(rng, n)
+        cout = np.zeros(n, complex)
+        cback = np.zeros(n, complex)
+        rin = rng.uniform(-1, 1, n)
+        rout = np.zeros(n // 2 + 1, complex)
+        rback = np.zeros(n)
+        args = [a.ctypes.data_as(dp) for a in (cin, cout, cback, rin, rout, rback)]
+        assert lib.testfft_cxx(ctypes.c_size_t(n), *args) == 0
+        assert rel(cout, np.fft.fft(cin)) < TOL64
+        assert rel(cback, cin) < TOL64
+        assert rel(rout, np.fft.rfft(rin)) < TOL64
+        assert rel(rback, rin) < TOL64
+    for shape in [(12, 20), (5, 6, 7)]:
+        x = rnd(rng, shape)
+        out = np.zeros(shape, complex)
+        ns = (ctypes.c_size_t * 3)(*shape)
+        assert lib.testfft_multi_cxx(len(shape), ns, x.ctypes.data_as(dp), out.ctypes.data_as(dp)) == 0
+        assert rel(out, np.fft.fftn(x)) < TOL64
+        y = x.copy()
+        assert lib.testfft_multi_cxx(len(shape), ns, None, y.ctypes.data_as(dp)) == 0
+        assert rel(y, np.fft.fftn(x)) < TOL64
