@@ -124,34 +124,37 @@ __device__ void stage(const cplx<T> *in, cplx<T> *out, int nb, int ld,
   }
 }
 
-// Stage for any other radix: O(R^2) butterfly straight from LDS.
+// Stage for any other radix: O(R^2) butterflies straight from LDS, one
+// output u of butterfly j per thread (a single large prime radix still
+// spreads over the block).  The twiddle of term t is w^{tk} e^{2pi i ut/R}
+// = zN[t E1 mod N] with E1 = k twstep + u N/R, stepped without division.
 template<typename T>
 __device__ void stage_generic(const cplx<T> *in, cplx<T> *out, int nb,
                               int ld, int m, int Ns, int R, int sign,
                               const DimArgs &d, int tid, int nt)
 {
   const int mR=m/R;
-  const int total=nb*mR;
+  const int total=nb*m;
   const int64_t twstep=(int64_t) (m/(Ns*R))*d.q;
   const int64_t wR=d.N/R; // zN index of e^{2pi i/R}
   for(int idx=tid; idx < total; idx += nt) {
-    int b=idx/mR;
-    int j=idx-b*mR;
+    int b=idx/m;
+    int rem=idx-b*m;
+    int u=rem/mR;
+    int j=rem-u*mR;
     int k=j % Ns;
     const cplx<T> *ib=in+(int64_t) b*ld;
-    cplx<T> *ob=out+(int64_t) b*ld;
-    int o0=(j/Ns)*Ns*R+k;
-    for(int u=0; u < R; ++u) {
-      cplx<T> s=mk((T) 0,(T) 0);
-      for(int t=0; t < R; ++t) {
-        cplx<T> x=ib[j+t*mR];
-        // twiddle w^{tk} e^{2pi i ut/R}
-        int64_t e=(int64_t) t*k*twstep+(int64_t) ((u*t) % R)*wR;
-        cplx<T> w=ztab<T>(d,e % d.N);
-        s=s+(sign > 0 ? x*w : mulc(x,w));
-      }
-      ob[o0+u*Ns]=s;
+    const int64_t E1=((int64_t) k*twstep+(int64_t) u*wR) % d.N;
+    int64_t e=0;
+    cplx<T> s=mk((T) 0,(T) 0);
+    for(int t=0; t < R; ++t) {
+      cplx<T> x=ib[j+t*mR];
+      cplx<T> w=ztab<T>(d,e);
+      s=s+(sign > 0 ? x*w : mulc(x,w));
+      e += E1;
+      if(e >= d.N) e -= d.N;
     }
+    out[(int64_t) b*ld+(j/Ns)*Ns*R+k+u*Ns]=s;
   }
 }
 
