@@ -9,7 +9,10 @@
 // way: residue r takes the inputs at indices q s + r, and the passes
 // accumulate into the output.  m is the largest divisor of n that the
 // sub-FFT layer takes (at most 4096), so powers of two up to 4096 are one
-// register pass and up to 8192 two; other lengths cost q passes.
+// register pass and up to 8192 two.  Longer contiguous 1D transforms
+// (q > 2) run in four steps instead: DFTs of length n1 = q down the columns
+// of the n1 x n2 matrix (n2 = m), the twiddles exp(sign 2 pi i j k1 / n),
+// and DFTs of length n2 along the rows written transposed, three passes.
 #include "fftwpp_amd.h"
 
 #include <algorithm>
@@ -21,13 +24,18 @@ extern "C" void fftwpp_amd_internal_set_error(const char *msg);
 extern "C" int fftwpp_amd_internal_fft_map(int dtype, int op, const void *x, size_t xs,
                                            size_t xd, void *y, size_t ys, size_t yd,
                                            size_t n, size_t nb, double scale, void *stream);
+extern "C" int fftwpp_amd_internal_fft_twiddle(int dtype, void *y, size_t n, size_t n2,
+                                               size_t nb, int sign, void *stream);
 
 struct fftwpp_amd_fft {
   int kind, dims, sign, dtype;
   size_t n[3], q[3];
   size_t howmany, istride, ostride, idist, odist;
   fftwpp_amd_dim *d[3];
-  void *tmp[2];
+  // four-step split of a long 1D transform: lengths n1 x n2, one dimension each
+  size_t n1, n2;
+  fftwpp_amd_dim *f1, *f2;
+  void *tmp[3];
   size_t tmpBytes;
 };
 
@@ -72,6 +80,38 @@ int axis(const fftwpp_amd_fft *h, int a, int sign, const char *x, size_t xS,
       if(rc) return rc;
     }
   }
+  return FFTWPP_AMD_OK;
+}
+
+// One DFT of length n (1 or 2 residue passes) of a q = 1 dimension: column
+// form as in axis().
+int single(const fftwpp_amd_fft *h, fftwpp_amd_dim *d, int sign, const char *x, size_t xS,
+           size_t C, size_t nb, size_t xb, char *y, size_t yS, size_t yb, void *stream)
+{
+  const size_t E=elt(h->dtype);
+  for(size_t b0=0; b0 < nb; b0 += maxBatch) {
+    const size_t cnt=std::min(maxBatch,nb-b0);
+    int rc=sign > 0 ?
+      fftwpp_amd_forward_many(d,h->dtype,x+b0*xb*E,xS,C,cnt,xb,y+b0*yb*E,yS,yb,0,1,stream) :
+      fftwpp_amd_backward_many(d,h->dtype,x+b0*xb*E,xS,xb,y+b0*yb*E,yS,C,cnt,yb,0,1,0,1.0,
+                               stream);
+    if(rc) return rc;
+  }
+  return FFTWPP_AMD_OK;
+}
+
+// H contiguous transforms of length n = n1 n2 from x to y (x may equal y)
+// through the temporary t: element j1 n2 + j2 is row j1, column j2.
+int fourstep(const fftwpp_amd_fft *h, const char *x, char *y, char *t, size_t H, void *stream)
+{
+  const size_t n=h->n[0], n1=h->n1, n2=h->n2, E=elt(h->dtype);
+  // columns: t[k1 n2 + j2] = sum_j1 x[j1 n2 + j2] w_n1^{j1 k1}
+  int rc=single(h,h->f1,h->sign,x,n2,n2,H,n,t,n2,n,stream);
+  if(rc) return rc;
+  if((rc=fftwpp_amd_internal_fft_twiddle(h->dtype,t,n,n2,H,h->sign,stream))) return rc;
+  // rows k1, written transposed: y[k2 n1 + k1] = sum_j2 t[k1 n2 + j2] w_n2^{j2 k2}
+  for(size_t b=0; b < H; ++b)
+    if((rc=single(h,h->f2,h->sign,t+b*n*E,1,1,n1,n2,y+b*n*E,n1,1,stream))) return rc;
   return FFTWPP_AMD_OK;
 }
 
@@ -145,6 +185,13 @@ int fftwpp_amd_fft_create(fftwpp_amd_fft **out, int kind, int dims, const size_t
     }
     int rc=fftwpp_amd_dim_create(&h->d[a],n[a],m,h->q[a],FFTWPP_AMD_KIND_COMPLEX);
     if(rc) { fftwpp_amd_fft_destroy(h); return rc; }
+    if(dims == 1 && h->q[0] > 2) {
+      h->n1=h->q[0];
+      h->n2=m;
+      rc=fftwpp_amd_dim_create(&h->f1,h->n1,h->n1,1,FFTWPP_AMD_KIND_COMPLEX);
+      if(!rc) rc=fftwpp_amd_dim_create(&h->f2,h->n2,h->n2,1,FFTWPP_AMD_KIND_COMPLEX);
+      if(rc) { fftwpp_amd_fft_destroy(h); return rc; }
+    }
   }
   *out=h;
   return FFTWPP_AMD_OK;
@@ -155,6 +202,8 @@ int fftwpp_amd_fft_destroy(fftwpp_amd_fft *h)
   if(!h) return FFTWPP_AMD_OK;
   for(int a=0; a < 3; ++a)
     if(h->d[a]) fftwpp_amd_dim_destroy(h->d[a]);
+  if(h->f1) fftwpp_amd_dim_destroy(h->f1);
+  if(h->f2) fftwpp_amd_dim_destroy(h->f2);
   for(void *p : h->tmp)
     if(p) fftwpp_amd_free(p);
   delete h;
@@ -184,13 +233,18 @@ int fftwpp_amd_fft_execute(fftwpp_amd_fft *h, const void *in, void *out, double 
     const bool r2c=h->kind == FFTWPP_AMD_FFT_R2C;
     if((rc=fftwpp_amd_internal_fft_map(dt,r2c ? 1 : 3,x,h->istride,h->idist,t0,1,n,n,H,1.0,stream)))
       return rc;
-    if((rc=axis(h,0,h->sign,t0,1,1,H,n,t1,1,n,stream))) return rc;
+    rc=h->f1 ? fourstep(h,t0,t1,(char *) h->tmp[2],H,stream) :
+      axis(h,0,h->sign,t0,1,1,H,n,t1,1,n,stream);
+    if(rc) return rc;
     return fftwpp_amd_internal_fft_map(dt,r2c ? 0 : 2,t1,1,n,y,h->ostride,h->odist,
                                        r2c ? n/2+1 : n,H,scale,stream);
   }
   if(h->dims == 1) {
     const bool alias=x == y;
-    if(alias) {
+    if(h->f1 && h->istride == 1 && h->ostride == 1 && h->idist == n && h->odist == n) {
+      if((rc=temporaries(h,n*H))) return rc;
+      rc=fourstep(h,x,y,(char *) h->tmp[0],H,stream);
+    } else if(alias) {
       if((rc=temporaries(h,n*H))) return rc;
       if((rc=fftwpp_amd_internal_fft_map(dt,0,x,h->istride,h->idist,h->tmp[0],1,n,n,H,1.0,stream)))
         return rc;

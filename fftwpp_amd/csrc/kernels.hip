@@ -357,6 +357,23 @@ __global__ void fft_map_kernel(int op, const void *x, int64_t xs, int64_t xd,
   reinterpret_cast<cplx<T> *>(y)[b*yd+j*ys]=s*v;
 }
 
+// The twiddle step of a four-step DFT of length n = n1 n2: y[b n + k1 n2 + j]
+// *= exp(sign 2 pi i j k1 / n), the angle reduced exactly in integers and
+// evaluated in FP64.
+template<typename T>
+__global__ void fft_twiddle_kernel(cplx<T> *y, int64_t n, int64_t n2, int64_t nb, int sign)
+{
+  int64_t i=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  if(i >= n*nb) return;
+  int64_t r=i % n;
+  int64_t k1=r/n2, j=r-k1*n2;
+  int64_t e=(j*k1) % n;   // j k1 < n
+  double sn, cs;
+  sincospi(2.0*(double) e/(double) n,&sn,&cs);
+  cplx<T> w=mk((T) cs,(T) (sign > 0 ? sn : -sn));
+  y[i]=y[i]*w;
+}
+
 template<typename T>
 __global__ void scale_kernel(cplx<T> *f, int64_t n, T s)
 {
@@ -1305,6 +1322,26 @@ int fftwpp_amd_internal_fft_map(int dtype, int op, const void *x, size_t xs,
                        (int64_t) xd,y,(int64_t) ys,(int64_t) yd,(int64_t) n,(int64_t) nb,
                        scale);
   return launch_check("fft_map_kernel");
+}
+
+// The twiddle step of fft_twiddle_kernel over nb contiguous transforms
+// (internal to the library).
+int fftwpp_amd_internal_fft_twiddle(int dtype, void *y, size_t n, size_t n2, size_t nb,
+                                    int sign, void *stream)
+{
+  if(n == 0 || nb == 0) return FFTWPP_AMD_OK;
+  if(!y || n2 == 0 || n % n2) return fail(FFTWPP_AMD_ERR_ARG,"bad twiddle step");
+  const size_t total=n*nb;
+  if((total+255)/256 > 0xffffffffu) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"twiddle step too large");
+  const unsigned blocks=(unsigned) ((total+255)/256);
+  hipStream_t st=(hipStream_t) stream;
+  if(dtype == FFTWPP_AMD_F32)
+    hipLaunchKernelGGL(fft_twiddle_kernel<float>,dim3(blocks),dim3(256),0,st,(cplx<float> *) y,
+                       (int64_t) n,(int64_t) n2,(int64_t) nb,sign);
+  else
+    hipLaunchKernelGGL(fft_twiddle_kernel<double>,dim3(blocks),dim3(256),0,st,(cplx<double> *) y,
+                       (int64_t) n,(int64_t) n2,(int64_t) nb,sign);
+  return launch_check("fft_twiddle_kernel");
 }
 
 int fftwpp_amd_multiply(int dtype, int mult, void *F0, const void *F1,
