@@ -340,6 +340,87 @@ def bench_hconv2d(torch, dev, steps, warmup):
             "geometry": [list(v) for v in plan.geometry()]}
 
 
+def bench_real3d(torch, dev, steps, warmup):
+    """3D real convolution 256^3 -> 512^3 (fftPadReal outer dimension), FP64,
+    checked against the complex path on the same real data."""
+    import numpy as np
+    from fftwpp_amd.torch_conv import DeviceConvolution
+    L = 256
+    rng = np.random.Generator(np.random.PCG64(SEED + 5))
+    f = torch.tensor(rng.uniform(-1, 1, (L, L, L)), device=dev)
+    g = torch.tensor(rng.uniform(-1, 1, (L, L, L)), device=dev)
+    out = torch.empty_like(f)
+    plan = DeviceConvolution((L, L, L), kind="real", dtype=torch.float64, M=[2 * L] * 3)
+    ms = _time_calls(torch, lambda: plan.convolve_out(out, f, g), steps, warmup)
+    ref = DeviceConvolution((L, L, L), M=[2 * L] * 3)
+    fc, gc = f.to(torch.complex128), g.to(torch.complex128)
+    hc = torch.empty_like(fc)
+    ref.convolve_out(hc, fc, gc)
+    err = float(torch.linalg.norm(out - hc.real) / torch.linalg.norm(hc.real))
+    bytes_alg = CONV_UNITS * 8 * L ** 3
+    return {"workload": "3D real 256^3 -> 512^3 (M = 2L), FP64, uniform[-1,1] seed %d" % (SEED + 5),
+            "conv_per_s": round(1000.0 / ms, 2), "ms_per_conv": round(ms, 4),
+            "achieved_GBs": round(bytes_alg / (ms * 1e-3) / 1e9, 1),
+            "frac_of_hbm_peak": round(bytes_alg / (ms * 1e-3) / 1e9 / PEAK_GBS, 4),
+            "bytes_alg": bytes_alg, "rel_l2_vs_complex_path": err,
+            "note": "runs as the complex convolution of f + i g (twice the real model's bytes)"}
+
+
+def bench_user_multiplier(torch, dev, steps, warmup):
+    """The 3D 256^3 -> 512^3 workload through the user-multiplier path (a
+    device binary product of tests/support/multipliers.hip: every dimension
+    expanded, the multiplier called once), against the fused path."""
+    import ctypes
+    import numpy as np
+    from fftwpp_amd.torch_conv import DeviceConvolution, UserConvolution
+    lib = os.path.join(ROOT, "tests", "support", "libtestmult.so")
+    if not os.path.exists(lib):
+        return {"skipped": "tests/support/libtestmult.so not built"}
+    mult = ctypes.cast(ctypes.CDLL(lib).testmult_binary, ctypes.c_void_p)
+    L = 256
+    rng = np.random.Generator(np.random.PCG64(SEED + 6))
+    f = torch.tensor(rng.uniform(-1, 1, (L, L, L)) + 1j * rng.uniform(-1, 1, (L, L, L)), device=dev)
+    g = torch.tensor(rng.uniform(-1, 1, (L, L, L)) + 1j * rng.uniform(-1, 1, (L, L, L)), device=dev)
+    plan = UserConvolution((L, L, L), 2, 1, mult, M=[2 * L] * 3, m=[256] * 3)
+    a = f.clone()
+    ms = _time_calls(torch, lambda: plan(a, g), steps, warmup)
+    a.copy_(f)
+    plan(a, g)
+    fused = DeviceConvolution((L, L, L), M=[2 * L] * 3)
+    h = torch.empty_like(f)
+    fused.convolve_out(h, f, g)
+    err = float(torch.linalg.norm(a - h) / torch.linalg.norm(h))
+    del plan, a, f, g, h
+    torch.cuda.empty_cache()
+    return {"workload": "3D complex 256^3 -> 512^3, FP64, user device multiplier F0 *= F1",
+            "conv_per_s": round(1000.0 / ms, 2), "ms_per_conv": round(ms, 4),
+            "rel_l2_vs_fused": err}
+
+
+def bench_fft(torch, dev, steps, warmup):
+    """Standalone FFTs: 1D complex n = 2^20 and 3D complex 256^3, FP64, sign
+    -1; algorithmic bytes one read and one write of the array."""
+    import numpy as np
+    from fftwpp_amd.torch_fft import DeviceFFT
+    res = {}
+    rng = np.random.Generator(np.random.PCG64(SEED + 7))
+    for name, shape in [("fft1d_2^20", (1 << 20,)), ("fft3d_256", (256, 256, 256))]:
+        x = torch.tensor(rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape), device=dev)
+        y = torch.empty_like(x)
+        plan = DeviceFFT(shape if len(shape) > 1 else shape[0], -1, dtype=torch.float64)
+        ms = _time_calls(torch, lambda: plan(x, y), 5 * steps, warmup)
+        err = float(torch.linalg.norm(y - torch.fft.fftn(x)) / torch.linalg.norm(y))
+        # torch.fft (the vendor FFT library) on the same array, for comparison
+        ms_t = _time_calls(torch, lambda: torch.fft.fftn(x), 5 * steps, warmup)
+        nbytes = 2 * 16 * x.numel()
+        res[name] = {"ms": round(ms, 4), "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "frac_of_hbm_peak": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_GBS, 4),
+                     "rel_l2_vs_torch_fft": err, "torch_fft_ms": round(ms_t, 4)}
+        del x, y, plan
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     args = parse()
     if args.traffic_child:
@@ -450,6 +531,9 @@ def main():
         result["batched"] = bench_batched(torch, dev, args.steps, args.warmup)
         result["conv1d"] = bench_conv1d(torch, dev, args.steps)
         result["hconv2d"] = bench_hconv2d(torch, dev, args.steps, args.warmup)
+        result["real3d"] = bench_real3d(torch, dev, args.steps, args.warmup)
+        result["user_multiplier"] = bench_user_multiplier(torch, dev, min(args.steps, 5), 1)
+        result["fft"] = bench_fft(torch, dev, args.steps, args.warmup)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(f, g, L)

@@ -32,6 +32,14 @@ __global__ void ternary_kernel(cx<T> *F0, const cx<T> *F1, const cx<T> *F2, size
   if(i < n) F0[i]=mul(mul(F0[i],F1[i]),F2[i]);
 }
 
+// F0 <- F0 F1: the binary product of multBinary, as a user multiplier
+template<typename T>
+__global__ void binary_kernel(cx<T> *F0, const cx<T> *F1, size_t n)
+{
+  size_t i=(size_t) blockIdx.x*TPB+threadIdx.x;
+  if(i < n) F0[i]=mul(F0[i],F1[i]);
+}
+
 // F0 <- F0 F0: autoconvolution (A = 1, B = 1)
 template<typename T>
 __global__ void square_kernel(cx<T> *F0, size_t n)
@@ -82,6 +90,7 @@ void launch(int which, void **F, size_t n, const fftwpp_amd_layout *l, hipStream
     case 0: hipLaunchKernelGGL(ternary_kernel<T>,dim3(blocks(n)),dim3(TPB),0,s,G[0],G[1],G[2],n); break;
     case 1: hipLaunchKernelGGL(square_kernel<T>,dim3(blocks(n)),dim3(TPB),0,s,G[0],n); break;
     case 2: hipLaunchKernelGGL(pair_kernel<T>,dim3(blocks(n)),dim3(TPB),0,s,G[0],G[1],n); break;
+    case 4: hipLaunchKernelGGL(binary_kernel<T>,dim3(blocks(n)),dim3(TPB),0,s,G[0],G[1],n); break;
     default: hipLaunchKernelGGL(weighted_kernel<T>,dim3(blocks(n)),dim3(TPB),0,s,G[0],G[1],n,*l); break;
   }
 }
@@ -117,6 +126,8 @@ void testmult_pair(void **F, size_t n, const fftwpp_amd_layout *l, void *s, void
 { dispatch(2,F,n,l,s); }
 void testmult_weighted(void **F, size_t n, const fftwpp_amd_layout *l, void *s, void *)
 { dispatch(3,F,n,l,s); }
+void testmult_binary(void **F, size_t n, const fftwpp_amd_layout *l, void *s, void *)
+{ dispatch(4,F,n,l,s); }
 
 // The C++ face on host arrays (interleaved complex doubles), as a user of
 // convolve.h writes it (tests/hybridconv.cc:44-46 with A = 3 or a weighted
