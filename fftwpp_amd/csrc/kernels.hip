@@ -638,7 +638,9 @@ static int fwd_many_t(const fftwpp_amd_dim *d, const void *f, size_t S,
                       hipStream_t st)
 {
   const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
-  if(fast256(d,S,SF)) {
+  // the 256-point column kernel fills its twiddle tables per block: taken
+  // when a block has a full tile of columns (single rows go to the LDS path)
+  if(C >= (size_t) Col256<T>::TC && fast256(d,S,SF)) {
     constexpr int TC=Col256<T>::TC;
     dim3 grid((unsigned) ((C+TC-1)/TC),(unsigned) nbatch);
     size_t lds=col256_lds<T>(nres);
@@ -650,7 +652,9 @@ static int fwd_many_t(const fftwpp_amd_dim *d, const void *f, size_t S,
                        (int64_t) r0,(int64_t) nres,(const cplx<T> *) a.zN,(int64_t) d->q);
     return launch_check("colfwd256_kernel");
   }
-  if(cols_reg(d))
+  // register columns: tiles of several columns below m = 2048, so single
+  // rows of shorter lengths go to the LDS path
+  if(cols_reg(d) && (C >= 8 || d->m >= 2048))
     return colreg_t<T>(a,true,f,S,bsi,F,SF,bso,C,nbatch,r0,nres,0,1.0,st);
   size_t elt=sizeof(cplx<T>);
   size_t per=2*(d->m+1)*elt;
@@ -675,7 +679,7 @@ static int bwd_many_t(const fftwpp_amd_dim *d, const void *F, size_t SF,
                       int accumulate, double scale, hipStream_t st)
 {
   const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
-  if(fast256(d,S,SF)) {
+  if(C >= (size_t) Col256<T>::TC && fast256(d,S,SF)) {
     constexpr int TC=Col256<T>::TC;
     dim3 grid((unsigned) ((C+TC-1)/TC),(unsigned) nbatch);
     size_t lds=col256_lds<T>(nres);
@@ -688,7 +692,7 @@ static int bwd_many_t(const fftwpp_amd_dim *d, const void *F, size_t SF,
                        (int64_t) d->q);
     return launch_check("colbwd256_kernel");
   }
-  if(cols_reg(d))
+  if(cols_reg(d) && (C >= 8 || d->m >= 2048))
     return colreg_t<T>(a,false,F,SF,bsi,f,S,bso,C,nbatch,r0,nres,accumulate,scale,st);
   size_t elt=sizeof(cplx<T>);
   size_t per=2*(d->m+1)*elt;
@@ -1074,6 +1078,8 @@ int fftwpp_amd_internal_rows_fit(const fftwpp_amd_dim *d, int dtype)
 // 2^30): stage 0 = column transforms (forward_many / backward_many), 1 =
 // fused rows (conv_rows).  Returns 0 for the generic LDS kernels, 1 for the
 // m = 256 register kernels, 2 for the register-FFT kernels of rowsreg.h.
+// Column transforms take the register kernels when a tile has at least 8
+// columns (or m >= 2048); fewer columns run on the LDS kernels.
 int fftwpp_amd_internal_engine(const fftwpp_amd_dim *d, int stage)
 {
   if(!d) return -1;
