@@ -98,11 +98,14 @@ __device__ __forceinline__ cplx<T> hfold(const DimArgs &d, const cplx<T> *f,
 // ------------------------------------------------------------------------
 // fwd_many: F[b][(r-r0)*m+s][c] = sum_k f[b][k][c] zeta_N^{k(qs+r)}
 // ------------------------------------------------------------------------
+// Column c of the input starts at c*ics and of the output at c*ocs: unit
+// strides for interleaved columns; the row strides when contiguous rows are
+// taken as the columns (then consecutive threads walk along a row).
 template<typename T>
 __global__ void fwd_many_kernel(DimArgs d, const cplx<T> *f, int64_t S,
                                 int64_t C, int64_t bsi, cplx<T> *F,
                                 int64_t SF, int64_t bso, int64_t r0,
-                                int64_t nres, int TC)
+                                int64_t nres, int TC, int64_t ics, int64_t ocs)
 {
   extern __shared__ char smem_raw[];
   cplx<T> *A=reinterpret_cast<cplx<T> *>(smem_raw);
@@ -112,21 +115,25 @@ __global__ void fwd_many_kernel(DimArgs d, const cplx<T> *f, int64_t S,
   const int64_t c0=xcd_tile(blockIdx.x,gridDim.x)*TC;
   const int64_t b=blockIdx.y;
   const int nc=(int) min((int64_t) TC,C-c0);
-  const cplx<T> *fb=f+b*bsi+c0;
-  cplx<T> *Fb=F+b*bso+c0;
+  const cplx<T> *fb=f+b*bsi+c0*ics;
+  cplx<T> *Fb=F+b*bso+c0*ocs;
   const int tid=threadIdx.x, nt=blockDim.x;
   for(int64_t rr=0; rr < nres; ++rr) {
     int64_t r=r0+rr;
     for(int idx=tid; idx < nc*m; idx += nt) {
-      int sigma=idx/nc, cc=idx-sigma*nc;
-      A[cc*ld+sigma]=fold<T>(d,fb+cc,S,sigma,r);
+      int sigma, cc;
+      if(ics == 1) { sigma=idx/nc; cc=idx-sigma*nc; }
+      else { cc=idx/m; sigma=idx-cc*m; }
+      A[cc*ld+sigma]=fold<T>(d,fb+cc*ics,S,sigma,r);
     }
     __syncthreads();
     cplx<T> *X=lds_fft<T>(A,B,nc,ld,d,1);
     cplx<T> *Fr=Fb+rr*m*SF;
     for(int idx=tid; idx < nc*m; idx += nt) {
-      int s=idx/nc, cc=idx-s*nc;
-      Fr[(int64_t) s*SF+cc]=X[cc*ld+s];
+      int s, cc;
+      if(ocs == 1) { s=idx/nc; cc=idx-s*nc; }
+      else { cc=idx/m; s=idx-cc*m; }
+      Fr[(int64_t) s*SF+cc*ocs]=X[cc*ld+s];
     }
     __syncthreads();
   }
@@ -140,7 +147,8 @@ template<typename T>
 __global__ void bwd_many_kernel(DimArgs d, const cplx<T> *F, int64_t SF,
                                 int64_t bsi, cplx<T> *f, int64_t S,
                                 int64_t C, int64_t bso, int64_t r0,
-                                int64_t nres, int accumulate, T scale, int TC)
+                                int64_t nres, int accumulate, T scale, int TC,
+                                int64_t ics, int64_t ocs)
 {
   extern __shared__ char smem_raw[];
   cplx<T> *A=reinterpret_cast<cplx<T> *>(smem_raw);
@@ -150,27 +158,31 @@ __global__ void bwd_many_kernel(DimArgs d, const cplx<T> *F, int64_t SF,
   const int64_t c0=xcd_tile(blockIdx.x,gridDim.x)*TC;
   const int64_t b=blockIdx.y;
   const int nc=(int) min((int64_t) TC,C-c0);
-  const cplx<T> *Fb=F+b*bsi+c0;
-  cplx<T> *fb=f+b*bso+c0;
+  const cplx<T> *Fb=F+b*bsi+c0*ics;
+  cplx<T> *fb=f+b*bso+c0*ocs;
   const int tid=threadIdx.x, nt=blockDim.x;
   const int L=(int) d.L;
   for(int64_t rr=0; rr < nres; ++rr) {
     int64_t r=r0+rr;
     const cplx<T> *Fr=Fb+rr*m*SF;
     for(int idx=tid; idx < nc*m; idx += nt) {
-      int s=idx/nc, cc=idx-s*nc;
-      A[cc*ld+s]=Fr[(int64_t) s*SF+cc];
+      int s, cc;
+      if(ics == 1) { s=idx/nc; cc=idx-s*nc; }
+      else { cc=idx/m; s=idx-cc*m; }
+      A[cc*ld+s]=Fr[(int64_t) s*SF+cc*ics];
     }
     __syncthreads();
     cplx<T> *X=lds_fft<T>(A,B,nc,ld,d,-1);
     bool last=rr == nres-1;
     bool add=accumulate || rr > 0;
     for(int idx=tid; idx < nc*L; idx += nt) {
-      int j=idx/nc, cc=idx-j*nc;
+      int j, cc;
+      if(ocs == 1) { j=idx/nc; cc=idx-j*nc; }
+      else { cc=idx/L; j=idx-cc*L; }
       int64_t k=j+d.k0;
       int sigma=(int) floormod<T>(k,d.m);
       cplx<T> v=mulc(X[cc*ld+sigma],zeta<T>(d,k*r));
-      cplx<T> *o=fb+(int64_t) j*S+cc;
+      cplx<T> *o=fb+(int64_t) j*S+cc*ocs;
       if(add) v=v+*o;
       if(last) v=scale*v;
       *o=v;
@@ -659,6 +671,12 @@ static int fwd_many_t(const fftwpp_amd_dim *d, const void *f, size_t S,
   size_t elt=sizeof(cplx<T>);
   size_t per=2*(d->m+1)*elt;
   if(per > lds_max()) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"m too large for the LDS transform");
+  // single columns of many batch entries: the entries become the columns
+  size_t ics=1, ocs=1;
+  if(C == 1 && nbatch > 1) {
+    C=nbatch; ics=bsi; ocs=bso;
+    nbatch=1; bsi=bso=0;
+  }
   size_t TC=1;
   while(TC < 16 && TC < C && 2*TC*per <= lds_budget()) TC *= 2;
   size_t lds=TC*per;
@@ -668,7 +686,7 @@ static int fwd_many_t(const fftwpp_amd_dim *d, const void *f, size_t S,
   hipLaunchKernelGGL(fwd_many_kernel<T>,grid,dim3(256),lds,st,a,
                      (const cplx<T> *) f,(int64_t) S,(int64_t) C,
                      (int64_t) bsi,(cplx<T> *) F,(int64_t) SF,(int64_t) bso,
-                     (int64_t) r0,(int64_t) nres,(int) TC);
+                     (int64_t) r0,(int64_t) nres,(int) TC,(int64_t) ics,(int64_t) ocs);
   return launch_check("fwd_many_kernel");
 }
 
@@ -697,6 +715,12 @@ static int bwd_many_t(const fftwpp_amd_dim *d, const void *F, size_t SF,
   size_t elt=sizeof(cplx<T>);
   size_t per=2*(d->m+1)*elt;
   if(per > lds_max()) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"m too large for the LDS transform");
+  // single columns of many batch entries: the entries become the columns
+  size_t ics=1, ocs=1;
+  if(C == 1 && nbatch > 1) {
+    C=nbatch; ics=bsi; ocs=bso;
+    nbatch=1; bsi=bso=0;
+  }
   size_t TC=1;
   while(TC < 16 && TC < C && 2*TC*per <= lds_budget()) TC *= 2;
   size_t lds=TC*per;
@@ -706,7 +730,8 @@ static int bwd_many_t(const fftwpp_amd_dim *d, const void *F, size_t SF,
   hipLaunchKernelGGL(bwd_many_kernel<T>,grid,dim3(256),lds,st,a,
                      (const cplx<T> *) F,(int64_t) SF,(int64_t) bsi,
                      (cplx<T> *) f,(int64_t) S,(int64_t) C,(int64_t) bso,
-                     (int64_t) r0,(int64_t) nres,accumulate,(T) scale,(int) TC);
+                     (int64_t) r0,(int64_t) nres,accumulate,(T) scale,(int) TC,
+                     (int64_t) ics,(int64_t) ocs);
   return launch_check("bwd_many_kernel");
 }
 
