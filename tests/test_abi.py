@@ -109,3 +109,39 @@ def test_host_pointers_rejected_by_device_entry(built):
     rc = L.fftwpp_amd_conv_create(ctypes.byref(h), 1, 0, _lib.sizes([8]), None, None, 0)
     # without a device the twiddle upload fails with a HIP status, not a crash
     assert rc != 0
+
+
+def test_cxx_headers_compile_and_link(built, tmp_path):
+    """A C++ user of convolve.h and fftw++.h (the reference's class names,
+    a user device multiplier with the reference signature) compiles with g++
+    and links against the library; nothing is run."""
+    src = tmp_path / "user.cc"
+    src.write_text(r'''
+#include "convolve.h"
+#include "fftw++.h"
+using namespace fftwpp;
+static void mult(Complex **F, size_t n, Indices *indices, size_t) {
+  (void) F; (void) n; (void) indices->layout; (void) indices->stream;
+}
+int run(size_t L, Complex **f, Complex *g) {
+  Application app(3,1,mult);
+  fftPad fft(L,3*L-2,app);
+  Convolution C(&fft);
+  C.convolve(f);
+  fft1d Forward(L,-1);
+  Forward.fft(g);
+  mfft1d Many(L,1,4,1,L);
+  rcfft1d R(2*L);
+  crfft1d Cr(2*L);
+  fft2d F2(L,L,-1);
+  fft3d F3(L,L,L,1);
+  return (int) (R.normalization()+Cr.normalization()+F2.normalization()+F3.normalization()+
+                 Many.normalization());
+}
+int main() { return 0; }
+''')
+    exe = tmp_path / "user"
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), str(src),
+                        "-L", os.path.dirname(LIB), "-lfftwpp_amd", "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
