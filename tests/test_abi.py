@@ -145,3 +145,29 @@ int main() { return 0; }
                         "-L", os.path.dirname(LIB), "-lfftwpp_amd", "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_fft_and_general_plan_argument_checks(built):
+    """Host-side validation of the standalone FFT and user-multiplier plans:
+    rejected before any device memory is touched."""
+    from fftwpp_amd import _lib
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    P = ctypes.c_void_p
+    # bad dimension count, kind, sign, dtype
+    assert L.fftwpp_amd_fft_create(ctypes.byref(h), 0, 0, _lib.sizes([8]), -1, 0, 1, 0, 0, 0, 0) == -1
+    assert L.fftwpp_amd_fft_create(ctypes.byref(h), 3, 1, _lib.sizes([8]), -1, 0, 1, 0, 0, 0, 0) == -1
+    assert L.fftwpp_amd_fft_create(ctypes.byref(h), 0, 1, _lib.sizes([8]), 2, 0, 1, 0, 0, 0, 0) == -1
+    assert L.fftwpp_amd_fft_create(ctypes.byref(h), 0, 1, _lib.sizes([8]), -1, 5, 1, 0, 0, 0, 0) == -1
+    assert L.fftwpp_amd_fft_create(ctypes.byref(h), 0, 1, _lib.sizes([0]), -1, 0, 1, 0, 0, 0, 0) == -1
+    # real multidimensional transforms are not provided
+    assert L.fftwpp_amd_fft_create(ctypes.byref(h), 1, 2, _lib.sizes([8, 8]), -1, 0, 1, 0, 0, 0, 0) == -3
+    # a prime above the largest sub-FFT has no divisor in [n/4096, 4096]
+    assert L.fftwpp_amd_fft_create(ctypes.byref(h), 0, 1, _lib.sizes([5003]), -1, 0, 1, 0, 0, 0, 0) == -3
+    assert h.value is None
+    # user-multiplier plans: A, B in [1, 8], a multiplier is required
+    dummy = P(1)
+    for A, B, mult in [(0, 1, dummy), (2, 9, dummy), (2, 1, None)]:
+        rc = L.fftwpp_amd_conv_create_general(ctypes.byref(h), 1, 0, _lib.sizes([16]), None, None,
+                                              0, A, B, mult, None)
+        assert rc == -1, (A, B)
