@@ -13,6 +13,10 @@
 // (q > 2) run in four steps instead: DFTs of length n1 = q down the columns
 // of the n1 x n2 matrix (n2 = m), the twiddles exp(sign 2 pi i j k1 / n),
 // and DFTs of length n2 along the rows written transposed, three passes.
+// A 1D length with no divisor in [n/4096, 4096] (a prime above 4096, say)
+// runs through Bluestein's algorithm: chirp multiply, a circular
+// convolution of a power-of-two length >= 2n - 1 by two such transforms,
+// chirp multiply.
 #include "fftwpp_amd.h"
 
 #include <algorithm>
@@ -26,6 +30,10 @@ extern "C" int fftwpp_amd_internal_fft_map(int dtype, int op, const void *x, siz
                                            size_t n, size_t nb, double scale, void *stream);
 extern "C" int fftwpp_amd_internal_fft_twiddle(int dtype, void *y, size_t n, size_t n2,
                                                size_t nb, int sign, void *stream);
+extern "C" int fftwpp_amd_internal_fft_chirp(int dtype, int op, const void *x, size_t xs,
+                                             size_t xd, void *y, size_t ys, size_t yd,
+                                             size_t n, size_t len, size_t nb, int sign,
+                                             double scale, void *stream);
 
 struct fftwpp_amd_fft {
   int kind, dims, sign, dtype;
@@ -35,6 +43,12 @@ struct fftwpp_amd_fft {
   // four-step split of a long 1D transform: lengths n1 x n2, one dimension each
   size_t n1, n2;
   fftwpp_amd_dim *f1, *f2;
+  // Bluestein: transforms of the power of two blen >= 2n - 1 (sign -1, +1),
+  // the chirp w (n words), the transformed kernel / blen (blen words) and
+  // blen words per transform of work
+  size_t blen;
+  fftwpp_amd_fft *bf, *bi;
+  void *chirp, *bhat, *bwork;
   void *tmp[3];
   size_t tmpBytes;
 };
@@ -115,6 +129,49 @@ int fourstep(const fftwpp_amd_fft *h, const char *x, char *y, char *t, size_t H,
   return FFTWPP_AMD_OK;
 }
 
+// Bluestein's algorithm for H transforms of length n: element j of
+// transform t at x[t xd + j xs], output at y[t yd + k ys].
+int bluestein(const fftwpp_amd_fft *h, const void *x, size_t xs, size_t xd, void *y,
+              size_t ys, size_t yd, size_t H, double scale, void *stream)
+{
+  const size_t n=h->n[0], len=h->blen, E=elt(h->dtype);
+  int rc=fftwpp_amd_internal_fft_chirp(h->dtype,2,x,xs,xd,h->bwork,1,len,n,len,H,h->sign,1.0,
+                                       stream);
+  if(!rc) rc=fftwpp_amd_fft_execute(h->bf,h->bwork,h->bwork,1.0,stream);
+  for(size_t t=0; !rc && t < H; ++t)
+    rc=fftwpp_amd_multiply(h->dtype,FFTWPP_AMD_MULT_BINARY,(char *) h->bwork+t*len*E,h->bhat,
+                           len,stream);
+  if(!rc) rc=fftwpp_amd_fft_execute(h->bi,h->bwork,h->bwork,1.0,stream);
+  if(!rc) rc=fftwpp_amd_internal_fft_chirp(h->dtype,3,h->bwork,1,len,y,ys,yd,n,len,H,h->sign,
+                                           scale,stream);
+  return rc;
+}
+
+int bluesteinSetup(fftwpp_amd_fft *h)
+{
+  const size_t n=h->n[0], H=h->howmany, E=elt(h->dtype);
+  size_t len=1;
+  while(len < 2*n-1) len *= 2;
+  h->blen=len;
+  int rc=fftwpp_amd_fft_create(&h->bf,FFTWPP_AMD_FFT_C2C,1,&len,-1,h->dtype,H,1,1,len,len);
+  if(!rc) rc=fftwpp_amd_fft_create(&h->bi,FFTWPP_AMD_FFT_C2C,1,&len,1,h->dtype,H,1,1,len,len);
+  if(!rc) rc=fftwpp_amd_malloc(&h->chirp,n*E);
+  if(!rc) rc=fftwpp_amd_malloc(&h->bhat,len*E);
+  if(!rc) rc=fftwpp_amd_malloc(&h->bwork,len*H*E);
+  if(rc) return rc;
+  // the kernel's transform, scaled by 1/len for the inverse
+  fftwpp_amd_fft *one=NULL;
+  rc=fftwpp_amd_fft_create(&one,FFTWPP_AMD_FFT_C2C,1,&len,-1,h->dtype,1,1,1,len,len);
+  if(!rc) rc=fftwpp_amd_internal_fft_chirp(h->dtype,0,NULL,0,0,h->chirp,1,n,n,len,1,h->sign,1.0,
+                                           NULL);
+  if(!rc) rc=fftwpp_amd_internal_fft_chirp(h->dtype,1,NULL,0,0,h->bhat,1,len,n,len,1,h->sign,
+                                           1.0,NULL);
+  if(!rc) rc=fftwpp_amd_fft_execute(one,h->bhat,h->bhat,1.0/(double) len,NULL);
+  if(!rc) rc=fftwpp_amd_synchronize(NULL);
+  fftwpp_amd_fft_destroy(one);
+  return rc;
+}
+
 int temporaries(fftwpp_amd_fft *h, size_t words)
 {
   const size_t bytes=words*elt(h->dtype);
@@ -180,8 +237,14 @@ int fftwpp_amd_fft_create(fftwpp_amd_fft **out, int kind, int dims, const size_t
     const size_t m=subsize(n[a]);
     h->q[a]=n[a]/m;
     if(h->q[a] > maxPasses) {
-      fftwpp_amd_fft_destroy(h);
-      return error(FFTWPP_AMD_ERR_UNSUPPORTED,"fftwpp_amd_fft_create: length without a divisor in [n/4096, 4096]");
+      // Bluestein's power of two must itself split into four steps (<= 2^24)
+      if(dims > 1 || n[a] > ((size_t) 1 << 23)) {
+        fftwpp_amd_fft_destroy(h);
+        return error(FFTWPP_AMD_ERR_UNSUPPORTED,"fftwpp_amd_fft_create: no divisor in [n/4096, 4096] (a multidimensional axis, or a 1D length above 2^23)");
+      }
+      int rc=bluesteinSetup(h);
+      if(rc) { fftwpp_amd_fft_destroy(h); return rc; }
+      continue;
     }
     int rc=fftwpp_amd_dim_create(&h->d[a],n[a],m,h->q[a],FFTWPP_AMD_KIND_COMPLEX);
     if(rc) { fftwpp_amd_fft_destroy(h); return rc; }
@@ -204,6 +267,10 @@ int fftwpp_amd_fft_destroy(fftwpp_amd_fft *h)
     if(h->d[a]) fftwpp_amd_dim_destroy(h->d[a]);
   if(h->f1) fftwpp_amd_dim_destroy(h->f1);
   if(h->f2) fftwpp_amd_dim_destroy(h->f2);
+  fftwpp_amd_fft_destroy(h->bf);
+  fftwpp_amd_fft_destroy(h->bi);
+  for(void *p : {h->chirp,h->bhat,h->bwork})
+    if(p) fftwpp_amd_free(p);
   for(void *p : h->tmp)
     if(p) fftwpp_amd_free(p);
   delete h;
@@ -233,12 +300,15 @@ int fftwpp_amd_fft_execute(fftwpp_amd_fft *h, const void *in, void *out, double 
     const bool r2c=h->kind == FFTWPP_AMD_FFT_R2C;
     if((rc=fftwpp_amd_internal_fft_map(dt,r2c ? 1 : 3,x,h->istride,h->idist,t0,1,n,n,H,1.0,stream)))
       return rc;
-    rc=h->f1 ? fourstep(h,t0,t1,(char *) h->tmp[2],H,stream) :
+    rc=h->blen ? bluestein(h,t0,1,n,t1,1,n,H,1.0,stream) :
+      h->f1 ? fourstep(h,t0,t1,(char *) h->tmp[2],H,stream) :
       axis(h,0,h->sign,t0,1,1,H,n,t1,1,n,stream);
     if(rc) return rc;
     return fftwpp_amd_internal_fft_map(dt,r2c ? 0 : 2,t1,1,n,y,h->ostride,h->odist,
                                        r2c ? n/2+1 : n,H,scale,stream);
   }
+  if(h->dims == 1 && h->blen)
+    return bluestein(h,x,h->istride,h->idist,y,h->ostride,h->odist,H,scale,stream);
   if(h->dims == 1) {
     const bool alias=x == y;
     if(h->f1 && h->istride == 1 && h->ostride == 1 && h->idist == n && h->odist == n) {

@@ -386,6 +386,45 @@ __global__ void fft_twiddle_kernel(cplx<T> *y, int64_t n, int64_t n2, int64_t nb
   y[i]=y[i]*w;
 }
 
+// Bluestein's algorithm for a length n without a usable divisor: with
+// w_k = exp(sign pi i k^2 / n), X_k = w_k sum_j (x_j w_j) conj(w_{k-j}), a
+// circular convolution of length len >= 2n - 1.  op 0: the chirp w (y, n
+// words); op 1: the convolution kernel b_j = conj(w_j), b_{len-j} = conj(w_j)
+// for 0 < j < n, else 0 (y, len words); op 2: y[b yd + j] = x[b xd + j xs] w_j
+// for j < n and 0 up to len; op 3: y[b yd + k ys] = s w_k x[b xd + k] for k < n.
+// k^2 is reduced modulo 2n in integers (k < 2^31) before the FP64 angle.
+template<typename T>
+__global__ void fft_chirp_kernel(int op, const cplx<T> *x, int64_t xs, int64_t xd,
+                                 cplx<T> *y, int64_t ys, int64_t yd, int64_t n,
+                                 int64_t len, int64_t nb, int sign, T s)
+{
+  int64_t i=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  const int64_t cnt=op == 3 ? n : (op == 0 ? n : len);
+  if(i >= cnt*nb) return;
+  int64_t b=i/cnt, j=i-b*cnt;
+  auto chirp=[&](int64_t k) {
+    int64_t e=(k*k) % (2*n);
+    double sn, cs;
+    sincospi((double) e/(double) n,&sn,&cs);
+    return mk((T) cs,(T) (sign > 0 ? sn : -sn));
+  };
+  if(op == 0) { y[j]=chirp(j); return; }
+  if(op == 1) {
+    cplx<T> v=mk((T) 0,(T) 0);
+    if(j < n) v=conjg(chirp(j));
+    else if(len-j < n) v=conjg(chirp(len-j));
+    y[j]=v;
+    return;
+  }
+  if(op == 2) {
+    cplx<T> v=mk((T) 0,(T) 0);
+    if(j < n) v=x[b*xd+j*xs]*chirp(j);
+    y[b*yd+j]=v;
+    return;
+  }
+  y[b*yd+j*ys]=s*(x[b*xd+j]*chirp(j));
+}
+
 template<typename T>
 __global__ void scale_kernel(cplx<T> *f, int64_t n, T s)
 {
@@ -1373,6 +1412,32 @@ int fftwpp_amd_internal_fft_twiddle(int dtype, void *y, size_t n, size_t n2, siz
     hipLaunchKernelGGL(fft_twiddle_kernel<double>,dim3(blocks),dim3(256),0,st,(cplx<double> *) y,
                        (int64_t) n,(int64_t) n2,(int64_t) nb,sign);
   return launch_check("fft_twiddle_kernel");
+}
+
+// The steps of fft_chirp_kernel (internal to the library).
+int fftwpp_amd_internal_fft_chirp(int dtype, int op, const void *x, size_t xs, size_t xd,
+                                  void *y, size_t ys, size_t yd, size_t n, size_t len,
+                                  size_t nb, int sign, double scale, void *stream)
+{
+  if(n == 0 || nb == 0) return FFTWPP_AMD_OK;
+  if(!y || op < 0 || op > 3 || (op >= 2 && !x) || len < n || n >= ((size_t) 1 << 31))
+    return fail(FFTWPP_AMD_ERR_ARG,"bad chirp step");
+  const size_t cnt=(op == 0 || op == 3) ? n : len;
+  const size_t total=cnt*nb;
+  if((total+255)/256 > 0xffffffffu) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"chirp step too large");
+  const unsigned blocks=(unsigned) ((total+255)/256);
+  hipStream_t st=(hipStream_t) stream;
+  if(dtype == FFTWPP_AMD_F32)
+    hipLaunchKernelGGL(fft_chirp_kernel<float>,dim3(blocks),dim3(256),0,st,op,
+                       (const cplx<float> *) x,(int64_t) xs,(int64_t) xd,(cplx<float> *) y,
+                       (int64_t) ys,(int64_t) yd,(int64_t) n,(int64_t) len,(int64_t) nb,sign,
+                       (float) scale);
+  else
+    hipLaunchKernelGGL(fft_chirp_kernel<double>,dim3(blocks),dim3(256),0,st,op,
+                       (const cplx<double> *) x,(int64_t) xs,(int64_t) xd,(cplx<double> *) y,
+                       (int64_t) ys,(int64_t) yd,(int64_t) n,(int64_t) len,(int64_t) nb,sign,
+                       scale);
+  return launch_check("fft_chirp_kernel");
 }
 
 int fftwpp_amd_multiply(int dtype, int mult, void *F0, const void *F1,

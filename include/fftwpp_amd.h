@@ -355,7 +355,10 @@ int fftwpp_amd_batch_execute(fftwpp_amd_batch *h, void *out, const void *f,
  * A length n = m q runs as q residue passes of m-point sub-FFTs (m the
  * largest divisor of n the sub-FFT layer takes, at most 4096 here), writing
  * natural order through strides: a length with such a divisor and small q
- * is one or two passes; any length is supported. */
+ * is one or two passes, longer contiguous 1D transforms take four steps
+ * (columns of q, twiddles, rows of m).  A 1D length without a divisor in
+ * [n/4096, 4096] (up to 2^23) runs through Bluestein's algorithm; such a
+ * length as an axis of a 2D/3D transform is FFTWPP_AMD_ERR_UNSUPPORTED. */
 #define FFTWPP_AMD_FFT_C2C 0
 #define FFTWPP_AMD_FFT_R2C 1
 #define FFTWPP_AMD_FFT_C2R 2

@@ -68,11 +68,46 @@ def test_c2c_1d(gpu, n):
             assert rel(got, dft(x, sign)) < tol, (n, p, sign)
 
 
-def test_c2c_unsupported_length(gpu):
+@pytest.mark.parametrize("n", [4099, 5003, 8198, 10007, 65537, 1000003])
+def test_c2c_bluestein(gpu, n):
+    # lengths with no divisor in [n/4096, 4096] (primes above 4096, 2 x 4099):
+    # Bluestein's algorithm over a power-of-two convolution
+    from fftwpp_amd.torch_fft import DeviceFFT
+    rng = np.random.default_rng(n)
+    x = rnd(rng, n)
+    for p, tol in PRECISIONS:
+        real, cplx, npc, npr = types(p)
+        for sign in (-1, 1):
+            plan = DeviceFFT(n, sign, dtype=real)
+            got = run(gpu, plan, x.astype(npc), (n,), cplx)
+            assert rel(got, dft(x, sign)) < tol, (n, p, sign)
+
+
+def test_bluestein_layouts_and_real(gpu):
+    # strided transforms, in place, and r2c / c2r through Bluestein
+    import torch
     from fftwpp_amd import FFTWPPError
     from fftwpp_amd.torch_fft import DeviceFFT
+    n, M, stride, dist = 5003, 3, 2, 10011
+    rng = np.random.default_rng(1)
+    extent = (M - 1) * dist + (n - 1) * stride + 1
+    buf = rnd(rng, extent)
+    idx = np.arange(M)[:, None] * dist + np.arange(n)[None, :] * stride
+    plan = DeviceFFT(n, -1, dtype=torch.float64, howmany=M, istride=stride, ostride=stride,
+                     idist=dist, odist=dist)
+    t = torch.tensor(buf, device=gpu)
+    plan(t, t)
+    torch.cuda.synchronize()
+    assert rel(t.cpu().numpy()[idx], np.fft.fft(buf[idx], axis=1)) < TOL64
+    xr = rng.uniform(-1, 1, n)
+    got = run(gpu, DeviceFFT(n, -1, kind="r2c", dtype=torch.float64), xr, (n // 2 + 1,),
+              torch.complex128)
+    assert rel(got, np.fft.rfft(xr)) < TOL64
+    back = run(gpu, DeviceFFT(n, 1, kind="c2r", dtype=torch.float64), np.fft.rfft(xr), (n,),
+               torch.float64)
+    assert rel(back, xr * n) < TOL64
     with pytest.raises(FFTWPPError):
-        DeviceFFT(5003)        # prime beyond the largest sub-FFT: no divisor in [n/4096, 4096]
+        DeviceFFT((5003, 4), -1)   # an axis of a multidimensional transform
 
 
 @pytest.mark.parametrize("n,M,stride,dist", [(48, 5, 1, 48), (48, 5, 5, 1), (30, 3, 2, 70),
