@@ -49,6 +49,11 @@ struct fftwpp_amd_fft {
   size_t blen;
   fftwpp_amd_fft *bf, *bi;
   void *chirp, *bhat, *bwork;
+  // real 2D/3D transforms: r2c / c2r of the last axis over all leading
+  // indices (contiguous real rows, and rows padded to 2 (n/2+1) reals as an
+  // in-place transform stores them), complex transforms of the leading axes
+  fftwpp_amd_fft *rows, *rowsPad;
+  size_t lead, half;
   void *tmp[3];
   size_t tmpBytes;
 };
@@ -211,8 +216,8 @@ int fftwpp_amd_fft_create(fftwpp_amd_fft **out, int kind, int dims, const size_t
     return error(FFTWPP_AMD_ERR_ARG,"fftwpp_amd_fft_create: bad kind, dims, sign or dtype");
   if(kind == FFTWPP_AMD_FFT_R2C) sign=-1;
   if(kind == FFTWPP_AMD_FFT_C2R) sign=1;
-  if(dims > 1 && (kind != FFTWPP_AMD_FFT_C2C || howmany > 1))
-    return error(FFTWPP_AMD_ERR_UNSUPPORTED,"fftwpp_amd_fft_create: multidimensional transforms are single complex transforms");
+  if(dims > 1 && howmany > 1)
+    return error(FFTWPP_AMD_ERR_UNSUPPORTED,"fftwpp_amd_fft_create: multidimensional transforms are single transforms");
   if(howmany == 0) howmany=1;
   for(int a=0; a < dims; ++a)
     if(n[a] == 0) return error(FFTWPP_AMD_ERR_ARG,"fftwpp_amd_fft_create: zero length");
@@ -232,8 +237,23 @@ int fftwpp_amd_fft_create(fftwpp_amd_fft **out, int kind, int dims, const size_t
   if(!h) return error(FFTWPP_AMD_ERR_HIP,"fftwpp_amd_fft_create: out of memory");
   h->kind=kind; h->dims=dims; h->sign=sign; h->dtype=dtype;
   h->howmany=howmany; h->istride=istride; h->ostride=ostride; h->idist=idist; h->odist=odist;
-  for(int a=0; a < dims; ++a) {
-    h->n[a]=n[a];
+  for(int a=0; a < dims; ++a) h->n[a]=n[a];
+  int axes=dims;
+  if(kind != FFTWPP_AMD_FFT_C2C && dims > 1) {
+    // the last axis as 1D real transforms of all leading indices
+    const size_t nl=n[dims-1], hl=nl/2+1;
+    h->lead=1;
+    for(int a=0; a+1 < dims; ++a) h->lead *= n[a];
+    h->half=hl;
+    const bool r2c=kind == FFTWPP_AMD_FFT_R2C;
+    int rc=fftwpp_amd_fft_create(&h->rows,kind,1,&nl,sign,dtype,h->lead,1,1,r2c ? nl : hl,
+                                 r2c ? hl : nl);
+    if(!rc) rc=fftwpp_amd_fft_create(&h->rowsPad,kind,1,&nl,sign,dtype,h->lead,1,1,
+                                     r2c ? 2*hl : hl,r2c ? hl : 2*hl);
+    if(rc) { fftwpp_amd_fft_destroy(h); return rc; }
+    axes=dims-1;
+  }
+  for(int a=0; a < axes; ++a) {
     const size_t m=subsize(n[a]);
     h->q[a]=n[a]/m;
     if(h->q[a] > maxPasses) {
@@ -269,6 +289,8 @@ int fftwpp_amd_fft_destroy(fftwpp_amd_fft *h)
   if(h->f2) fftwpp_amd_dim_destroy(h->f2);
   fftwpp_amd_fft_destroy(h->bf);
   fftwpp_amd_fft_destroy(h->bi);
+  fftwpp_amd_fft_destroy(h->rows);
+  fftwpp_amd_fft_destroy(h->rowsPad);
   for(void *p : {h->chirp,h->bhat,h->bwork})
     if(p) fftwpp_amd_free(p);
   for(void *p : h->tmp)
@@ -293,6 +315,39 @@ int fftwpp_amd_fft_execute(fftwpp_amd_fft *h, const void *in, void *out, double 
   const char *x=(const char *) in;
   char *y=(char *) out;
   int rc;
+  if(h->rows) {
+    // real 2D/3D: rows of the last axis, then the leading axes on lead x half
+    // complex words (in place: real rows padded to 2 half words)
+    fftwpp_amd_fft *rows=x == y ? h->rowsPad : h->rows;
+    const size_t words=h->lead*h->half;
+    if((rc=temporaries(h,words))) return rc;
+    const int la=h->dims-1;
+    auto leading=[&](const char *src, char *last) -> const char * {
+      const char *cur=src;
+      for(int a=0; a < la; ++a) {
+        size_t outer=1, inner=h->half;
+        for(int e=0; e < a; ++e) outer *= h->n[e];
+        for(int e=a+1; e < la; ++e) inner *= h->n[e];
+        char *dst=a == la-1 && last ? last : (char *) h->tmp[(a+1) % 2];
+        if(cur == dst) dst=(char *) h->tmp[a % 2];
+        if((rc=axis(h,a,h->sign,cur,inner,inner,outer,h->n[a]*inner,dst,inner,
+                    h->n[a]*inner,stream))) return NULL;
+        cur=dst;
+      }
+      return cur;
+    };
+    if(h->kind == FFTWPP_AMD_FFT_R2C) {
+      char *t0=(char *) h->tmp[0];
+      if((rc=fftwpp_amd_fft_execute(rows,x,t0,1.0,stream))) return rc;
+      if(!leading(t0,y)) return rc;
+      if(scale != 1.0)
+        rc=fftwpp_amd_internal_fft_map(dt,0,y,1,words,y,1,words,words,1,scale,stream);
+      return rc;
+    }
+    const char *cur=leading(x,NULL);
+    if(!cur) return rc;
+    return fftwpp_amd_fft_execute(rows,cur,y,scale,stream);
+  }
   if(h->kind != FFTWPP_AMD_FFT_C2C) {
     // through complex sequences of length n in two temporaries
     if((rc=temporaries(h,n*H))) return rc;

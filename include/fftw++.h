@@ -1,5 +1,6 @@
-/* fftw++.h -- the transform classes of FFTW++'s fftw++.h (fftw++.h:556-1444:
- * fft1d, mfft1d, fft2d, fft3d, rcfft1d, crfft1d, mrcfft1d, mcrfft1d) on the
+/* fftw++.h -- the transform classes of FFTW++'s fftw++.h (fftw++.h:556-1585:
+ * fft1d, mfft1d, fft2d, fft3d, rcfft1d, crfft1d, mrcfft1d, mcrfft1d, rcfft2d,
+ * crfft2d, rcfft3d, crfft3d) on the
  * MI355X, over the standalone FFT entry points of fftwpp_amd.h.
  *
  * Same constructors and fft / fftNormalized methods; the transforms run on
@@ -7,8 +8,8 @@
  * enqueued on the default stream and waited for) or host pointers (staged
  * through device buffers).  A NULL output means in place.  Differences:
  * the `threads` arguments and the in/out arrays passed to constructors are
- * accepted and unused (they chose FFTW plans); multidimensional transforms
- * are complex (rcfft2d/3d and the Transpose class are not provided).
+ * accepted and unused (they chose FFTW plans); the Transpose class is not
+ * provided.
  */
 #ifndef FFTWPP_AMD_FFTWPP_H
 #define FFTWPP_AMD_FFTWPP_H
@@ -37,7 +38,11 @@ protected:
     size_t inl=kind == FFTWPP_AMD_FFT_C2R ? half : len;
     size_t outl=kind == FFTWPP_AMD_FFT_R2C ? half : len;
     if(dims > 1) {
-      inWords=outWords=fftwpp_amd_fft_normalization(h);
+      // real transforms: lead x (n/2+1) complex words against lead x n reals
+      const size_t all=fftwpp_amd_fft_normalization(h), last=n[dims-1];
+      const size_t cw=all/last*(last/2+1);
+      inWords=kind == FFTWPP_AMD_FFT_C2R ? cw : all;
+      outWords=kind == FFTWPP_AMD_FFT_R2C ? cw : all;
       return;
     }
     if(istride == 0) istride=1;
@@ -58,8 +63,10 @@ protected:
       if(!rc) rc=fftwpp_amd_synchronize(NULL);
     } else {
       // host arrays: one device buffer per distinct array
-      const size_t ib=inWords*inSize, ob=outWords*outSize;
       const bool inplace=in == out;
+      // in place, real rows are padded to the complex row length
+      size_t ib=inWords*inSize, ob=outWords*outSize;
+      if(inplace) ib=ob=std::max(ib,ob);
       void *din=NULL, *dout=NULL;
       rc=fftwpp_amd_malloc(&din,inplace ? std::max(ib,ob) : ib);
       if(!rc && !inplace) rc=fftwpp_amd_malloc(&dout,ob);
@@ -155,6 +162,59 @@ public:
   }
   crfft1d(size_t nx, Complex *, double * =NULL, size_t =fftw::maxthreads) {
     create(FFTWPP_AMD_FFT_C2R,1,&nx,1,1,1,1,0,0);
+  }
+};
+
+// fftw++.h:1249-1388, 1436-1585: real 2D and 3D transforms of row-major
+// arrays, nx x ... x (nz/2+1) complex words against nx x ... x nz reals (in
+// place: real rows padded to 2 (nz/2+1) doubles, as FFTW stores them).
+class rcfft2d : public fftwPlan {
+public:
+  rcfft2d(size_t nx, size_t ny, Complex * =NULL, size_t =fftw::maxthreads) {
+    const size_t n[2]={nx,ny};
+    create(FFTWPP_AMD_FFT_R2C,2,n,-1,1,1,1,0,0);
+  }
+  rcfft2d(size_t nx, size_t ny, double *, Complex * =NULL, size_t =fftw::maxthreads) {
+    const size_t n[2]={nx,ny};
+    create(FFTWPP_AMD_FFT_R2C,2,n,-1,1,1,1,0,0);
+  }
+};
+
+class crfft2d : public fftwPlan {
+public:
+  crfft2d(size_t nx, size_t ny, double * =NULL, size_t =fftw::maxthreads) {
+    const size_t n[2]={nx,ny};
+    create(FFTWPP_AMD_FFT_C2R,2,n,1,1,1,1,0,0);
+  }
+  crfft2d(size_t nx, size_t ny, Complex *, double * =NULL, size_t =fftw::maxthreads) {
+    const size_t n[2]={nx,ny};
+    create(FFTWPP_AMD_FFT_C2R,2,n,1,1,1,1,0,0);
+  }
+};
+
+class rcfft3d : public fftwPlan {
+public:
+  rcfft3d(size_t nx, size_t ny, size_t nz, Complex * =NULL, size_t =fftw::maxthreads) {
+    const size_t n[3]={nx,ny,nz};
+    create(FFTWPP_AMD_FFT_R2C,3,n,-1,1,1,1,0,0);
+  }
+  rcfft3d(size_t nx, size_t ny, size_t nz, double *, Complex * =NULL,
+          size_t =fftw::maxthreads) {
+    const size_t n[3]={nx,ny,nz};
+    create(FFTWPP_AMD_FFT_R2C,3,n,-1,1,1,1,0,0);
+  }
+};
+
+class crfft3d : public fftwPlan {
+public:
+  crfft3d(size_t nx, size_t ny, size_t nz, double * =NULL, size_t =fftw::maxthreads) {
+    const size_t n[3]={nx,ny,nz};
+    create(FFTWPP_AMD_FFT_C2R,3,n,1,1,1,1,0,0);
+  }
+  crfft3d(size_t nx, size_t ny, size_t nz, Complex *, double * =NULL,
+          size_t =fftw::maxthreads) {
+    const size_t n[3]={nx,ny,nz};
+    create(FFTWPP_AMD_FFT_C2R,3,n,1,1,1,1,0,0);
   }
 };
 

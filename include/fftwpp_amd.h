@@ -346,12 +346,16 @@ int fftwpp_amd_batch_execute(fftwpp_amd_batch *h, void *out, const void *f,
  *   words, or (dims = 1) `howmany` transforms with element j of transform t
  *   at j*istride + t*idist (output: ostride, odist), as mfft1d
  *   (fftw++.h:827-896).
- * kind FFTWPP_AMD_FFT_R2C: sign -1, dims = 1: n real words in (istride,
- *   idist in reals), n/2+1 complex words out (ostride, odist), as
- *   rcfft1d/mrcfft1d (fftw++.h:917-955,1023-1072).
- * kind FFTWPP_AMD_FFT_C2R: sign +1, dims = 1: n/2+1 complex words of a
- *   Hermitian spectrum in, n real words out, as crfft1d/mcrfft1d
- *   (fftw++.h:957-975,1122-1171).
+ * kind FFTWPP_AMD_FFT_R2C: sign -1: n real words in (istride, idist in
+ *   reals), n/2+1 complex words out (ostride, odist), as rcfft1d/mrcfft1d
+ *   (fftw++.h:917-955,1023-1072); dims 2, 3: one transform of row-major
+ *   n[0] x ... x n[d-1] reals to n[0] x ... x (n[d-1]/2+1) complex words,
+ *   as rcfft2d/rcfft3d (fftw++.h:1249-1316,1436-1520).
+ * kind FFTWPP_AMD_FFT_C2R: sign +1, the inverse layouts (a Hermitian
+ *   spectrum in, reals out), as crfft1d/mcrfft1d/crfft2d/crfft3d
+ *   (fftw++.h:957-975,1122-1171,1318-1388,1522-1585).
+ * In place (in == out), real 2D/3D rows are padded to 2 (n[d-1]/2+1) reals
+ * as FFTW stores them.
  * A length n = m q runs as q residue passes of m-point sub-FFTs (m the
  * largest divisor of n the sub-FFT layer takes, at most 4096 here), writing
  * natural order through strides: a length with such a divisor and small q

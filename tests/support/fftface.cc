@@ -50,4 +50,27 @@ int testfft_multi_cxx(int dims, const size_t *n, double *in, double *out)
   return 0;
 }
 
+// rcfft2d / rcfft3d of the reals `in` (n[0] x ... x n[dims-1]) into `out`
+// (... x (n[dims-1]/2+1) complex), then crfft2d / crfft3d normalised of out
+// into `back`.
+int testfft_real_multi_cxx(int dims, const size_t *n, double *in, double *out, double *back)
+{
+  try {
+    if(dims == 2) {
+      rcfft2d F(n[0],n[1],in,(Complex *) out);
+      F.fft(in,(Complex *) out);
+      crfft2d B(n[0],n[1],(Complex *) out,back);
+      B.fftNormalized((Complex *) out,back);
+    } else {
+      rcfft3d F(n[0],n[1],n[2],in,(Complex *) out);
+      F.fft(in,(Complex *) out);
+      crfft3d B(n[0],n[1],n[2],(Complex *) out,back);
+      B.fftNormalized((Complex *) out,back);
+    }
+  } catch(const std::exception &) {
+    return 1;
+  }
+  return 0;
+}
+
 } // extern "C"

@@ -160,8 +160,8 @@ def test_fft_and_general_plan_argument_checks(built):
     assert L.fftwpp_amd_fft_create(ctypes.byref(h), 0, 1, _lib.sizes([8]), 2, 0, 1, 0, 0, 0, 0) == -1
     assert L.fftwpp_amd_fft_create(ctypes.byref(h), 0, 1, _lib.sizes([8]), -1, 5, 1, 0, 0, 0, 0) == -1
     assert L.fftwpp_amd_fft_create(ctypes.byref(h), 0, 1, _lib.sizes([0]), -1, 0, 1, 0, 0, 0, 0) == -1
-    # real multidimensional transforms are not provided
-    assert L.fftwpp_amd_fft_create(ctypes.byref(h), 1, 2, _lib.sizes([8, 8]), -1, 0, 1, 0, 0, 0, 0) == -3
+    # multidimensional transforms are single transforms
+    assert L.fftwpp_amd_fft_create(ctypes.byref(h), 1, 2, _lib.sizes([8, 8]), -1, 0, 2, 0, 0, 0, 0) == -3
     # a prime above the largest sub-FFT as an axis of a multidimensional
     # transform (1D lengths take Bluestein's algorithm instead)
     assert L.fftwpp_amd_fft_create(ctypes.byref(h), 0, 2, _lib.sizes([5003, 4]), -1, 0, 1, 0, 0, 0,

@@ -230,3 +230,51 @@ def test_cxx_face_host_arrays(gpu):
         y = x.copy()
         assert lib.testfft_multi_cxx(len(shape), ns, None, y.ctypes.data_as(dp)) == 0
         assert rel(y, np.fft.fftn(x)) < TOL64
+
+
+@pytest.mark.parametrize("shape", [(8, 10), (7, 9), (64, 48), (256, 256), (5, 6, 7), (64, 64, 64),
+                                   (3, 5003), (40, 30, 20)])
+def test_real_multi(gpu, shape):
+    # rcfft2d / rcfft3d against numpy.fft.rfftn, crfft2d / crfft3d against
+    # irfftn (unnormalised), out of place and in place (padded real rows)
+    import torch
+    from fftwpp_amd.torch_fft import DeviceFFT
+    rng = np.random.default_rng(sum(shape))
+    x = rng.uniform(-1, 1, shape)
+    X = np.fft.rfftn(x)
+    N = int(np.prod(shape))
+    hl = shape[-1] // 2 + 1
+    for p, tol in PRECISIONS:
+        real, cplx, npc, npr = types(p)
+        r2c = DeviceFFT(shape, -1, kind="r2c", dtype=real)
+        got = run(gpu, r2c, x.astype(npr), X.shape, cplx)
+        assert rel(got, X) < tol, (shape, p, "r2c")
+        c2r = DeviceFFT(shape, 1, kind="c2r", dtype=real)
+        back = run(gpu, c2r, X.astype(npc), shape, real)
+        assert rel(back, np.fft.irfftn(X, shape, axes=tuple(range(len(shape)))) * N) < tol, (shape, p, "c2r")
+        # in place: rows of 2 hl reals hold the data, then the spectrum
+        pad = np.zeros(shape[:-1] + (2 * hl,))
+        pad[..., :shape[-1]] = x
+        t = torch.tensor(pad.astype(npr), device=gpu)
+        tc = t.view(cplx)
+        r2c(t, tc)
+        torch.cuda.synchronize()
+        assert rel(tc.cpu().numpy(), X) < tol, (shape, p, "r2c in place")
+        c2r(tc, t, scale=1.0 / N)
+        torch.cuda.synchronize()
+        assert rel(t.cpu().numpy()[..., :shape[-1]], x) < tol, (shape, p, "c2r in place")
+
+
+def test_cxx_face_real_multi(gpu):
+    lib = ctypes.CDLL(TESTLIB)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rng = np.random.default_rng(23)
+    for shape in [(12, 20), (5, 6, 7)]:
+        x = rng.uniform(-1, 1, shape)
+        out = np.zeros(shape[:-1] + (shape[-1] // 2 + 1,), complex)
+        back = np.zeros(shape)
+        ns = (ctypes.c_size_t * 3)(*shape)
+        assert lib.testfft_real_multi_cxx(len(shape), ns, x.ctypes.data_as(dp),
+                                          out.ctypes.data_as(dp), back.ctypes.data_as(dp)) == 0
+        assert rel(out, np.fft.rfftn(x)) < TOL64
+        assert rel(back, x) < TOL64
