@@ -425,6 +425,38 @@ __global__ void fft_chirp_kernel(int op, const cplx<T> *x, int64_t xs, int64_t x
   y[b*yd+j*ys]=s*(x[b*xd+j]*chirp(j));
 }
 
+// out[c][r] = in[r][c] for a rows x cols matrix of W-double elements: 32 x 32
+// element tiles through LDS, rows read and written along their length.
+template<int W>
+__global__ void transpose_tile_kernel(const double *in, double *out, int64_t rows, int64_t cols)
+{
+  __shared__ double t[32][32*W+1];
+  const int64_t r0=(int64_t) blockIdx.y*32, c0=(int64_t) blockIdx.x*32;
+  for(int i=threadIdx.y; i < 32; i += 8) {
+    const int64_t r=r0+i;
+    for(int k=threadIdx.x; k < 32*W; k += 32)
+      if(r < rows && c0+k/W < cols) t[i][k]=in[(r*cols+c0)*W+k];
+  }
+  __syncthreads();
+  for(int i=threadIdx.y; i < 32; i += 8) {
+    const int64_t c=c0+i;
+    for(int k=threadIdx.x; k < 32*W; k += 32)
+      if(c < cols && r0+k/W < rows) out[(c*rows+r0)*W+k]=t[k/W][i*W+k%W];
+  }
+}
+
+// The same for elements of any w doubles, one double per thread in output
+// order.
+__global__ void transpose_words_kernel(const double *in, double *out, int64_t rows,
+                                       int64_t cols, int64_t w)
+{
+  const int64_t i=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  if(i >= rows*cols*w) return;
+  const int64_t e=i % w, rc=i/w;
+  const int64_t c=rc/rows, r=rc-c*rows;
+  out[i]=in[(r*cols+c)*w+e];
+}
+
 template<typename T>
 __global__ void scale_kernel(cplx<T> *f, int64_t n, T s)
 {
@@ -1438,6 +1470,30 @@ int fftwpp_amd_internal_fft_chirp(int dtype, int op, const void *x, size_t xs, s
                        (int64_t) ys,(int64_t) yd,(int64_t) n,(int64_t) len,(int64_t) nb,sign,
                        scale);
   return launch_check("fft_chirp_kernel");
+}
+
+int fftwpp_amd_transpose(size_t rows, size_t cols, size_t words, const void *in,
+                         void *out, void *stream)
+{
+  if(rows == 0 || cols == 0 || words == 0) return FFTWPP_AMD_OK;
+  if(!in || !out || in == out) return fail(FFTWPP_AMD_ERR_ARG,"transpose: distinct device arrays required");
+  hipStream_t st=(hipStream_t) stream;
+  if((words == 1 || words == 2) && (rows+31)/32 <= 65535 && (cols+31)/32 <= 0x7fffffff) {
+    dim3 grid((unsigned) ((cols+31)/32),(unsigned) ((rows+31)/32));
+    if(words == 1)
+      hipLaunchKernelGGL(transpose_tile_kernel<1>,grid,dim3(32,8),0,st,(const double *) in,
+                         (double *) out,(int64_t) rows,(int64_t) cols);
+    else
+      hipLaunchKernelGGL(transpose_tile_kernel<2>,grid,dim3(32,8),0,st,(const double *) in,
+                         (double *) out,(int64_t) rows,(int64_t) cols);
+    return launch_check("transpose_tile_kernel");
+  }
+  const size_t total=rows*cols*words;
+  if((total+255)/256 > 0xffffffffu) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"transpose too large");
+  hipLaunchKernelGGL(transpose_words_kernel,dim3((unsigned) ((total+255)/256)),dim3(256),0,st,
+                     (const double *) in,(double *) out,(int64_t) rows,(int64_t) cols,
+                     (int64_t) words);
+  return launch_check("transpose_words_kernel");
 }
 
 int fftwpp_amd_multiply(int dtype, int mult, void *F0, const void *F1,

@@ -1,6 +1,6 @@
 /* fftw++.h -- the transform classes of FFTW++'s fftw++.h (fftw++.h:556-1585:
  * fft1d, mfft1d, fft2d, fft3d, rcfft1d, crfft1d, mrcfft1d, mcrfft1d, rcfft2d,
- * crfft2d, rcfft3d, crfft3d) on the
+ * crfft2d, rcfft3d, crfft3d, Transpose) on the
  * MI355X, over the standalone FFT entry points of fftwpp_amd.h.
  *
  * Same constructors and fft / fftNormalized methods; the transforms run on
@@ -8,8 +8,7 @@
  * enqueued on the default stream and waited for) or host pointers (staged
  * through device buffers).  A NULL output means in place.  Differences:
  * the `threads` arguments and the in/out arrays passed to constructors are
- * accepted and unused (they chose FFTW plans); the Transpose class is not
- * provided.
+ * accepted and unused (they chose FFTW plans).
  */
 #ifndef FFTWPP_AMD_FFTWPP_H
 #define FFTWPP_AMD_FFTWPP_H
@@ -97,6 +96,50 @@ public:
   void fftNormalized(Complex *in, Complex *out=NULL) { run(in,out,Normalization()); }
   void fftNormalized(double *in, Complex *out=NULL) { run(in,out,Normalization()); }
   void fftNormalized(Complex *in, double *out) { run(in,out,Normalization()); }
+};
+
+// fftw++.h:436-497: out[c][r] = in[r][c] for a rows x cols matrix of
+// elements of `length` T's (T a multiple of a double in size); in place when
+// out is NULL or in (through a device buffer).
+class Transpose {
+  size_t rows, cols, words;
+  bool inplace;
+public:
+  template<class T>
+  Transpose(size_t rows, size_t cols, size_t length, T *in, T *out=NULL,
+            size_t =fftw::maxthreads) : rows(rows), cols(cols) {
+    static_assert(sizeof(T) % sizeof(double) == 0,"Transpose takes multiples of doubles");
+    words=length*(sizeof(T)/sizeof(double));
+    if(!out) out=in;
+    inplace=out == in;
+  }
+
+  template<class T>
+  void transpose(T *in, T *out=NULL) {
+    if(rows == 0 || cols == 0 || words == 0) return;
+    if(!out) out=in;
+    if(inplace != (out == in))
+      throw Error(FFTWPP_AMD_ERR_ARG,"Transpose: in-place mode differs from construction");
+    const size_t bytes=rows*cols*words*sizeof(double);
+    const bool dev=fftwpp_amd_is_device_pointer(in) && fftwpp_amd_is_device_pointer(out);
+    int rc=FFTWPP_AMD_OK;
+    if(dev && !inplace) {
+      rc=fftwpp_amd_transpose(rows,cols,words,in,out,NULL);
+    } else {
+      // a device copy of the input (host arrays, or an in-place transpose)
+      void *din=NULL, *dout=NULL;
+      rc=fftwpp_amd_malloc(&din,bytes);
+      if(!rc) rc=fftwpp_amd_memcpy(din,in,bytes,NULL);
+      if(!rc && !dev) rc=fftwpp_amd_malloc(&dout,bytes);
+      if(!rc) rc=fftwpp_amd_transpose(rows,cols,words,din,dev ? (void *) out : dout,NULL);
+      if(!rc && !dev) rc=fftwpp_amd_memcpy(out,dout,bytes,NULL);
+      if(!rc) rc=fftwpp_amd_synchronize(NULL);
+      if(din) fftwpp_amd_free(din);
+      if(dout) fftwpp_amd_free(dout);
+    }
+    if(!rc) rc=fftwpp_amd_synchronize(NULL);
+    if(rc) throw Error(rc,fftwpp_amd_last_error());
+  }
 };
 
 // fftw++.h:556-574: one complex transform of length nx.

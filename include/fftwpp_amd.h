@@ -380,6 +380,12 @@ int fftwpp_amd_fft_execute(fftwpp_amd_fft *h, const void *in, void *out,
 /* n[0]*...*n[dims-1]: the factor fftNormalized divides by. */
 size_t fftwpp_amd_fft_normalization(const fftwpp_amd_fft *h);
 
+/* out[c][r] = in[r][c] for a rows x cols matrix of elements of `words`
+ * doubles (device arrays, in != out): the Transpose class of fftw++.h:436-497
+ * (an FFTW rank-0 guru plan there). */
+int fftwpp_amd_transpose(size_t rows, size_t cols, size_t words, const void *in,
+                         void *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,4 +73,26 @@ int testfft_real_multi_cxx(int dims, const size_t *n, double *in, double *out, d
   return 0;
 }
 
+// Transpose of a rows x cols matrix of Complex (length 1) or of 3-double
+// elements: out-of-place from in to out, then in place on out (back to in).
+int testtranspose_cxx(size_t rows, size_t cols, int complexElements, double *in, double *out)
+{
+  try {
+    if(complexElements) {
+      Transpose T(rows,cols,1,(Complex *) in,(Complex *) out);
+      T.transpose((Complex *) in,(Complex *) out);
+      Transpose B(cols,rows,1,(Complex *) out);
+      B.transpose((Complex *) out);
+    } else {
+      Transpose T(rows,cols,3,in,out);
+      T.transpose(in,out);
+      Transpose B(cols,rows,3,out);
+      B.transpose(out);
+    }
+  } catch(const std::exception &) {
+    return 1;
+  }
+  return 0;
+}
+
 } // extern "C"

@@ -278,3 +278,31 @@ def test_cxx_face_real_multi(gpu):
                                           out.ctypes.data_as(dp), back.ctypes.data_as(dp)) == 0
         assert rel(out, np.fft.rfftn(x)) < TOL64
         assert rel(back, x) < TOL64
+
+
+@pytest.mark.parametrize("rows,cols,w", [(1, 1, 1), (33, 70, 1), (256, 256, 2), (1000, 3, 2),
+                                         (17, 45, 3), (64, 31, 5)])
+def test_transpose(gpu, rows, cols, w):
+    # fftwpp_amd_transpose (the Transpose class's plan): out[c][r] = in[r][c]
+    import torch
+    from fftwpp_amd._lib import check, lib
+    rng = np.random.default_rng(rows * cols + w)
+    x = rng.uniform(-1, 1, (rows, cols, w))
+    t = torch.tensor(x, device=gpu)
+    o = torch.zeros((cols, rows, w), dtype=torch.float64, device=gpu)
+    check(lib().fftwpp_amd_transpose(rows, cols, w, t.data_ptr(), o.data_ptr(), None))
+    torch.cuda.synchronize()
+    assert np.array_equal(o.cpu().numpy(), x.transpose(1, 0, 2))
+
+
+def test_cxx_transpose_host_arrays(gpu):
+    lib = ctypes.CDLL(TESTLIB)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rng = np.random.default_rng(29)
+    for rows, cols, cplx, w in [(20, 37, 1, 2), (9, 4, 0, 3)]:
+        x = rng.uniform(-1, 1, (rows, cols, w))
+        out = np.zeros((cols, rows, w))
+        assert lib.testtranspose_cxx(ctypes.c_size_t(rows), ctypes.c_size_t(cols), cplx,
+                                     x.ctypes.data_as(dp), out.ctypes.data_as(dp)) == 0
+        # out was transposed back in place: equal to the input again
+        assert np.array_equal(out.reshape(rows, cols, w), x)
