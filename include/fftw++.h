@@ -8,7 +8,8 @@
  * enqueued on the default stream and waited for) or host pointers (staged
  * through device buffers).  A NULL output means in place.  Differences:
  * the `threads` arguments and the in/out arrays passed to constructors are
- * accepted and unused (they chose FFTW plans).
+ * accepted and unused (they chose FFTW plans); the origin-shifted real
+ * transforms (fft0, fft0Normalized, deNyquist) are not provided.
  */
 #ifndef FFTWPP_AMD_FFTWPP_H
 #define FFTWPP_AMD_FFTWPP_H
