@@ -425,6 +425,38 @@ __global__ void fft_chirp_kernel(int op, const cplx<T> *x, int64_t xs, int64_t x
   y[b*yd+j*ys]=s*(x[b*xd+j]*chirp(j));
 }
 
+// Origin shifts of real 2D/3D transforms.  op 0: negate the reals of row i
+// (2D, i odd) or row (i, j) (3D, i + j odd) of an nx x [ny x] rows x rowlen
+// array; op 1: zero the Nyquist modes of a shifted 2D spectrum (nx x nyp:
+// row 0 for even nx, column nyp - 1 for even ny); op 2: the same in 3D
+// (nx x ny x nzp: plane 0, rows j = 0, column nzp - 1 for even nx, ny, nz).
+__global__ void fft_shift_kernel(int op, double *data, int64_t nx, int64_t ny, int64_t nz,
+                                 int64_t rowlen)
+{
+  const int64_t i=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  if(op == 0) {
+    const int64_t rows=nz ? nx*ny : nx;
+    if(i >= rows*rowlen) return;
+    const int64_t row=i/rowlen;
+    const int64_t par=nz ? row/ny+row % ny : row;
+    if(par & 1) data[i]=-data[i];
+    return;
+  }
+  cplx<double> *f=reinterpret_cast<cplx<double> *>(data);
+  if(op == 1) {
+    const int64_t nyp=ny/2+1;
+    if(i >= nx*nyp) return;
+    const int64_t x=i/nyp, y=i-x*nyp;
+    if((nx % 2 == 0 && x == 0) || (ny % 2 == 0 && y == nyp-1)) f[i]=mk(0.0,0.0);
+    return;
+  }
+  const int64_t nzp=nz/2+1;
+  if(i >= nx*ny*nzp) return;
+  const int64_t z=i % nzp, xy=i/nzp, y=xy % ny, x=xy/ny;
+  if((nx % 2 == 0 && x == 0) || (ny % 2 == 0 && y == 0) || (nz % 2 == 0 && z == nzp-1))
+    f[i]=mk(0.0,0.0);
+}
+
 // out[c][r] = in[r][c] for a rows x cols matrix of W-double elements: 32 x 32
 // element tiles through LDS, rows read and written along their length.
 template<int W>
@@ -1470,6 +1502,22 @@ int fftwpp_amd_internal_fft_chirp(int dtype, int op, const void *x, size_t xs, s
                        (int64_t) ys,(int64_t) yd,(int64_t) n,(int64_t) len,(int64_t) nb,sign,
                        scale);
   return launch_check("fft_chirp_kernel");
+}
+
+int fftwpp_amd_fft_shift(int op, double *data, size_t nx, size_t ny, size_t nz,
+                         size_t rowlen, void *stream)
+{
+  if(!data || op < 0 || op > 2 || nx == 0 || ny == 0 || (op == 2 && nz == 0))
+    return fail(FFTWPP_AMD_ERR_ARG,"fft_shift: bad arguments");
+  if(op == 0 && (nx % 2 || (nz && ny % 2)))
+    return fail(FFTWPP_AMD_ERR_ARG,"shift of the Fourier origin needs even nx (and even ny in 3D)");
+  size_t total=op == 0 ? (nz ? nx*ny : nx)*rowlen : op == 1 ? nx*(ny/2+1) : nx*ny*(nz/2+1);
+  if(total == 0) return FFTWPP_AMD_OK;
+  if((total+255)/256 > 0xffffffffu) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"fft_shift too large");
+  hipLaunchKernelGGL(fft_shift_kernel,dim3((unsigned) ((total+255)/256)),dim3(256),0,
+                     (hipStream_t) stream,op,data,(int64_t) nx,(int64_t) ny,(int64_t) nz,
+                     (int64_t) rowlen);
+  return launch_check("fft_shift_kernel");
 }
 
 int fftwpp_amd_transpose(size_t rows, size_t cols, size_t words, const void *in,

@@ -8,8 +8,8 @@
  * enqueued on the default stream and waited for) or host pointers (staged
  * through device buffers).  A NULL output means in place.  Differences:
  * the `threads` arguments and the in/out arrays passed to constructors are
- * accepted and unused (they chose FFTW plans); the origin-shifted real
- * transforms (fft0, fft0Normalized, deNyquist) are not provided.
+ * accepted and unused (they chose FFTW plans); fft0 on host arrays leaves
+ * the host input unshifted (the reference negates it in place).
  */
 #ifndef FFTWPP_AMD_FFTWPP_H
 #define FFTWPP_AMD_FFTWPP_H
@@ -54,12 +54,50 @@ protected:
     outWords=(howmany-1)*odist+(outl-1)*ostride+1;
   }
 
-  void run(const void *in, void *out, double scale) {
+  // Origin shift of the real 2D/3D classes (fftw++.h:144-222): side 1
+  // negates the input reals before an r2c transform, side 2 the output
+  // reals after a c2r one; sdims, sn: the real array's shape.
+  int shiftSide=0, sdims=0;
+  size_t sn[3]={0,0,0};
+
+  void origin(int side, int dims, const size_t *n) {
+    shiftSide=side;
+    sdims=dims;
+    for(int d=0; d < dims; ++d) sn[d]=n[d];
+  }
+
+  int shiftReals(void *data, bool inplace) {
+    const size_t last=sn[sdims-1];
+    const size_t rowlen=inplace ? 2*(last/2+1) : last;
+    return fftwpp_amd_fft_shift(0,(double *) data,sn[0],sdims == 2 ? last : sn[1],
+                                sdims == 3 ? last : 0,rowlen,NULL);
+  }
+
+  void denyquist(Complex *f) {
+    const size_t last=sn[sdims-1];
+    const size_t bytes=(sdims == 2 ? sn[0] : sn[0]*sn[1])*(last/2+1)*sizeof(Complex);
+    const bool dev=fftwpp_amd_is_device_pointer(f);
+    void *d=f;
+    int rc=dev ? FFTWPP_AMD_OK : fftwpp_amd_malloc(&d,bytes);
+    if(!rc && !dev) rc=fftwpp_amd_memcpy(d,f,bytes,NULL);
+    if(!rc) rc=fftwpp_amd_fft_shift(sdims == 2 ? 1 : 2,(double *) d,sn[0],
+                                    sdims == 2 ? last : sn[1],sdims == 3 ? last : 0,0,NULL);
+    if(!rc && !dev) rc=fftwpp_amd_memcpy(f,d,bytes,NULL);
+    if(!rc) rc=fftwpp_amd_synchronize(NULL);
+    if(!dev && d) fftwpp_amd_free(d);
+    if(rc) throw Error(rc,fftwpp_amd_last_error());
+  }
+
+  void run(const void *in, void *out, double scale, bool shift=false) {
     if(!out) out=(void *) in;
     const bool dev=fftwpp_amd_is_device_pointer(in) && fftwpp_amd_is_device_pointer(out);
-    int rc;
+    shift=shift && shiftSide;
+    int rc=FFTWPP_AMD_OK;
     if(dev) {
-      rc=fftwpp_amd_fft_execute(h,in,out,scale,NULL);
+      // on device arrays the input shift is made in place, as the reference does
+      if(shift && shiftSide == 1) rc=shiftReals((void *) in,in == out);
+      if(!rc) rc=fftwpp_amd_fft_execute(h,in,out,scale,NULL);
+      if(!rc && shift && shiftSide == 2) rc=shiftReals(out,in == out);
       if(!rc) rc=fftwpp_amd_synchronize(NULL);
     } else {
       // host arrays: one device buffer per distinct array
@@ -72,7 +110,11 @@ protected:
       if(!rc && !inplace) rc=fftwpp_amd_malloc(&dout,ob);
       if(inplace) dout=din;
       if(!rc) rc=fftwpp_amd_memcpy(din,in,ib,NULL);
+      // host arrays: the shift applies to the device copy (the host input is
+      // left as it was)
+      if(!rc && shift && shiftSide == 1) rc=shiftReals(din,inplace);
       if(!rc) rc=fftwpp_amd_fft_execute(h,din,dout,scale,NULL);
+      if(!rc && shift && shiftSide == 2) rc=shiftReals(dout,inplace);
       if(!rc) rc=fftwpp_amd_memcpy(out,dout,ob,NULL);
       if(!rc) rc=fftwpp_amd_synchronize(NULL);
       if(din) fftwpp_amd_free(din);
@@ -97,6 +139,14 @@ public:
   void fftNormalized(Complex *in, Complex *out=NULL) { run(in,out,Normalization()); }
   void fftNormalized(double *in, Complex *out=NULL) { run(in,out,Normalization()); }
   void fftNormalized(Complex *in, double *out) { run(in,out,Normalization()); }
+  // With the Fourier origin shifted to (nx/2, 0) in 2D or (nx/2, ny/2, 0) in
+  // 3D for the real 2D/3D classes; the plain transforms for the others.
+  void fft0(Complex *in, Complex *out=NULL) { run(in,out,1.0,true); }
+  void fft0(double *in, Complex *out=NULL) { run(in,out,1.0,true); }
+  void fft0(Complex *in, double *out) { run(in,out,1.0,true); }
+  void fft0Normalized(Complex *in, Complex *out=NULL) { run(in,out,Normalization(),true); }
+  void fft0Normalized(Complex *in, double *out) { run(in,out,Normalization(),true); }
+  void fft0Normalized(double *in, Complex *out) { run(in,out,Normalization(),true); }
 };
 
 // fftw++.h:436-497: out[c][r] = in[r][c] for a rows x cols matrix of
@@ -217,11 +267,16 @@ public:
   rcfft2d(size_t nx, size_t ny, Complex * =NULL, size_t =fftw::maxthreads) {
     const size_t n[2]={nx,ny};
     create(FFTWPP_AMD_FFT_R2C,2,n,-1,1,1,1,0,0);
+    origin(1,2,n);
   }
   rcfft2d(size_t nx, size_t ny, double *, Complex * =NULL, size_t =fftw::maxthreads) {
     const size_t n[2]={nx,ny};
     create(FFTWPP_AMD_FFT_R2C,2,n,-1,1,1,1,0,0);
+    origin(1,2,n);
   }
+  // fftw++.h:1342-1360 (2D), 1466-1495 (3D): zero the Nyquist modes of a
+  // shifted spectrum
+  void deNyquist(Complex *f) { denyquist(f); }
 };
 
 class crfft2d : public fftwPlan {
@@ -229,11 +284,16 @@ public:
   crfft2d(size_t nx, size_t ny, double * =NULL, size_t =fftw::maxthreads) {
     const size_t n[2]={nx,ny};
     create(FFTWPP_AMD_FFT_C2R,2,n,1,1,1,1,0,0);
+    origin(2,2,n);
   }
   crfft2d(size_t nx, size_t ny, Complex *, double * =NULL, size_t =fftw::maxthreads) {
     const size_t n[2]={nx,ny};
     create(FFTWPP_AMD_FFT_C2R,2,n,1,1,1,1,0,0);
+    origin(2,2,n);
   }
+  // fftw++.h:1342-1360 (2D), 1466-1495 (3D): zero the Nyquist modes of a
+  // shifted spectrum
+  void deNyquist(Complex *f) { denyquist(f); }
 };
 
 class rcfft3d : public fftwPlan {
@@ -241,12 +301,17 @@ public:
   rcfft3d(size_t nx, size_t ny, size_t nz, Complex * =NULL, size_t =fftw::maxthreads) {
     const size_t n[3]={nx,ny,nz};
     create(FFTWPP_AMD_FFT_R2C,3,n,-1,1,1,1,0,0);
+    origin(1,3,n);
   }
   rcfft3d(size_t nx, size_t ny, size_t nz, double *, Complex * =NULL,
           size_t =fftw::maxthreads) {
     const size_t n[3]={nx,ny,nz};
     create(FFTWPP_AMD_FFT_R2C,3,n,-1,1,1,1,0,0);
+    origin(1,3,n);
   }
+  // fftw++.h:1342-1360 (2D), 1466-1495 (3D): zero the Nyquist modes of a
+  // shifted spectrum
+  void deNyquist(Complex *f) { denyquist(f); }
 };
 
 class crfft3d : public fftwPlan {
@@ -254,12 +319,17 @@ public:
   crfft3d(size_t nx, size_t ny, size_t nz, double * =NULL, size_t =fftw::maxthreads) {
     const size_t n[3]={nx,ny,nz};
     create(FFTWPP_AMD_FFT_C2R,3,n,1,1,1,1,0,0);
+    origin(2,3,n);
   }
   crfft3d(size_t nx, size_t ny, size_t nz, Complex *, double * =NULL,
           size_t =fftw::maxthreads) {
     const size_t n[3]={nx,ny,nz};
     create(FFTWPP_AMD_FFT_C2R,3,n,1,1,1,1,0,0);
+    origin(2,3,n);
   }
+  // fftw++.h:1342-1360 (2D), 1466-1495 (3D): zero the Nyquist modes of a
+  // shifted spectrum
+  void deNyquist(Complex *f) { denyquist(f); }
 };
 
 // fftw++.h:1023-1072, 1122-1171: M real-to-complex / complex-to-real

@@ -380,6 +380,15 @@ int fftwpp_amd_fft_execute(fftwpp_amd_fft *h, const void *in, void *out,
 /* n[0]*...*n[dims-1]: the factor fftNormalized divides by. */
 size_t fftwpp_amd_fft_normalization(const fftwpp_amd_fft *h);
 
+/* Origin-shifted real 2D/3D transforms (FP64 device arrays).  op 0: the
+ * Shift of fftw++.h:144-222, negating the reals of odd rows i (2D, nx x
+ * rowlen) or of rows (i, j) with i + j odd (3D, nx x ny x rowlen), rowlen =
+ * ny or nz, or 2 (n/2+1) for padded in-place rows; nx (and ny in 3D) even.
+ * op 1 / 2: deNyquist of a shifted 2D (nx x ny/2+1) / 3D (nx x ny x nz/2+1)
+ * spectrum, fftw++.h:1342-1360,1466-1495. */
+int fftwpp_amd_fft_shift(int op, double *data, size_t nx, size_t ny, size_t nz,
+                         size_t rowlen, void *stream);
+
 /* out[c][r] = in[r][c] for a rows x cols matrix of elements of `words`
  * doubles (device arrays, in != out): the Transpose class of fftw++.h:436-497
  * (an FFTW rank-0 guru plan there). */

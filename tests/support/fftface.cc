@@ -73,6 +73,34 @@ int testfft_real_multi_cxx(int dims, const size_t *n, double *in, double *out, d
   return 0;
 }
 
+// Origin-shifted real transforms on host arrays: out = rcfft{2,3}d fft0 of
+// in; back = crfft{2,3}d fft0Normalized of out (= in); nyq = deNyquist of a
+// copy of out.
+int testfft0_cxx(int dims, const size_t *n, double *in, double *out, double *back, double *nyq)
+{
+  try {
+    const size_t words=(dims == 2 ? n[0] : n[0]*n[1])*(n[dims-1]/2+1);
+    if(dims == 2) {
+      rcfft2d F(n[0],n[1],in,(Complex *) out);
+      F.fft0(in,(Complex *) out);
+      crfft2d B(n[0],n[1],(Complex *) out,back);
+      B.fft0Normalized((Complex *) out,back);
+      memcpy(nyq,out,words*sizeof(Complex));
+      F.deNyquist((Complex *) nyq);
+    } else {
+      rcfft3d F(n[0],n[1],n[2],in,(Complex *) out);
+      F.fft0(in,(Complex *) out);
+      crfft3d B(n[0],n[1],n[2],(Complex *) out,back);
+      B.fft0Normalized((Complex *) out,back);
+      memcpy(nyq,out,words*sizeof(Complex));
+      F.deNyquist((Complex *) nyq);
+    }
+  } catch(const std::exception &) {
+    return 1;
+  }
+  return 0;
+}
+
 // Transpose of a rows x cols matrix of Complex (length 1) or of 3-double
 // elements: out-of-place from in to out, then in place on out (back to in).
 int testtranspose_cxx(size_t rows, size_t cols, int complexElements, double *in, double *out)

@@ -306,3 +306,35 @@ def test_cxx_transpose_host_arrays(gpu):
                                      x.ctypes.data_as(dp), out.ctypes.data_as(dp)) == 0
         # out was transposed back in place: equal to the input again
         assert np.array_equal(out.reshape(rows, cols, w), x)
+
+
+@pytest.mark.parametrize("shape", [(8, 10), (12, 7), (6, 4, 10), (4, 8, 5)])
+def test_cxx_fft0_denyquist(gpu, shape):
+    # rcfft2d/3d fft0: the transform of x (-1)^i (2D) or x (-1)^(i+j) (3D)
+    # (fftw++.h:144-222); crfft fft0Normalized inverts it; deNyquist zeroes
+    # the shifted Nyquist modes (fftw++.h:1342-1360, 1466-1495)
+    lib = ctypes.CDLL(TESTLIB)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rng = np.random.default_rng(sum(shape))
+    x = rng.uniform(-1, 1, shape)
+    ix = np.indices(shape)
+    sign = (-1.0) ** (ix[0] if len(shape) == 2 else ix[0] + ix[1])
+    half = shape[:-1] + (shape[-1] // 2 + 1,)
+    out = np.zeros(half, complex)
+    back = np.zeros(shape)
+    nyq = np.zeros(half, complex)
+    ns = (ctypes.c_size_t * 3)(*shape)
+    assert lib.testfft0_cxx(len(shape), ns, x.ctypes.data_as(dp), out.ctypes.data_as(dp),
+                            back.ctypes.data_as(dp), nyq.ctypes.data_as(dp)) == 0
+    X = np.fft.rfftn(x * sign)
+    assert rel(out, X) < TOL64
+    assert rel(back, x) < TOL64
+    want = X.copy()
+    if shape[0] % 2 == 0:
+        want[0] = 0
+    if len(shape) == 3 and shape[1] % 2 == 0:
+        want[:, 0] = 0
+    if shape[-1] % 2 == 0:
+        want[..., -1] = 0
+    assert np.array_equal(nyq == 0, want == 0)
+    assert rel(nyq, want) < TOL64
