@@ -1,18 +1,20 @@
 /* fftwpp_amd.h -- C ABI of the MI355X (gfx950) hybrid dealiased convolution
  * library libfftwpp_amd.so.
  *
- * Two layers are declared here:
+ * Three layers are declared here:
  *
  *   1. The kernel layer: padded residue transforms of one dimension, the
  *      fused row convolutions, the device multipliers and the Hermitian
  *      symmetrisers.  All pointers are device pointers; `stream` is a
  *      hipStream_t passed as void* (NULL = the default stream).  The C++
  *      drop-in classes (convolve.h) are written against this layer only.
- *   2. The convolution layer: whole 1D/2D/3D convolutions (complex or
- *      centred Hermitian, FP64 or FP32) on device or host pointers, the
- *      adjoint (correlation) entry points, and the batched entry points.
- *      These extend the reference C wrapper (cfftw++.h) with device pointers,
- *      a precision choice, status codes, batching and adjoints.
+ *   2. The convolution layer: whole 1D/2D/3D convolutions (complex,
+ *      centred, centred Hermitian or real, FP64 or FP32) on device pointers,
+ *      the adjoint (correlation) entry points, user device multipliers and
+ *      the batched entry points.  These extend the reference C wrapper
+ *      (cfftw++.h) with device pointers, a precision choice, status codes,
+ *      batching, general multipliers and adjoints.
+ *   3. Standalone FFTs and transposes (the transform classes of fftw++.h).
  *
  * Every function returns 0 (FFTWPP_AMD_OK) on success or a negative status
  * code; fftwpp_amd_last_error() describes the last failure of the calling
