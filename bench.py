@@ -392,9 +392,12 @@ def bench_user_multiplier(torch, dev, steps, warmup):
     err = float(torch.linalg.norm(a - h) / torch.linalg.norm(h))
     del plan, a, f, g, h
     torch.cuda.empty_cache()
+    bytes_alg = CONV_UNITS * 16 * L ** 3
     return {"workload": "3D complex 256^3 -> 512^3, FP64, user device multiplier F0 *= F1",
             "conv_per_s": round(1000.0 / ms, 2), "ms_per_conv": round(ms, 4),
-            "rel_l2_vs_fused": err}
+            "achieved_GBs": round(bytes_alg / (ms * 1e-3) / 1e9, 1),
+            "frac_of_hbm_peak": round(bytes_alg / (ms * 1e-3) / 1e9 / PEAK_GBS, 4),
+            "bytes_alg": bytes_alg, "rel_l2_vs_fused": err}
 
 
 def bench_fft(torch, dev, steps, warmup):
