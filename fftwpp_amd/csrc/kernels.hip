@@ -693,7 +693,7 @@ static int colreg_launch(const DimArgs &a, bool fwd, const void *in,
                          size_t s_in, size_t bsi, void *out, size_t s_out,
                          size_t bso, size_t C, size_t nbatch, size_t r0,
                          size_t nres, int accumulate, double scale,
-                         hipStream_t st)
+                         size_t ics, size_t ocs, hipStream_t st)
 {
   typedef ColReg<T,M> CR;
   const size_t lds=M*sizeof(cplx<T>)+CR::TC*CR::CS*sizeof(T);
@@ -705,12 +705,14 @@ static int colreg_launch(const DimArgs &a, bool fwd, const void *in,
       if((rc=allow_lds(colfwd_reg_kernel<T,M,2>,lds))) return rc;
       hipLaunchKernelGGL((colfwd_reg_kernel<T,M,2>),grid,dim3(CR::NT),lds,st,a,
                          (const cplx<T> *) in,(int64_t) s_in,(int64_t) C,(int64_t) bsi,
-                         (cplx<T> *) out,(int64_t) s_out,(int64_t) bso,(int64_t) r0,(int64_t) nres);
+                         (cplx<T> *) out,(int64_t) s_out,(int64_t) bso,(int64_t) r0,(int64_t) nres,
+                         (int64_t) ics,(int64_t) ocs);
     } else {
       if((rc=allow_lds(colfwd_reg_kernel<T,M,1>,lds))) return rc;
       hipLaunchKernelGGL((colfwd_reg_kernel<T,M,1>),grid,dim3(CR::NT),lds,st,a,
                          (const cplx<T> *) in,(int64_t) s_in,(int64_t) C,(int64_t) bsi,
-                         (cplx<T> *) out,(int64_t) s_out,(int64_t) bso,(int64_t) r0,(int64_t) nres);
+                         (cplx<T> *) out,(int64_t) s_out,(int64_t) bso,(int64_t) r0,(int64_t) nres,
+                         (int64_t) ics,(int64_t) ocs);
     }
     return launch_check("colfwd_reg_kernel");
   }
@@ -719,13 +721,13 @@ static int colreg_launch(const DimArgs &a, bool fwd, const void *in,
     hipLaunchKernelGGL((colbwd_reg_kernel<T,M,2>),grid,dim3(CR::NT),lds,st,a,
                        (const cplx<T> *) in,(int64_t) s_in,(int64_t) bsi,(cplx<T> *) out,
                        (int64_t) s_out,(int64_t) C,(int64_t) bso,(int64_t) r0,(int64_t) nres,
-                       accumulate,(T) scale);
+                       accumulate,(T) scale,(int64_t) ics,(int64_t) ocs);
   } else {
     if((rc=allow_lds(colbwd_reg_kernel<T,M,1>,lds))) return rc;
     hipLaunchKernelGGL((colbwd_reg_kernel<T,M,1>),grid,dim3(CR::NT),lds,st,a,
                        (const cplx<T> *) in,(int64_t) s_in,(int64_t) bsi,(cplx<T> *) out,
                        (int64_t) s_out,(int64_t) C,(int64_t) bso,(int64_t) r0,(int64_t) nres,
-                       accumulate,(T) scale);
+                       accumulate,(T) scale,(int64_t) ics,(int64_t) ocs);
   }
   return launch_check("colbwd_reg_kernel");
 }
@@ -734,9 +736,9 @@ template<typename T>
 static int colreg_t(const DimArgs &a, bool fwd, const void *in, size_t s_in,
                     size_t bsi, void *out, size_t s_out, size_t bso, size_t C,
                     size_t nbatch, size_t r0, size_t nres, int accumulate,
-                    double scale, hipStream_t st)
+                    double scale, size_t ics, size_t ocs, hipStream_t st)
 {
-#define FFTWPP_AMD_COLREG(MM) case MM: return colreg_launch<T,MM>(a,fwd,in,s_in,bsi,out,s_out,bso,C,nbatch,r0,nres,accumulate,scale,st);
+#define FFTWPP_AMD_COLREG(MM) case MM: return colreg_launch<T,MM>(a,fwd,in,s_in,bsi,out,s_out,bso,C,nbatch,r0,nres,accumulate,scale,ics,ocs,st);
   switch(a.m) {
   FFTWPP_AMD_COLREG(16) FFTWPP_AMD_COLREG(32) FFTWPP_AMD_COLREG(64)
   FFTWPP_AMD_COLREG(128) FFTWPP_AMD_COLREG(256) FFTWPP_AMD_COLREG(512)
@@ -753,9 +755,16 @@ static int fwd_many_t(const fftwpp_amd_dim *d, const void *f, size_t S,
                       hipStream_t st)
 {
   const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
+  // single columns of many batch entries: the entries become the columns
+  // (column strides ics, ocs: the row strides)
+  size_t ics=1, ocs=1;
+  if(C == 1 && nbatch > 1) {
+    C=nbatch; ics=bsi; ocs=bso;
+    nbatch=1; bsi=bso=0;
+  }
   // the 256-point column kernel fills its twiddle tables per block: taken
-  // when a block has a full tile of columns (single rows go to the LDS path)
-  if(C >= (size_t) Col256<T>::TC && fast256(d,S,SF)) {
+  // for interleaved columns with a full tile of them
+  if(ics == 1 && ocs == 1 && C >= (size_t) Col256<T>::TC && fast256(d,S,SF)) {
     constexpr int TC=Col256<T>::TC;
     dim3 grid((unsigned) ((C+TC-1)/TC),(unsigned) nbatch);
     size_t lds=col256_lds<T>(nres);
@@ -767,19 +776,13 @@ static int fwd_many_t(const fftwpp_amd_dim *d, const void *f, size_t S,
                        (int64_t) r0,(int64_t) nres,(const cplx<T> *) a.zN,(int64_t) d->q);
     return launch_check("colfwd256_kernel");
   }
-  // register columns: tiles of several columns below m = 2048, so single
-  // rows of shorter lengths go to the LDS path
+  // register columns: tiles of several columns below m = 2048, so a single
+  // short column goes to the LDS path
   if(cols_reg(d) && (C >= 8 || d->m >= 2048))
-    return colreg_t<T>(a,true,f,S,bsi,F,SF,bso,C,nbatch,r0,nres,0,1.0,st);
+    return colreg_t<T>(a,true,f,S,bsi,F,SF,bso,C,nbatch,r0,nres,0,1.0,ics,ocs,st);
   size_t elt=sizeof(cplx<T>);
   size_t per=2*(d->m+1)*elt;
   if(per > lds_max()) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"m too large for the LDS transform");
-  // single columns of many batch entries: the entries become the columns
-  size_t ics=1, ocs=1;
-  if(C == 1 && nbatch > 1) {
-    C=nbatch; ics=bsi; ocs=bso;
-    nbatch=1; bsi=bso=0;
-  }
   size_t TC=1;
   while(TC < 16 && TC < C && 2*TC*per <= lds_budget()) TC *= 2;
   size_t lds=TC*per;
@@ -800,7 +803,13 @@ static int bwd_many_t(const fftwpp_amd_dim *d, const void *F, size_t SF,
                       int accumulate, double scale, hipStream_t st)
 {
   const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
-  if(C >= (size_t) Col256<T>::TC && fast256(d,S,SF)) {
+  // single columns of many batch entries: the entries become the columns
+  size_t ics=1, ocs=1;
+  if(C == 1 && nbatch > 1) {
+    C=nbatch; ics=bsi; ocs=bso;
+    nbatch=1; bsi=bso=0;
+  }
+  if(ics == 1 && ocs == 1 && C >= (size_t) Col256<T>::TC && fast256(d,S,SF)) {
     constexpr int TC=Col256<T>::TC;
     dim3 grid((unsigned) ((C+TC-1)/TC),(unsigned) nbatch);
     size_t lds=col256_lds<T>(nres);
@@ -814,16 +823,10 @@ static int bwd_many_t(const fftwpp_amd_dim *d, const void *F, size_t SF,
     return launch_check("colbwd256_kernel");
   }
   if(cols_reg(d) && (C >= 8 || d->m >= 2048))
-    return colreg_t<T>(a,false,F,SF,bsi,f,S,bso,C,nbatch,r0,nres,accumulate,scale,st);
+    return colreg_t<T>(a,false,F,SF,bsi,f,S,bso,C,nbatch,r0,nres,accumulate,scale,ics,ocs,st);
   size_t elt=sizeof(cplx<T>);
   size_t per=2*(d->m+1)*elt;
   if(per > lds_max()) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"m too large for the LDS transform");
-  // single columns of many batch entries: the entries become the columns
-  size_t ics=1, ocs=1;
-  if(C == 1 && nbatch > 1) {
-    C=nbatch; ics=bsi; ocs=bso;
-    nbatch=1; bsi=bso=0;
-  }
   size_t TC=1;
   while(TC < 16 && TC < C && 2*TC*per <= lds_budget()) TC *= 2;
   size_t lds=TC*per;

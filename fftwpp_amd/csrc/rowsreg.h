@@ -393,12 +393,14 @@ template<typename T, int M> struct ColReg {
   static __device__ __forceinline__ int xm(int c) { return (c*BETA) & (M < 32 ? M-1 : 31); }
 };
 
-// F[b][(r-r0) M + s][c] = sum_j f[b][j][c] w_N^{(j+k0)(q s + r)}, j < L.
+// F[b][(r-r0) M + s][c] = sum_j f[b][j][c] w_N^{(j+k0)(q s + r)}, j < L;
+// column c starts at c*ics of f and c*ocs of F (unit strides for
+// interleaved columns, row strides for contiguous rows taken as columns).
 template<typename T, int M, int P2>
 __global__ void __launch_bounds__(512)
 colfwd_reg_kernel(DimArgs d, const cplx<T> *f, int64_t S, int64_t C,
                   int64_t bsi, cplx<T> *F, int64_t SF, int64_t bso, int64_t r0,
-                  int64_t nres)
+                  int64_t nres, int64_t ics, int64_t ocs)
 {
   typedef ColReg<T,M> CR;
   constexpr int V=CR::V, TPS=CR::TPS, TC=CR::TC;
@@ -412,8 +414,8 @@ colfwd_reg_kernel(DimArgs d, const cplx<T> *f, int64_t S, int64_t C,
   const int xm=CR::xm(c);
   const int64_t c0=xcd_tile(blockIdx.x,gridDim.x)*TC;
   const bool ok=c0+c < C;
-  const cplx<T> *fb=f+(int64_t) blockIdx.y*bsi+c0+c;
-  cplx<T> *Fb=F+(int64_t) blockIdx.y*bso+c0+c;
+  const cplx<T> *fb=f+(int64_t) blockIdx.y*bsi+(c0+c)*ics;
+  cplx<T> *Fb=F+(int64_t) blockIdx.y*bso+(c0+c)*ocs;
   const int N=(int) d.N, L=(int) d.L, k0=(int) d.k0;
   // the column, read once: element j = j0 + M i with j0 = (sigma - k0) mod M
   cplx<T> raw[P2][V];
@@ -453,7 +455,7 @@ template<typename T, int M, int P2>
 __global__ void __launch_bounds__(512)
 colbwd_reg_kernel(DimArgs d, const cplx<T> *F, int64_t SF, int64_t bsi,
                   cplx<T> *f, int64_t S, int64_t C, int64_t bso, int64_t r0,
-                  int64_t nres, int accumulate, T scale)
+                  int64_t nres, int accumulate, T scale, int64_t ics, int64_t ocs)
 {
   typedef ColReg<T,M> CR;
   constexpr int V=CR::V, TPS=CR::TPS, TC=CR::TC;
@@ -467,8 +469,8 @@ colbwd_reg_kernel(DimArgs d, const cplx<T> *F, int64_t SF, int64_t bsi,
   const int xm=CR::xm(c);
   const int64_t c0=xcd_tile(blockIdx.x,gridDim.x)*TC;
   const bool ok=c0+c < C;
-  const cplx<T> *Fb=F+(int64_t) blockIdx.y*bsi+c0+c;
-  cplx<T> *fb=f+(int64_t) blockIdx.y*bso+c0+c;
+  const cplx<T> *Fb=F+(int64_t) blockIdx.y*bsi+(c0+c)*ics;
+  cplx<T> *fb=f+(int64_t) blockIdx.y*bso+(c0+c)*ocs;
   const int N=(int) d.N, L=(int) d.L, k0=(int) d.k0;
   cplx<T> acc[P2][V];
 #pragma unroll
