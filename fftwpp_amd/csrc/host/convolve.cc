@@ -694,8 +694,9 @@ static void requireBinary(size_t A, size_t B, int code)
 // Convolution::operate, convolve.h:1120-1135): every dimension expanded to
 // all its residues, the multiplier called once on the padded arrays.
 // ------------------------------------------------------------------------
-// forward_many/backward_many take one grid row per batch entry: batches are
-// split to stay within the grid's y extent.
+// forward_many/backward_many take one grid row per batch entry: batches of
+// several columns are split to stay within the grid's y extent (single
+// columns of many entries run as the columns of one launch).
 static const size_t maxBatch=65535;
 
 void generalConvolution(fftBase *const *dims, size_t nd, size_t A, size_t B,
@@ -731,9 +732,10 @@ void generalConvolution(fftBase *const *dims, size_t nd, size_t A, size_t B,
       const size_t out=outer(d), in=inner(d);
       const char *src=d == 0 ? (const char *) f[a] : P[d-1];
       char *dst=d+1 == nd ? (char *) F[a] : P[d];
-      for(size_t b0=0; b0 < out; b0 += maxBatch)
+      const size_t step=in == 1 ? out : maxBatch;
+      for(size_t b0=0; b0 < out; b0 += step)
         check(fftwpp_amd_forward_many(dims[d]->dev,dtype,src+b0*Ld[d]*in*E,in,in,
-                                      std::min(maxBatch,out-b0),Ld[d]*in,
+                                      std::min(step,out-b0),Ld[d]*in,
                                       dst+b0*Nd[d]*in*E,in,Nd[d]*in,0,dims[d]->q,
                                       stream),"user multiplier: forward");
     }
@@ -759,9 +761,10 @@ void generalConvolution(fftBase *const *dims, size_t nd, size_t A, size_t B,
       const size_t out=outer(d), in=inner(d);
       const char *src=d+1 == nd ? (const char *) F[b] : P[d];
       char *dst=d == 0 ? (char *) f[b] : P[d-1];
-      for(size_t b0=0; b0 < out; b0 += maxBatch)
+      const size_t step=in == 1 ? out : maxBatch;
+      for(size_t b0=0; b0 < out; b0 += step)
         check(fftwpp_amd_backward_many(dims[d]->dev,dtype,src+b0*Nd[d]*in*E,in,Nd[d]*in,
-                                       dst+b0*Ld[d]*in*E,in,in,std::min(maxBatch,out-b0),
+                                       dst+b0*Ld[d]*in*E,in,in,std::min(step,out-b0),
                                        Ld[d]*in,0,dims[d]->q,0,d == 0 ? scale : 1.0,
                                        stream),"user multiplier: backward");
     }

@@ -62,7 +62,8 @@ namespace {
 
 const size_t maxSub=4096;     // largest sub-FFT size m
 const size_t maxPasses=4096;  // largest residue count q = n/m
-const size_t maxBatch=65535;  // grid rows per launch
+const size_t maxBatch=65535;  // grid rows per launch (single columns of many
+                              // entries run as columns and need no split)
 
 int error(int rc, const char *msg)
 {
@@ -87,8 +88,9 @@ int axis(const fftwpp_amd_fft *h, int a, int sign, const char *x, size_t xS,
          void *stream)
 {
   const size_t q=h->q[a], E=elt(h->dtype);
-  for(size_t b0=0; b0 < nb; b0 += maxBatch) {
-    const size_t cnt=std::min(maxBatch,nb-b0);
+  const size_t step=C == 1 ? nb : maxBatch;
+  for(size_t b0=0; b0 < nb; b0 += step) {
+    const size_t cnt=std::min(step,nb-b0);
     const char *xx=x+b0*xb*E;
     char *yy=y+b0*yb*E;
     for(size_t r=0; r < q; ++r) {
@@ -108,8 +110,9 @@ int single(const fftwpp_amd_fft *h, fftwpp_amd_dim *d, int sign, const char *x, 
            size_t C, size_t nb, size_t xb, char *y, size_t yS, size_t yb, void *stream)
 {
   const size_t E=elt(h->dtype);
-  for(size_t b0=0; b0 < nb; b0 += maxBatch) {
-    const size_t cnt=std::min(maxBatch,nb-b0);
+  const size_t step=C == 1 ? nb : maxBatch;
+  for(size_t b0=0; b0 < nb; b0 += step) {
+    const size_t cnt=std::min(step,nb-b0);
     int rc=sign > 0 ?
       fftwpp_amd_forward_many(d,h->dtype,x+b0*xb*E,xS,C,cnt,xb,y+b0*yb*E,yS,yb,0,1,stream) :
       fftwpp_amd_backward_many(d,h->dtype,x+b0*xb*E,xS,xb,y+b0*yb*E,yS,C,cnt,yb,0,1,0,1.0,

@@ -126,8 +126,8 @@ def test_user_multipliers_centered(gpu, oracle, mults, shape, m):
 
 
 def test_user_multiplier_3d_large(gpu, oracle, mults):
-    # 128^3 -> 255^3 minimum: the z stage runs over more than 65535 rows (the
-    # batch is split); the difference output is f - g at any size
+    # 128^3 -> 255^3 minimum: the z stage runs over more than 65535 rows (as
+    # the columns of one launch); the difference output is f - g at any size
     import torch
     shape = (128, 128, 128)
     rng = np.random.default_rng(128)
