@@ -135,8 +135,14 @@ int run(size_t L, Complex **f, Complex *g) {
   crfft1d Cr(2*L);
   fft2d F2(L,L,-1);
   fft3d F3(L,L,L,1);
+  rcfft2d R2(L,L,(double *) g);
+  crfft3d C3(L,L,L,g);
+  R2.fft0((double *) g,g);
+  C3.deNyquist(g);
+  Transpose T(L,L,1,g);
+  T.transpose(g);
   return (int) (R.normalization()+Cr.normalization()+F2.normalization()+F3.normalization()+
-                 Many.normalization());
+                 Many.normalization()+R2.normalization()+C3.normalization());
 }
 int main() { return 0; }
 ''')
