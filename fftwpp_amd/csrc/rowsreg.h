@@ -409,7 +409,11 @@ colfwd_reg_kernel(DimArgs d, const cplx<T> *f, int64_t S, int64_t C,
   const cplx<T> *zN=(const cplx<T> *) d.zN;
   fill_twr<T,M,V>(tw,zN,(int) d.q,threadIdx.x,CR::NT);
   __syncthreads();
-  const int c=threadIdx.x % TC, t=threadIdx.x/TC;
+  // interleaved columns: consecutive lanes on consecutive columns; rows
+  // (column stride ics > 1): a row's threads on consecutive lanes
+  const bool rowmap=ics != 1;
+  const int c=rowmap ? threadIdx.x/TPS : threadIdx.x % TC;
+  const int t=rowmap ? threadIdx.x % TPS : threadIdx.x/TC;
   T *xb=reinterpret_cast<T *>(tw+M)+c*CR::CS;
   const int xm=CR::xm(c);
   const int64_t c0=xcd_tile(blockIdx.x,gridDim.x)*TC;
@@ -464,7 +468,11 @@ colbwd_reg_kernel(DimArgs d, const cplx<T> *F, int64_t SF, int64_t bsi,
   const cplx<T> *zN=(const cplx<T> *) d.zN;
   fill_twr<T,M,V>(tw,zN,(int) d.q,threadIdx.x,CR::NT);
   __syncthreads();
-  const int c=threadIdx.x % TC, t=threadIdx.x/TC;
+  // interleaved columns: consecutive lanes on consecutive columns; rows
+  // (column stride ics > 1): a row's threads on consecutive lanes
+  const bool rowmap=ics != 1;
+  const int c=rowmap ? threadIdx.x/TPS : threadIdx.x % TC;
+  const int t=rowmap ? threadIdx.x % TPS : threadIdx.x/TC;
   T *xb=reinterpret_cast<T *>(tw+M)+c*CR::CS;
   const int xm=CR::xm(c);
   const int64_t c0=xcd_tile(blockIdx.x,gridDim.x)*TC;
