@@ -111,7 +111,8 @@ def test_user_multipliers(gpu, oracle, mults, name, shape, M, m):
 
 
 @pytest.mark.parametrize("shape,m", [((41,), None), ((40,), [4]), ((11, 14), None),
-                                     ((5, 6, 9), None)])
+                                     ((5, 6, 9), None), ((6, 8, 200), [4, 8, 256]),
+                                     ((12, 300), [16, 512])])
 def test_user_multipliers_centered(gpu, oracle, mults, shape, m):
     rng = np.random.default_rng(7 + sum(shape))
     f, g = rnd(rng, shape), rnd(rng, shape)
