@@ -430,8 +430,10 @@ def main():
         traffic_child(args.traffic_child, args.size)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus N>1 is not supported "
-                                   "(see DESIGN.md for how the path would shard)"}))
+        print(json.dumps({"error": "one GPU only: --gpus N>1 is not supported (the batched "
+                                   "items would shard across GPUs with no exchange, the 3D and "
+                                   "2D Hermitian outer residues by an all-to-all per residue "
+                                   "pair)"}))
         sys.exit(2)
     traffic = None
     # never nest counter passes inside a profiled process
