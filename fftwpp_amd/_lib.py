@@ -62,6 +62,7 @@ SIGNATURES = {
     "fftwpp_amd_fft_destroy": (I, [P]),
     "fftwpp_amd_fft_execute": (I, [P, P, P, D, P]),
     "fftwpp_amd_fft_normalization": (Z, [P]),
+    "fftwpp_amd_fft_normalize": (I, [P, P, P]),
     "fftwpp_amd_transpose": (I, [Z, Z, Z, P, P, P]),
     "fftwpp_amd_fft_shift": (I, [I, P, Z, Z, Z, Z, P]),
     "fftwpp_amd_conv_create_general": (I, [ctypes.POINTER(P), I, I, ZP, ZP, ZP, I, Z, Z, P, P]),

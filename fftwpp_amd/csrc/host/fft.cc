@@ -307,6 +307,23 @@ size_t fftwpp_amd_fft_normalization(const fftwpp_amd_fft *h)
   return h ? total(h) : 0;
 }
 
+int fftwpp_amd_fft_normalize(const fftwpp_amd_fft *h, void *out, void *stream)
+{
+  if(!h || !out) return error(FFTWPP_AMD_ERR_ARG,"fftwpp_amd_fft_normalize: null argument");
+  if(!fftwpp_amd_is_device_pointer(out))
+    return error(FFTWPP_AMD_ERR_ARG,"fftwpp_amd_fft_normalize: out must be a device pointer");
+  const double s=1.0/(double) total(h);
+  const int op=h->kind == FFTWPP_AMD_FFT_C2R ? 4 : 0;
+  if(h->dims == 1) {
+    const size_t len=h->kind == FFTWPP_AMD_FFT_R2C ? h->n[0]/2+1 : h->n[0];
+    return fftwpp_amd_internal_fft_map(h->dtype,op,out,h->ostride,h->odist,out,h->ostride,
+                                       h->odist,len,h->howmany,s,stream);
+  }
+  // multidimensional: contiguous complex words, or contiguous real rows
+  const size_t words=h->kind == FFTWPP_AMD_FFT_R2C ? h->lead*h->half : total(h);
+  return fftwpp_amd_internal_fft_map(h->dtype,op,out,1,words,out,1,words,words,1,s,stream);
+}
+
 int fftwpp_amd_fft_execute(fftwpp_amd_fft *h, const void *in, void *out, double scale,
                            void *stream)
 {

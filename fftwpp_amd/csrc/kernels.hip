@@ -344,7 +344,7 @@ __global__ void real_unpack_kernel(const cplx<T> *z, T *out, int64_t n, T s)
 // for j < n, b < nb, strides in elements of each side's type.  op 0: complex
 // copy; 1: real to complex (imaginary part 0); 2: complex to its real part;
 // 3: the n-point spectrum of a Hermitian sequence from its n/2+1 stored modes
-// (y_j = conj x_{n-j} for j > n/2).
+// (y_j = conj x_{n-j} for j > n/2); 4: real copy.
 template<typename T>
 __global__ void fft_map_kernel(int op, const void *x, int64_t xs, int64_t xd,
                                void *y, int64_t ys, int64_t yd, int64_t n,
@@ -356,6 +356,10 @@ __global__ void fft_map_kernel(int op, const void *x, int64_t xs, int64_t xd,
   if(op == 1) {
     T v=reinterpret_cast<const T *>(x)[b*xd+j*xs];
     reinterpret_cast<cplx<T> *>(y)[b*yd+j*ys]=mk(s*v,(T) 0);
+    return;
+  }
+  if(op == 4) {
+    reinterpret_cast<T *>(y)[b*yd+j*ys]=s*reinterpret_cast<const T *>(x)[b*xd+j*xs];
     return;
   }
   const cplx<T> *xc=reinterpret_cast<const cplx<T> *>(x)+b*xd;
@@ -1445,7 +1449,7 @@ int fftwpp_amd_internal_fft_map(int dtype, int op, const void *x, size_t xs,
                                 size_t n, size_t nb, double scale, void *stream)
 {
   if(n == 0 || nb == 0) return FFTWPP_AMD_OK;
-  if(!x || !y || op < 0 || op > 3) return fail(FFTWPP_AMD_ERR_ARG,"bad element map");
+  if(!x || !y || op < 0 || op > 4) return fail(FFTWPP_AMD_ERR_ARG,"bad element map");
   const size_t total=n*nb;
   if((total+255)/256 > 0xffffffffu) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"element map too large");
   const unsigned blocks=(unsigned) ((total+255)/256);

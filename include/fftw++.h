@@ -88,6 +88,19 @@ protected:
     if(rc) throw Error(rc,fftwpp_amd_last_error());
   }
 
+  void normalizeOut(void *out) {
+    const bool dev=fftwpp_amd_is_device_pointer(out);
+    const size_t bytes=outWords*outSize;
+    void *d=out;
+    int rc=dev ? FFTWPP_AMD_OK : fftwpp_amd_malloc(&d,bytes);
+    if(!rc && !dev) rc=fftwpp_amd_memcpy(d,out,bytes,NULL);
+    if(!rc) rc=fftwpp_amd_fft_normalize(h,d,NULL);
+    if(!rc && !dev) rc=fftwpp_amd_memcpy(out,d,bytes,NULL);
+    if(!rc) rc=fftwpp_amd_synchronize(NULL);
+    if(!dev && d) fftwpp_amd_free(d);
+    if(rc) throw Error(rc,fftwpp_amd_last_error());
+  }
+
   void run(const void *in, void *out, double scale, bool shift=false) {
     if(!out) out=(void *) in;
     const bool dev=fftwpp_amd_is_device_pointer(in) && fftwpp_amd_is_device_pointer(out);
@@ -128,6 +141,10 @@ public:
   fftwPlan(const fftwPlan &)=delete;
   fftwPlan &operator=(const fftwPlan &)=delete;
   virtual ~fftwPlan() { fftwpp_amd_fft_destroy(h); }
+
+  // out /= normalization() over the output layout (fftw++.h:357-434).
+  void Normalize(Complex *out) { normalizeOut(out); }
+  void Normalize(double *out) { normalizeOut(out); }
 
   // The product of the transform lengths (what fftNormalized divides by).
   size_t normalization() const { return fftwpp_amd_fft_normalization(h); }

@@ -381,6 +381,9 @@ int fftwpp_amd_fft_execute(fftwpp_amd_fft *h, const void *in, void *out,
                            double scale, void *stream);
 /* n[0]*...*n[dims-1]: the factor fftNormalized divides by. */
 size_t fftwpp_amd_fft_normalization(const fftwpp_amd_fft *h);
+/* out <- out / normalization over the plan's output layout (contiguous
+ * output rows for real 2D/3D), as fftw::Normalize (fftw++.h:357-434). */
+int fftwpp_amd_fft_normalize(const fftwpp_amd_fft *h, void *out, void *stream);
 
 /* Origin-shifted real 2D/3D transforms (FP64 device arrays).  op 0: the
  * Shift of fftw++.h:144-222, negating the reals of odd rows i (2D, nx x

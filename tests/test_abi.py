@@ -141,6 +141,8 @@ int run(size_t L, Complex **f, Complex *g) {
   C3.deNyquist(g);
   Transpose T(L,L,1,g);
   T.transpose(g);
+  Forward.Normalize(g);
+  Cr.Normalize((double *) g);
   return (int) (R.normalization()+Cr.normalization()+F2.normalization()+F3.normalization()+
                  Many.normalization()+R2.normalization()+C3.normalization());
 }

@@ -338,3 +338,23 @@ def test_cxx_fft0_denyquist(gpu, shape):
         want[..., -1] = 0
     assert np.array_equal(nyq == 0, want == 0)
     assert rel(nyq, want) < TOL64
+
+
+def test_normalize(gpu):
+    # fftwpp_amd_fft_normalize (fftw::Normalize): out / N over the plan's output
+    # layout, for strided complex, r2c, c2r 2D and c2c 3D plans
+    import torch
+    from fftwpp_amd._lib import check, lib
+    from fftwpp_amd.torch_fft import DeviceFFT
+    rng = np.random.default_rng(41)
+    cases = [(DeviceFFT(48, -1, dtype=torch.float64, howmany=5, istride=5, ostride=5, idist=1,
+                        odist=1), torch.complex128, 240, 48),
+             (DeviceFFT(30, -1, kind="r2c", dtype=torch.float64), torch.complex128, 16, 30),
+             (DeviceFFT((6, 10), 1, kind="c2r", dtype=torch.float64), torch.float64, 60, 60),
+             (DeviceFFT((4, 5, 6), 1, dtype=torch.float32), torch.complex64, 120, 120)]
+    for plan, dt, count, N in cases:
+        v = rng.uniform(-1, 1, count) + (1j * rng.uniform(-1, 1, count) if dt.is_complex else 0)
+        t = torch.tensor(v, dtype=dt, device=gpu)
+        check(lib().fftwpp_amd_fft_normalize(plan._h, t.data_ptr(), None))
+        torch.cuda.synchronize()
+        assert rel(t.cpu().numpy(), v / N) < (1e-6 if dt == torch.complex64 else 1e-15)
