@@ -30,8 +30,10 @@ SIGNATURES = {
     "fftwpp_amd_batch_rows_lds": (Z, [P, I, Z, Z]),
     "fftwpp_amd_batch_rows_tps": (Z, [P, I]),
     "fftwpp_amd_batch_rows": (I, [P, I, I, P, Z, P, P, P, Z, Z, Z, Z, Z, P, Z, D, P]),
-    "fftwpp_amd_real_pack": (I, [I, P, P, P, Z, P]),
-    "fftwpp_amd_real_unpack": (I, [I, P, P, Z, D, P]),
+    "fftwpp_amd_real_balance_bytes": (Z, []),
+    "fftwpp_amd_real_balance": (I, [I, P, P, Z, P, P]),
+    "fftwpp_amd_real_pack": (I, [I, P, P, P, Z, P, P]),
+    "fftwpp_amd_real_unpack": (I, [I, P, P, Z, D, P, P]),
     "fftwpp_amd_multiply": (I, [I, I, P, P, Z, P]),
     "fftwpp_amd_scale": (I, [I, P, Z, D, P]),
     "fftwpp_amd_symmetrize_x": (I, [Z, Z, Z, P, Z, P]),

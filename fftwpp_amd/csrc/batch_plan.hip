@@ -101,6 +101,10 @@ const size_t BATCH_MAXM=4096;
 
 } // namespace
 
+// Recorded graphs kept per batch handle, one per (out, f, g) triple; the
+// least recently used one is replaced beyond this.
+constexpr size_t MAXGRAPHS=16;
+
 struct fftwpp_amd_batch {
   int dtype=0, mult=0, flags=0;
   size_t n=0, total=0;
@@ -549,9 +553,14 @@ int fftwpp_amd_batch_execute(fftwpp_amd_batch *h, void *out, const void *f,
     return bfail(FFTWPP_AMD_ERR_ARG,"fftwpp_amd_batch_execute: out must not be f or g");
   hipStream_t s=(hipStream_t) stream;
   if(!(h->flags & FFTWPP_AMD_BATCH_GRAPH)) return run_classes(*h,out,f,g,s);
-  for(Graph &gr : h->graphs)
-    if(gr.out == out && gr.f == f && gr.g == g)
-      return bhip(hipGraphLaunch(gr.exec,s),"hipGraphLaunch");
+  for(size_t i=0; i < h->graphs.size(); ++i) {
+    Graph &gr=h->graphs[i];
+    if(gr.out == out && gr.f == f && gr.g == g) {
+      // most recently used last
+      std::rotate(h->graphs.begin()+i,h->graphs.begin()+i+1,h->graphs.end());
+      return bhip(hipGraphLaunch(h->graphs.back().exec,s),"hipGraphLaunch");
+    }
+  }
   // first call on these arrays: run eagerly, then record the graph
   int rc=run_classes(*h,out,f,g,s);
   if(rc) return rc;
@@ -568,6 +577,15 @@ int fftwpp_amd_batch_execute(fftwpp_amd_batch *h, void *out, const void *f,
   if(rc) return rc;
   rc=bhip(hipGraphInstantiate(&gr.exec,gr.graph,nullptr,nullptr,0),"hipGraphInstantiate");
   if(rc) { (void) hipGraphDestroy(gr.graph); return rc; }
+  if(h->graphs.size() >= MAXGRAPHS) {
+    // the least recently used graph goes, after any launch of it, on
+    // whichever stream, has completed
+    (void) hipDeviceSynchronize();
+    Graph &old=h->graphs.front();
+    (void) hipGraphExecDestroy(old.exec);
+    (void) hipGraphDestroy(old.graph);
+    h->graphs.erase(h->graphs.begin());
+  }
   h->graphs.push_back(gr);
   return FFTWPP_AMD_OK;
 }

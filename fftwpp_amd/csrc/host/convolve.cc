@@ -666,8 +666,9 @@ static void staged(DeviceWork &work, Complex **f, size_t offset, size_t n,
 }
 
 // A binary convolution of real arrays f0, f1 (n real words each) into out:
-// z = f0 + i f1 in work slot 6, the complex self-convolution `self` of z in
-// place, out = Im z / 2.
+// z = f0 + i 2^e f1 in work slot 6 with the norm-balancing exponent e (work
+// slot 7), the complex self-convolution `self` of z in place, out =
+// 2^-e Im z / 2.
 template<class Self>
 static void realConvolution(DeviceWork &work, size_t n, int dtype, int code,
                             void *out, const void *f0, const void *f1,
@@ -676,9 +677,11 @@ static void realConvolution(DeviceWork &work, size_t n, int dtype, int code,
   if(code != FFTWPP_AMD_MULT_BINARY)
     throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"real convolutions take multBinary");
   void *z=work.get(6,n*(dtype == FFTWPP_AMD_F32 ? 8 : 16));
-  check(fftwpp_amd_real_pack(dtype,f0,f1,z,n,stream),"real pack");
+  void *bal=work.get(7,fftwpp_amd_real_balance_bytes());
+  check(fftwpp_amd_real_balance(dtype,f0,f1,n,bal,stream),"real balance");
+  check(fftwpp_amd_real_pack(dtype,f0,f1,z,n,bal,stream),"real pack");
   self(z);
-  check(fftwpp_amd_real_unpack(dtype,z,out,n,0.5,stream),"real unpack");
+  check(fftwpp_amd_real_unpack(dtype,z,out,n,0.5,bal,stream),"real unpack");
 }
 
 static void requireBinary(size_t A, size_t B, int code)

@@ -324,20 +324,97 @@ __global__ void multiply_kernel(int mult, cplx<T> *F0, const cplx<T> *F1,
   }
 }
 
-// Real convolutions: z = f0 + i f1, and out = s Im z (the halved imaginary
-// part of the self-convolution of z is f0 * f1).
+// Real convolutions: z = f0 + i 2^e f1, and out = s 2^-e Im z (the halved
+// imaginary part of the self-convolution of z is 2^e f0 * f1).  The exponent
+// e balances the two norms, ||f0|| ~ 2^e ||f1||, so that the rounding error
+// of the self-convolution, of order eps ||z||^2, stays of order
+// eps ||f0|| ||f1|| 2^e whatever the ratio of the input norms; scaling by a
+// power of two is exact.  e = 0 when either input is zero or no exponent is
+// given.
+constexpr int REAL_NORM_BLOCKS=256;
+
+// Partial sums of squares of f0 and f1 (in FP64) per block.
 template<typename T>
-__global__ void real_pack_kernel(const T *f0, const T *f1, cplx<T> *z, int64_t n)
+__global__ void __launch_bounds__(256) real_norms_kernel(const T *f0, const T *f1, int64_t n,
+                                                          double *part)
 {
-  int64_t i=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
-  if(i < n) z[i]=mk(f0[i],f1[i]);
+  __shared__ double s0[256], s1[256];
+  double a=0.0, b=0.0;
+  for(int64_t i=(int64_t) blockIdx.x*256+threadIdx.x; i < n; i += (int64_t) gridDim.x*256) {
+    const double x=f0[i], y=f1[i];
+    a=fma(x,x,a);
+    b=fma(y,y,b);
+  }
+  s0[threadIdx.x]=a;
+  s1[threadIdx.x]=b;
+  __syncthreads();
+  for(int w=128; w > 0; w >>= 1) {
+    if((int) threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x+w];
+      s1[threadIdx.x] += s1[threadIdx.x+w];
+    }
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) {
+    part[blockIdx.x]=s0[0];
+    part[REAL_NORM_BLOCKS+blockIdx.x]=s1[0];
+  }
+}
+
+// One block: the block sums in a fixed order, then the balancing exponent
+// e = round(log2(||f0|| / ||f1||)) written as an int after the partials.
+__global__ void __launch_bounds__(256) real_balance_kernel(double *part)
+{
+  __shared__ double s0[256], s1[256];
+  s0[threadIdx.x]=part[threadIdx.x];
+  s1[threadIdx.x]=part[REAL_NORM_BLOCKS+threadIdx.x];
+  __syncthreads();
+  for(int w=128; w > 0; w >>= 1) {
+    if((int) threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x+w];
+      s1[threadIdx.x] += s1[threadIdx.x+w];
+    }
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) {
+    int e=0;
+    if(s0[0] > 0.0 && s1[0] > 0.0) {
+      const double l=0.5*(log2(s0[0])-log2(s1[0]));
+      e=(int) rint(fmin(fmax(l,-900.0),900.0));
+    }
+    int *out=reinterpret_cast<int *>(part+2*REAL_NORM_BLOCKS);
+    out[0]=e;
+    out[1]=s0[0] == 0.0 || s1[0] == 0.0;   // a zero input: the product is zero
+  }
+}
+
+__device__ __forceinline__ int real_exponent(const double *bal)
+{
+  return bal ? reinterpret_cast<const int *>(bal+2*REAL_NORM_BLOCKS)[0] : 0;
+}
+
+__device__ __forceinline__ bool real_zero(const double *bal)
+{
+  return bal && reinterpret_cast<const int *>(bal+2*REAL_NORM_BLOCKS)[1] != 0;
 }
 
 template<typename T>
-__global__ void real_unpack_kernel(const cplx<T> *z, T *out, int64_t n, T s)
+__global__ void real_pack_kernel(const T *f0, const T *f1, cplx<T> *z, int64_t n,
+                                 const double *bal)
 {
+  const int e=real_exponent(bal);
   int64_t i=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
-  if(i < n) out[i]=s*z[i].y;
+  if(i < n) z[i]=mk(f0[i],(T) ldexp((double) f1[i],e));
+}
+
+template<typename T>
+__global__ void real_unpack_kernel(const cplx<T> *z, T *out, int64_t n, T s,
+                                   const double *bal)
+{
+  const int e=real_exponent(bal);
+  const bool zero=real_zero(bal);
+  int64_t i=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  if(i < n) out[i]=zero ? (T) 0 : (T) ldexp((double) (s*z[i].y),-e);
 }
 
 // Element maps of the standalone FFTs, y[b yd + j ys] = s op(x[b xd + j xs])
@@ -1410,8 +1487,30 @@ int fftwpp_amd_batch_rows(const fftwpp_amd_dim *d, int dtype, int mult,
     batch_rows_t<double>(d,mult,b,st);
 }
 
+size_t fftwpp_amd_real_balance_bytes(void)
+{
+  return (2*REAL_NORM_BLOCKS+1)*sizeof(double);   // partials, then e and the zero flag
+}
+
+int fftwpp_amd_real_balance(int dtype, const void *f0, const void *f1, size_t n,
+                            void *bal, void *stream)
+{
+  if(!f0 || !f1 || !bal) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  hipStream_t st=(hipStream_t) stream;
+  if(dtype == FFTWPP_AMD_F32)
+    hipLaunchKernelGGL(real_norms_kernel<float>,dim3(REAL_NORM_BLOCKS),dim3(256),0,st,
+                       (const float *) f0,(const float *) f1,(int64_t) n,(double *) bal);
+  else
+    hipLaunchKernelGGL(real_norms_kernel<double>,dim3(REAL_NORM_BLOCKS),dim3(256),0,st,
+                       (const double *) f0,(const double *) f1,(int64_t) n,(double *) bal);
+  int rc=launch_check("real_norms_kernel");
+  if(rc) return rc;
+  hipLaunchKernelGGL(real_balance_kernel,dim3(1),dim3(256),0,st,(double *) bal);
+  return launch_check("real_balance_kernel");
+}
+
 int fftwpp_amd_real_pack(int dtype, const void *f0, const void *f1, void *z,
-                         size_t n, void *stream)
+                         size_t n, const void *bal, void *stream)
 {
   if(n == 0) return FFTWPP_AMD_OK;
   if(!f0 || !f1 || !z) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
@@ -1419,15 +1518,17 @@ int fftwpp_amd_real_pack(int dtype, const void *f0, const void *f1, void *z,
   hipStream_t st=(hipStream_t) stream;
   if(dtype == FFTWPP_AMD_F32)
     hipLaunchKernelGGL(real_pack_kernel<float>,dim3(nb),dim3(256),0,st,
-                       (const float *) f0,(const float *) f1,(cplx<float> *) z,(int64_t) n);
+                       (const float *) f0,(const float *) f1,(cplx<float> *) z,(int64_t) n,
+                       (const double *) bal);
   else
     hipLaunchKernelGGL(real_pack_kernel<double>,dim3(nb),dim3(256),0,st,
-                       (const double *) f0,(const double *) f1,(cplx<double> *) z,(int64_t) n);
+                       (const double *) f0,(const double *) f1,(cplx<double> *) z,(int64_t) n,
+                       (const double *) bal);
   return launch_check("real_pack_kernel");
 }
 
 int fftwpp_amd_real_unpack(int dtype, const void *z, void *out, size_t n,
-                           double scale, void *stream)
+                           double scale, const void *bal, void *stream)
 {
   if(n == 0) return FFTWPP_AMD_OK;
   if(!z || !out) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
@@ -1435,10 +1536,12 @@ int fftwpp_amd_real_unpack(int dtype, const void *z, void *out, size_t n,
   hipStream_t st=(hipStream_t) stream;
   if(dtype == FFTWPP_AMD_F32)
     hipLaunchKernelGGL(real_unpack_kernel<float>,dim3(nb),dim3(256),0,st,
-                       (const cplx<float> *) z,(float *) out,(int64_t) n,(float) scale);
+                       (const cplx<float> *) z,(float *) out,(int64_t) n,(float) scale,
+                       (const double *) bal);
   else
     hipLaunchKernelGGL(real_unpack_kernel<double>,dim3(nb),dim3(256),0,st,
-                       (const cplx<double> *) z,(double *) out,(int64_t) n,scale);
+                       (const cplx<double> *) z,(double *) out,(int64_t) n,scale,
+                       (const double *) bal);
   return launch_check("real_unpack_kernel");
 }
 

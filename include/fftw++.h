@@ -26,6 +26,16 @@ protected:
   fftwpp_amd_fft *h=NULL;
   size_t inWords=0, outWords=0;   // array extents in elements
   size_t inSize=0, outSize=0;     // element bytes (sizeof(Complex) or sizeof(double))
+  // Reals that Normalize scales for a complex-to-real plan whose output rows
+  // are padded to 2 (n/2+1) reals (in place), as the reference records at
+  // construction (the `doubles` of crfft1d/2d/3d, fftw++.h:957-975,
+  // 1318-1331, 1522-1535); 0: the plan's contiguous output.
+  size_t paddedReals=0;
+
+  // The reference's in-place rule (Inplace, fftw++.h:88-96): no output
+  // array, or the output is the input.
+  static bool inplaceArgs(const void *in, const void *out) { return !out || in == out; }
+  void padOutput(size_t rows, size_t last) { paddedReals=2*rows*(last/2+1); }
 
   void create(int kind, int dims, const size_t *n, int sign, size_t howmany,
               size_t istride, size_t ostride, size_t idist, size_t odist) {
@@ -90,11 +100,16 @@ protected:
 
   void normalizeOut(void *out) {
     const bool dev=fftwpp_amd_is_device_pointer(out);
-    const size_t bytes=outWords*outSize;
+    const size_t bytes=paddedReals ? paddedReals*sizeof(double) : outWords*outSize;
     void *d=out;
     int rc=dev ? FFTWPP_AMD_OK : fftwpp_amd_malloc(&d,bytes);
     if(!rc && !dev) rc=fftwpp_amd_memcpy(d,out,bytes,NULL);
-    if(!rc) rc=fftwpp_amd_fft_normalize(h,d,NULL);
+    // padded rows: every real of the padded array (pairs of reals scaled as
+    // complex words), as the reference's Normalize(double *) does
+    if(!rc) rc=paddedReals ?
+      fftwpp_amd_scale(FFTWPP_AMD_F64,d,paddedReals/2,
+                       1.0/(double) fftwpp_amd_fft_normalization(h),NULL) :
+      fftwpp_amd_fft_normalize(h,d,NULL);
     if(!rc && !dev) rc=fftwpp_amd_memcpy(out,d,bytes,NULL);
     if(!rc) rc=fftwpp_amd_synchronize(NULL);
     if(!dev && d) fftwpp_amd_free(d);
@@ -270,9 +285,11 @@ class crfft1d : public fftwPlan {
 public:
   crfft1d(size_t nx, double * =NULL, size_t =fftw::maxthreads) {
     create(FFTWPP_AMD_FFT_C2R,1,&nx,1,1,1,1,0,0);
+    padOutput(1,nx);
   }
-  crfft1d(size_t nx, Complex *, double * =NULL, size_t =fftw::maxthreads) {
+  crfft1d(size_t nx, Complex *in, double *out=NULL, size_t =fftw::maxthreads) {
     create(FFTWPP_AMD_FFT_C2R,1,&nx,1,1,1,1,0,0);
+    if(inplaceArgs(in,out)) padOutput(1,nx);
   }
 };
 
@@ -302,11 +319,14 @@ public:
     const size_t n[2]={nx,ny};
     create(FFTWPP_AMD_FFT_C2R,2,n,1,1,1,1,0,0);
     origin(2,2,n);
+    padOutput(nx,ny);
   }
-  crfft2d(size_t nx, size_t ny, Complex *, double * =NULL, size_t =fftw::maxthreads) {
+  crfft2d(size_t nx, size_t ny, Complex *in, double *out=NULL,
+          size_t =fftw::maxthreads) {
     const size_t n[2]={nx,ny};
     create(FFTWPP_AMD_FFT_C2R,2,n,1,1,1,1,0,0);
     origin(2,2,n);
+    if(inplaceArgs(in,out)) padOutput(nx,ny);
   }
   // fftw++.h:1342-1360 (2D), 1466-1495 (3D): zero the Nyquist modes of a
   // shifted spectrum
@@ -337,12 +357,14 @@ public:
     const size_t n[3]={nx,ny,nz};
     create(FFTWPP_AMD_FFT_C2R,3,n,1,1,1,1,0,0);
     origin(2,3,n);
+    padOutput(nx*ny,nz);
   }
-  crfft3d(size_t nx, size_t ny, size_t nz, Complex *, double * =NULL,
+  crfft3d(size_t nx, size_t ny, size_t nz, Complex *in, double *out=NULL,
           size_t =fftw::maxthreads) {
     const size_t n[3]={nx,ny,nz};
     create(FFTWPP_AMD_FFT_C2R,3,n,1,1,1,1,0,0);
     origin(2,3,n);
+    if(inplaceArgs(in,out)) padOutput(nx*ny,nz);
   }
   // fftw++.h:1342-1360 (2D), 1466-1495 (3D): zero the Nyquist modes of a
   // shifted spectrum

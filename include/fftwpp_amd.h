@@ -145,13 +145,22 @@ int fftwpp_amd_batch_rows(const fftwpp_amd_dim *d, int dtype, int mult,
                           size_t ptotal, double scale, void *stream);
 
 /* Real convolutions (fftPadReal, convolve.h:805-980; tests/hybridconvr*.cc):
- * z[i] = f0[i] + i f1[i] for n real words of f0 and f1, and out[i] =
- * scale * Im z[i]; between the two, a complex self-convolution of z gives
- * 2 i f0 * f1 in its imaginary part. */
+ * z[i] = f0[i] + i 2^e f1[i] for n real words of f0 and f1, and out[i] =
+ * scale 2^-e Im z[i]; between the two, a complex self-convolution of z gives
+ * 2 i 2^e f0 * f1 in its imaginary part.  The exponent e = round(log2(||f0|| /
+ * ||f1||)) (0 when either input is zero) balances the norms so that the
+ * rounding error stays relative to ||f0|| ||f1||; fftwpp_amd_real_balance
+ * computes it on the device into `bal` (fftwpp_amd_real_balance_bytes()
+ * bytes of device memory), which pack and unpack read (bal = NULL: e = 0).
+ * The scaling by a power of two is exact; when either input is zero, unpack
+ * writes exact zeros. */
+size_t fftwpp_amd_real_balance_bytes(void);
+int fftwpp_amd_real_balance(int dtype, const void *f0, const void *f1,
+                            size_t n, void *bal, void *stream);
 int fftwpp_amd_real_pack(int dtype, const void *f0, const void *f1, void *z,
-                         size_t n, void *stream);
+                         size_t n, const void *bal, void *stream);
 int fftwpp_amd_real_unpack(int dtype, const void *z, void *out, size_t n,
-                           double scale, void *stream);
+                           double scale, const void *bal, void *stream);
 
 /* The multipliers on device arrays (convolve.cc:26-110): n counts complex
  * words for multBinary/multcorrelation and real words for realMultBinary. */
@@ -334,8 +343,13 @@ int fftwpp_amd_batch_describe(const fftwpp_amd_batch *h, char *buf,
                               size_t len);
 /* out <- f (*) g item by item (device pointers; out distinct from f, g) on the
  * HIP stream; with FFTWPP_AMD_BATCH_GRAPH the first call on a given
- * (out, f, g) runs directly and records the graph that later calls replay.
- * Calls on one handle must not overlap. */
+ * (out, f, g) runs directly and records the graph that later calls replay
+ * (up to 16 graphs per handle, the least recently used one replaced; the
+ * graphs hold the addresses, so arrays freed and reallocated at the same
+ * addresses replay the same graph, which is correct since the plan is
+ * immutable).  A handle is single-stream: its calls must not overlap, on one
+ * stream or several (they share the partial-sum buffers, the fork/join
+ * events and the side streams). */
 int fftwpp_amd_batch_execute(fftwpp_amd_batch *h, void *out, const void *f,
                              const void *g, void *stream);
 

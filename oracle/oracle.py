@@ -249,10 +249,12 @@ def _offsets(L, centered):
     return -(L // 2) if centered else 0
 
 
-def explicit_conv(f, g, N, centered=False):
+def explicit_conv(f, g, N, centered=False, workers=None):
     """Normalised length-N circular convolution of zero-padded (optionally
     centred) complex data, truncated back to the input shape.  ``N`` is an int
-    or a tuple with one padded length per dimension."""
+    or a tuple with one padded length per dimension.  ``workers`` (an int)
+    runs the transforms through scipy.fft's multithreaded pocketfft in place
+    (the same library routine as numpy.fft), for full-size arrays."""
     f = np.asarray(f, dtype=np.complex128)
     g = np.asarray(g, dtype=np.complex128)
     d = f.ndim
@@ -268,8 +270,16 @@ def explicit_conv(f, g, N, centered=False):
     ix = np.ix_(*idx)
     F[ix] = f
     G[ix] = g
-    H = np.fft.ifftn(np.fft.fftn(F) * np.fft.fftn(G))
-    return H[ix]
+    if workers is None:
+        H = np.fft.ifftn(np.fft.fftn(F) * np.fft.fftn(G))
+        return H[ix]
+    import scipy.fft as sf
+    F = sf.fftn(F, overwrite_x=True, workers=workers)
+    G = sf.fftn(G, overwrite_x=True, workers=workers)
+    F *= G
+    del G
+    F = sf.ifftn(F, overwrite_x=True, workers=workers)
+    return F[ix]
 
 
 def explicit_general(arrays, N, op, centered=False):
@@ -330,10 +340,11 @@ def expand_hermitian(f, compact_axes_even):
     return full
 
 
-def explicit_hconv(f, g, N):
+def explicit_hconv(f, g, N, workers=None):
     """Centred Hermitian convolution by explicit padding: the stored last axis
     holds H non-negative modes, leading axes are centred.  ``N``: padded length
-    per dimension (last entry for the Hermitian axis)."""
+    per dimension (last entry for the Hermitian axis).  ``workers`` as for
+    explicit_conv."""
     f = np.asarray(f, dtype=np.complex128)
     d = f.ndim
     Ns = (N,) * d if np.isscalar(N) else tuple(N)
@@ -350,9 +361,18 @@ def explicit_hconv(f, g, N):
     ix = np.ix_(*idx)
     FF[ix] = F
     GG[ix] = G
-    HH = np.fft.ifftn(np.fft.fftn(FF) * np.fft.fftn(GG))
-    out_idx = idx[:-1] + [np.arange(H)]
-    return HH[np.ix_(*out_idx)]
+    del F, G
+    out_idx = np.ix_(*(idx[:-1] + [np.arange(H)]))
+    if workers is None:
+        HH = np.fft.ifftn(np.fft.fftn(FF) * np.fft.fftn(GG))
+        return HH[out_idx]
+    import scipy.fft as sf
+    FF = sf.fftn(FF, overwrite_x=True, workers=workers)
+    GG = sf.fftn(GG, overwrite_x=True, workers=workers)
+    FF *= GG
+    del GG
+    FF = sf.ifftn(FF, overwrite_x=True, workers=workers)
+    return FF[out_idx]
 
 
 def hplane_of(lengths):

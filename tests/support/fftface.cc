@@ -73,6 +73,32 @@ int testfft_real_multi_cxx(int dims, const size_t *n, double *in, double *out, d
   return 0;
 }
 
+// In place on a host array of n[0] x ... x 2 (n[dims-1]/2+1) reals (FFTW's
+// padded rows) holding the input reals in the first n[dims-1] of each row:
+// rcfft{2,3}d in place, crfft{2,3}d in place (unnormalised), then Normalize
+// over the padded array (fftw++.h:357-370), which leaves the input reals.
+int testfft_inplace_normalize_cxx(int dims, const size_t *n, double *data)
+{
+  try {
+    if(dims == 2) {
+      rcfft2d F(n[0],n[1],(Complex *) data);
+      F.fft(data,(Complex *) data);
+      crfft2d B(n[0],n[1],(Complex *) data);
+      B.fft((Complex *) data,data);
+      B.Normalize(data);
+    } else {
+      rcfft3d F(n[0],n[1],n[2],(Complex *) data);
+      F.fft(data,(Complex *) data);
+      crfft3d B(n[0],n[1],n[2],(Complex *) data);
+      B.fft((Complex *) data,data);
+      B.Normalize(data);
+    }
+  } catch(const std::exception &) {
+    return 1;
+  }
+  return 0;
+}
+
 // Origin-shifted real transforms on host arrays: out = rcfft{2,3}d fft0 of
 // in; back = crfft{2,3}d fft0Normalized of out (= in); nyq = deNyquist of a
 // copy of out.

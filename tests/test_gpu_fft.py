@@ -358,3 +358,19 @@ def test_normalize(gpu):
         check(lib().fftwpp_amd_fft_normalize(plan._h, t.data_ptr(), None))
         torch.cuda.synchronize()
         assert rel(t.cpu().numpy(), v / N) < (1e-6 if dt == torch.complex64 else 1e-15)
+
+
+@pytest.mark.parametrize("shape", [(6, 10), (5, 7), (4, 6, 8), (3, 5, 7)])
+def test_cxx_inplace_real_normalize(gpu, shape):
+    """In-place rcfft{2,3}d then crfft{2,3}d on FFTW's padded rows, then
+    Normalize (fftw++.h:357-370) over the padded array: every row, the last
+    included, returns to the input reals."""
+    lib = ctypes.CDLL(TESTLIB)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rng = np.random.default_rng(sum(shape))
+    x = rng.uniform(-1, 1, shape)
+    padded = np.zeros(shape[:-1] + (2 * (shape[-1] // 2 + 1),))
+    padded[..., :shape[-1]] = x
+    ns = (ctypes.c_size_t * 3)(*shape)
+    assert lib.testfft_inplace_normalize_cxx(len(shape), ns, padded.ctypes.data_as(dp)) == 0
+    assert rel(padded[..., :shape[-1]], x) < TOL64

@@ -360,21 +360,33 @@ def test_autograd_finite_differences(gpu, oracle):
 
 # ---------------------------------------------------------------- full size
 
-def test_conv3d_256_full_size_sampled_direct(gpu, oracle):
-    """256^3 -> 512^3 in FP64 and FP32: FP64 checked against direct sums at
-    sampled outputs, FP32 against FP64 over the whole array, plus symmetry
-    f (*) g == g (*) f."""
+def _workers():
+    import os
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def test_conv3d_256_full_size(gpu, oracle):
+    """The 3D workload at full size, 256^3 -> 512^3 from seed 20261202: FP64
+    over the whole array against explicit zero padding to 512^3 (pocketfft),
+    and against direct sums at sampled outputs (an FFT-free check); FP32 over
+    the whole array against the FP64 oracle; plus f (*) g == g (*) f."""
     import torch
     from fftwpp_amd.torch_conv import DeviceConvolution
     f, g = oracle.inputs_3d(256)
     fd = torch.tensor(f, device=gpu)
     gd = torch.tensor(g, device=gpu)
-    plan = DeviceConvolution((256, 256, 256))
+    plan = DeviceConvolution((256, 256, 256), M=[512] * 3)
+    assert plan.geometry() == ((512,) * 3, (256,) * 3, (2,) * 3)
     h = plan(fd, gd)
     h2 = plan(gd, fd)
     torch.cuda.synchronize()
     H = h.cpu().numpy()
     assert oracle.rel_l2(h2.cpu().numpy(), H) < TOL64
+    del h2
     rng = np.random.default_rng(91)
     pts = [(255, 255, 255), (0, 0, 0), (128, 3, 250)] + \
         [tuple(rng.integers(0, 256, 3)) for _ in range(5)]
@@ -384,12 +396,50 @@ def test_conv3d_256_full_size_sampled_direct(gpu, oracle):
         num += abs(H[i, j, k] - direct) ** 2
         den += abs(direct) ** 2
     assert math.sqrt(num / den) < TOL64
-    p32 = DeviceConvolution((256, 256, 256), dtype=torch.complex64)
+    ref = oracle.explicit_conv(f, g, 512, workers=_workers())
+    err64 = oracle.rel_l2(H, ref)
+    print(f"fp64 256^3 rel L2 vs explicit padding (whole array): {err64:.3e}", flush=True)
+    assert err64 < TOL64
+    del H
+    p32 = DeviceConvolution((256, 256, 256), dtype=torch.complex64, M=[512] * 3)
     h32 = p32(fd.to(torch.complex64), gd.to(torch.complex64))
     torch.cuda.synchronize()
-    err = oracle.rel_l2(h32.cpu().numpy(), H)
-    print(f"fp32 256^3 rel L2 vs FP64: {err:.3e}", flush=True)
+    err = oracle.rel_l2(h32.cpu().numpy(), ref)
+    print(f"fp32 256^3 rel L2 vs FP64 oracle (whole array): {err:.3e}", flush=True)
     assert err < TOL32
+
+
+def test_hconv2d_config_full_size(gpu, oracle):
+    """The 2D centred Hermitian workload at full size: 2047 x 1024 stored
+    modes (kx in [-1023, 1023], ky in [0, 1023]), amplitude |k|^-1 with
+    uniform phase from seed 20261202, HermitianSymmetrizeX applied, M = 3070
+    per dimension (x: m = 1024, p = 2, q = 3; y: Hermitian, D = 2).  FP64
+    against explicit zero padding to 3070^2 (pocketfft) over the whole array,
+    FP32 against the same FP64 oracle; the drop-in handle of cfftw++.h
+    (fftwpp_create_hconv2d) on device arrays gives the same FP64 result."""
+    import torch
+    from fftwpp_amd import _lib
+    f, g = oracle.inputs_h2(2047, 1024)
+    ref = oracle.explicit_hconv(f, g, (3070, 3070), workers=_workers())
+    h, plan = run(gpu, "hermitian", f, g, M=[3070, 3070], lengths=[2047, 2047])
+    N, m, q = plan.geometry()
+    assert m == (1024, 1024) and q == (3, 3), (N, m, q)
+    err64 = oracle.rel_l2(h, ref)
+    print(f"2D-H 2047x1024 fp64 rel L2 vs explicit padding: {err64:.3e}", flush=True)
+    assert err64 < TOL64
+    h32, _ = run(gpu, "hermitian", f, g, M=[3070, 3070], lengths=[2047, 2047],
+                 dtype=torch.complex64)
+    err32 = oracle.rel_l2(h32, ref)
+    print(f"2D-H 2047x1024 fp32 rel L2 vs FP64 oracle: {err32:.3e}", flush=True)
+    assert err32 < TOL32
+    L = _lib.lib()
+    hd = L.fftwpp_create_hconv2d(2047, 2047)
+    a = torch.tensor(f, device=gpu)
+    b = torch.tensor(g, device=gpu)
+    assert L.fftwpp_hconv2d_convolve_device(hd, a.data_ptr(), b.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    L.fftwpp_hconv2d_delete(hd)
+    assert oracle.rel_l2(a.cpu().numpy(), ref) < TOL64
 
 
 # ---------------------------------------------------------------- m=256 kernels

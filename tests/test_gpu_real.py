@@ -99,3 +99,32 @@ def test_real_rejects_adjoint_and_complex_dtype(gpu, oracle):
     a = torch.ones(16, dtype=torch.float64, device=gpu)
     with pytest.raises(FFTWPPError):
         plan.adjoint(a, a, a)
+
+
+def _ratio_inputs(shape, ratio, seed):
+    rng = np.random.default_rng(seed)
+    f = rng.uniform(-1, 1, shape)
+    g = rng.uniform(-1, 1, shape) * ratio
+    return f, g
+
+
+@pytest.mark.parametrize("ratio", [1.0, 1e-3, 1e-6, 1e-9, 0.0, 1e6])
+def test_real_unbalanced_norms(gpu, oracle, ratio):
+    """Inputs whose norms differ by up to nine orders of magnitude (and a zero
+    input): the error stays relative to ||f|| ||g|| (the norms are balanced by
+    an exact power of two before the complex self-convolution).  1D L = 4096
+    and 2D 64 x 48 against the direct sums (tests/direct.cc), 3D 12 x 10 x 14
+    too; FP64 <= 1e-12, FP32 <= 1e-5."""
+    import torch
+    cases = [((4096,), oracle.direct_conv), ((64, 48), oracle.direct_conv2),
+             ((12, 10, 14), oracle.direct_conv3)]
+    for shape, direct in cases:
+        f, g = _ratio_inputs(shape, ratio, len(shape) + 7)
+        want = direct(f.astype(complex), g.astype(complex)).real
+        for dtype, tol in ((torch.float64, TOL64), (torch.float32, TOL32)):
+            h, h_inplace, _ = run_real(gpu, f, g, dtype=dtype)
+            err = oracle.rel_l2(h, want)
+            assert err < tol, (shape, ratio, dtype, err)
+            assert np.array_equal(h, h_inplace)
+            if ratio == 0.0:
+                assert not np.any(h)

@@ -234,6 +234,25 @@ def test_explicit_independent_of_padding(oracle):
         assert oracle.rel_l2(oracle.explicit_conv(f, g, N), a) < TOL
 
 
+def test_explicit_multithreaded_path(oracle):
+    """The in-place multithreaded path used for full-size arrays gives the
+    direct sums too (2D, 3D and the centred Hermitian 2D case)."""
+    rng = np.random.default_rng(9)
+    f, g = rnd(rng, 6, 5, 7), rnd(rng, 6, 5, 7)
+    d = oracle.direct_conv3(f, g)
+    assert oracle.rel_l2(oracle.explicit_conv(f, g, (11, 9, 13), workers=2), d) < TOL
+    f, g = rnd(rng, 9, 8), rnd(rng, 9, 8)
+    assert oracle.rel_l2(oracle.explicit_conv(f, g, (17, 16), workers=2),
+                         oracle.direct_conv2(f, g)) < TOL
+    for Lx, Ly in [(7, 7), (9, 5)]:
+        Hy = (Ly + 1) // 2
+        fh, gh = oracle.ramph2(Lx, Hy, 0), oracle.ramph2(Lx, Hy, 1)
+        Mx = 3 * ((Lx + 1) // 2) - 2 * (Lx % 2)
+        My = 3 * Hy - 2 * (Ly % 2)
+        assert oracle.rel_l2(oracle.explicit_hconv(fh, gh, (Mx, My), workers=2),
+                             oracle.direct_convh2(fh, gh)) < TOL
+
+
 def _cyclic_brute(f, g, N):
     # h[n] = sum over i + j = n (mod N) of f_i g_j, by definition
     h = np.zeros(N, dtype=complex)
