@@ -14,6 +14,16 @@
 // which engine takes the rows (stage 1) of a dimension (2: register FFT).
 extern "C" int fftwpp_amd_internal_rows_fit(const fftwpp_amd_dim *d, int dtype);
 extern "C" int fftwpp_amd_internal_engine(const fftwpp_amd_dim *d, int stage);
+// the item table {i rs, L, i L} of `rows` rows for the batched row kernel
+extern "C" int fftwpp_amd_internal_row_items(void *items, size_t rows, size_t rs,
+                                             size_t L, void *stream);
+// reference-layout blocks: T <-> block array through a table; Hermitian
+// half <-> full centred spectrum
+extern "C" int fftwpp_amd_internal_block_map(int scatter, const void *T, const void *tab,
+                                             size_t ne, size_t C, void *arr, int real,
+                                             void *stream);
+extern "C" int fftwpp_amd_internal_herm_expand(int op, const void *a, void *b, size_t H,
+                                               size_t C, int acc, void *stream);
 
 namespace fftwpp {
 
@@ -206,6 +216,9 @@ fftBase::fftBase(size_t L, size_t M, Application &app, size_t C, size_t S,
 fftBase::~fftBase()
 {
   fftwpp_amd_dim_destroy(dev);
+  fftwpp_amd_dim_destroy(hdev);
+  for(FaceBlock &fb : faceBlocks)
+    for(void *t : fb.tab) fftwpp_amd_free(t);
 }
 
 size_t fftBase::index(size_t r, size_t i)
@@ -247,63 +260,116 @@ void fftBase::initDevice()
   check(fftwpp_amd_dim_create(&dev,L,m,q,kind()),"device twiddles");
 }
 
-// Residues of block r and, for each output k < noutputs(r), the slot*m + s
-// of padded index index(r,k) = q*s + residue.
-void fftBase::blockResidues(size_t r, std::vector<size_t> &res,
-                            std::vector<uint32_t> &map)
+// Reference-layout block r: entries (padded position, array, word offset)
+// in the order the reference's routines write them (see forward()).
+// Arrays: 0 the block array F, 1 the input f (overwrite: block 0), 2 f + b
+// (overwrite: the middle blocks).
+fftBase::FaceBlock &fftBase::faceBlock(size_t r)
 {
-  size_t nk=noutputs(r);
-  res.clear();
-  map.resize(nk);
-  for(size_t k=0; k < nk; ++k) {
-    size_t i=index(r,k);
-    size_t rr=i % q, s=i/q;
-    size_t slot=0;
-    while(slot < res.size() && res[slot] != rr) ++slot;
-    if(slot == res.size()) res.push_back(rr);
-    map[k]=(uint32_t) (slot*m+s);
+  if(r >= faceBlocks.size()) faceBlocks.resize(r+1);
+  FaceBlock &fb=faceBlocks[r];
+  if(fb.ready) return fb;
+  struct Entry { size_t pos; int arr; size_t word; };
+  std::vector<Entry> e;
+  if(kind() == FFTWPP_AMD_KIND_HERMITIAN) {
+    const size_t stride=noutputs(0), D1=r == 0 ? D0 : D;
+    for(size_t d=0; d < D1; ++d)
+      for(size_t k=0; k < stride; ++k)
+        e.push_back({index(r,k+stride*d),0,2*b*d+C*k});
+  } else if(overwrite) {
+    for(size_t rr=0; rr < R; rr += increment(rr))
+      for(size_t k=0; k < noutputs(rr); ++k)
+        e.push_back({index(rr,k),rr == n-1 ? 0 : (rr == 0 ? 1 : 2),S*k});
+  } else {
+    for(size_t k=0; k < noutputs(r); ++k)
+      e.push_back({index(r,k),0,S*k});
   }
+  std::vector<uint64_t> h[3];
+  for(const Entry &x : e) {
+    if(x.pos >= m*q) throw Error(FFTWPP_AMD_ERR_ARG,"padded index beyond m q");
+    const size_t rr=x.pos % q, s=x.pos/q;
+    size_t slot=0;
+    while(slot < fb.res.size() && fb.res[slot] != rr) ++slot;
+    if(slot == fb.res.size()) fb.res.push_back(rr);
+    h[x.arr].push_back(slot*m+s);
+    h[x.arr].push_back(x.word);
+  }
+  for(int a=0; a < 3; ++a) {
+    fb.ne[a]=h[a].size()/2;
+    if(fb.ne[a] == 0) continue;
+    check(fftwpp_amd_malloc(&fb.tab[a],h[a].size()*sizeof(uint64_t)),"block table");
+    check(fftwpp_amd_memcpy(fb.tab[a],h[a].data(),h[a].size()*sizeof(uint64_t),NULL),"block table");
+  }
+  check(fftwpp_amd_synchronize(NULL),"block table");
+  fb.ready=true;
+  return fb;
 }
 
 void fftBase::forward(Complex *f, Complex *F, size_t r, Complex *)
 {
-  if(kind() == FFTWPP_AMD_KIND_HERMITIAN || overwrite)
-    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"direct forward() of Hermitian or overwrite transforms");
   if(!fftwpp_amd_is_device_pointer(f) || !fftwpp_amd_is_device_pointer(F))
     throw Error(FFTWPP_AMD_ERR_ARG,"forward() takes device pointers");
-  std::vector<size_t> res;
-  std::vector<uint32_t> map;
-  blockResidues(r,res,map);
-  // the reference layouts of D=1 blocks place residue n*u+r at F+u*m*S
-  for(size_t k=0; k < map.size(); k += m) {
-    size_t slot=map[k]/m;
-    for(size_t s=0; s < m; ++s)
-      if(map[k+s] != slot*m+s)
-        throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"forward() of conjugate-pair (D>1) blocks");
-    check(fftwpp_amd_forward_many(dev,FFTWPP_AMD_F64,f,S,C,1,0,F+k*S,S,0,
-                                  res[slot],1,NULL),"forward");
+  if(overwrite && r != 0)
+    throw Error(FFTWPP_AMD_ERR_ARG,"an in-place (overwrite) plan computes every block at r = 0");
+  if(r >= R) throw Error(FFTWPP_AMD_ERR_ARG,"residue block beyond R");
+  FaceBlock &fb=faceBlock(r);
+  const bool herm=kind() == FFTWPP_AMD_KIND_HERMITIAN;
+  const size_t nres=fb.res.size();
+  Complex *T=(Complex *) faceWork.get(0,nres*m*C*sizeof(Complex));
+  const fftwpp_amd_dim *dim=dev;
+  const void *src=f;
+  size_t Sin=S;
+  if(herm) {
+    // the full centred spectrum of 2H - 1 modes on the centred twin
+    const size_t H=inputLength();
+    if(!hdev) check(fftwpp_amd_dim_create(&hdev,2*H-1,m,q,FFTWPP_AMD_KIND_CENTERED),"Hermitian twin");
+    void *full=faceWork.get(1,(2*H-1)*C*sizeof(Complex));
+    check(fftwpp_amd_internal_herm_expand(0,f,full,H,C,0,NULL),"Hermitian expand");
+    dim=hdev;
+    src=full;
+    Sin=C;
   }
+  for(size_t slot=0; slot < nres; ++slot)
+    check(fftwpp_amd_forward_many(dim,FFTWPP_AMD_F64,src,Sin,C,1,0,T+slot*m*C,C,0,
+                                  fb.res[slot],1,NULL),"forward");
+  Complex *base[3]={F,f,f+b};
+  for(int a=0; a < 3; ++a)
+    if(fb.ne[a])
+      check(fftwpp_amd_internal_block_map(0,T,fb.tab[a],fb.ne[a],C,base[a],herm,NULL),
+            "forward layout");
 }
 
 void fftBase::backward(Complex *F, Complex *f, size_t r, Complex *)
 {
-  if(kind() == FFTWPP_AMD_KIND_HERMITIAN || overwrite)
-    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"direct backward() of Hermitian or overwrite transforms");
   if(!fftwpp_amd_is_device_pointer(f) || !fftwpp_amd_is_device_pointer(F))
     throw Error(FFTWPP_AMD_ERR_ARG,"backward() takes device pointers");
-  std::vector<size_t> res;
-  std::vector<uint32_t> map;
-  blockResidues(r,res,map);
-  bool first=true;
-  for(size_t k=0; k < map.size(); k += m) {
-    size_t slot=map[k]/m;
-    for(size_t s=0; s < m; ++s)
-      if(map[k+s] != slot*m+s)
-        throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"backward() of conjugate-pair (D>1) blocks");
-    check(fftwpp_amd_backward_many(dev,FFTWPP_AMD_F64,F+k*S,S,0,f,S,C,1,0,
-                                   res[slot],1,r > 0 || !first,1.0,NULL),"backward");
-    first=false;
+  if(overwrite && r != 0)
+    throw Error(FFTWPP_AMD_ERR_ARG,"an in-place (overwrite) plan computes every block at r = 0");
+  if(r >= R) throw Error(FFTWPP_AMD_ERR_ARG,"residue block beyond R");
+  FaceBlock &fb=faceBlock(r);
+  const bool herm=kind() == FFTWPP_AMD_KIND_HERMITIAN;
+  const size_t nres=fb.res.size();
+  const size_t tb=nres*m*C*sizeof(Complex);
+  Complex *T=(Complex *) faceWork.get(0,tb);
+  check(fftwpp_amd_memset(T,tb,NULL),"backward clear");
+  Complex *base[3]={F,f,f+b};
+  for(int a=0; a < 3; ++a)
+    if(fb.ne[a])
+      check(fftwpp_amd_internal_block_map(1,T,fb.tab[a],fb.ne[a],C,base[a],herm,NULL),
+            "backward layout");
+  if(herm) {
+    const size_t H=inputLength();
+    if(!hdev) check(fftwpp_amd_dim_create(&hdev,2*H-1,m,q,FFTWPP_AMD_KIND_CENTERED),"Hermitian twin");
+    void *full=faceWork.get(2,(2*H-1)*C*sizeof(Complex));
+    for(size_t slot=0; slot < nres; ++slot)
+      check(fftwpp_amd_backward_many(hdev,FFTWPP_AMD_F64,T+slot*m*C,C,0,full,C,C,1,0,
+                                     fb.res[slot],1,slot > 0,1.0,NULL),"backward");
+    check(fftwpp_amd_internal_herm_expand(1,full,f,H,C,r > 0,NULL),"Hermitian fold");
+    return;
   }
+  for(size_t slot=0; slot < nres; ++slot)
+    check(fftwpp_amd_backward_many(dev,FFTWPP_AMD_F64,T+slot*m*C,C,0,f,S,C,1,0,
+                                   fb.res[slot],1,r > 0 || slot > 0,1.0,NULL),"backward");
 }
 
 // ------------------------------------------------------------------------
@@ -873,14 +939,8 @@ static void fusedRows(fftBase *fft, DeviceWork &work, size_t slot, int dtype,
     void *items=work.get(slot+1,rows*3*sizeof(int64_t));
     const std::vector<size_t> key={rows,rs,L};
     if(work.keys[slot+1] != key) {
-      std::vector<int64_t> h(3*rows);
-      for(size_t i=0; i < rows; ++i) {
-        h[3*i]=(int64_t) (i*rs);
-        h[3*i+1]=(int64_t) L;
-        h[3*i+2]=(int64_t) (i*L);
-      }
-      check(fftwpp_amd_memcpy(items,h.data(),h.size()*sizeof(int64_t),stream),"item table");
-      check(fftwpp_amd_synchronize(stream),"item table");
+      // item i = {i rs, L, i L}, written on the device (no host staging)
+      check(fftwpp_amd_internal_row_items(items,rows,rs,L,stream),"item table");
       work.keys[slot+1]=key;
     }
     const size_t G=(q+D-1)/D;

@@ -666,6 +666,66 @@ __global__ void hplane_kernel(int op, int64_t Hx, int64_t Hy, int64_t x0,
   } else *l=mk((T) 0,(T) 0);
 }
 
+// Reference-layout residue blocks (fftBase::forward/backward of the C++
+// face).  Entry e of a block reads or writes C columns at word offset
+// tab[2e+1] of the block array (complex words, or reals when `real`), and
+// its value lives at word (tab[2e] * C + c) of the residue buffer T, where
+// tab[2e] = slot m + s for the residue in slot `slot` at sub-FFT position s.
+__global__ void block_gather_kernel(const cplx<double> *T, const uint64_t *tab,
+                                    int64_t ne, int64_t C, void *out, int real)
+{
+  const int64_t i=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  if(i >= ne*C) return;
+  const int64_t e=i/C, c=i-e*C;
+  const cplx<double> v=T[(int64_t) tab[2*e]*C+c];
+  const int64_t w=(int64_t) tab[2*e+1]+c;
+  if(real) reinterpret_cast<double *>(out)[w]=v.x;
+  else reinterpret_cast<cplx<double> *>(out)[w]=v;
+}
+
+__global__ void block_scatter_kernel(cplx<double> *T, const uint64_t *tab, int64_t ne,
+                                     int64_t C, const void *in, int real)
+{
+  const int64_t i=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  if(i >= ne*C) return;
+  const int64_t e=i/C, c=i-e*C;
+  const int64_t w=(int64_t) tab[2*e+1]+c;
+  T[(int64_t) tab[2*e]*C+c]=real ? mk(reinterpret_cast<const double *>(in)[w],0.0) :
+    reinterpret_cast<const cplx<double> *>(in)[w];
+}
+
+// Hermitian data of H stored modes (C columns) to the full centred sequence
+// of 2H - 1 modes k = -(H-1) ... H-1 (op 0: f(-k) = conj f(k)); op 1 the
+// way back, h(k) (+)= full(k) for k >= 0 (accumulate when acc).
+__global__ void herm_expand_kernel(int op, const cplx<double> *a, cplx<double> *b, int64_t H,
+                                   int64_t C, int acc)
+{
+  const int64_t i=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  if(op == 0) {
+    if(i >= (2*H-1)*C) return;
+    const int64_t j=i/C, c=i-j*C, k=j-(H-1);
+    b[i]=k >= 0 ? a[k*C+c] : conjg(a[-k*C+c]);
+  } else {
+    if(i >= H*C) return;
+    const cplx<double> v=a[i+(H-1)*C];
+    b[i]=acc ? b[i]+v : v;
+  }
+}
+
+// Item table {i rs, L, i L} of `rows` rows of length L at row stride rs (the
+// rows of a 2D/3D inner stage run as the items of the batched row kernel).
+__global__ void row_items_kernel(BatchItem *items, int64_t rows, int64_t rs, int64_t L)
+{
+  const int64_t i=(int64_t) blockIdx.x*blockDim.x+threadIdx.x;
+  if(i < rows) {
+    BatchItem it;
+    it.off=i*rs;
+    it.L=L;
+    it.poff=i*L;
+    items[i]=it;
+  }
+}
+
 // ------------------------------------------------------------------------
 // Host-side helpers of the C ABI
 // ------------------------------------------------------------------------
@@ -1268,6 +1328,45 @@ static int hrows_long_t(const fftwpp_amd_dim *d, const void *f, const void *g,
 }
 
 extern "C" {
+
+int fftwpp_amd_internal_block_map(int scatter, const void *T, const void *tab,
+                                   size_t ne, size_t C, void *arr, int real,
+                                   void *stream)
+{
+  if(!T || !tab || !arr) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  if(ne == 0 || C == 0) return FFTWPP_AMD_OK;
+  const unsigned nb=(unsigned) ((ne*C+255)/256);
+  hipStream_t st=(hipStream_t) stream;
+  if(scatter)
+    hipLaunchKernelGGL(block_scatter_kernel,dim3(nb),dim3(256),0,st,(cplx<double> *) T,
+                       (const uint64_t *) tab,(int64_t) ne,(int64_t) C,(const void *) arr,real);
+  else
+    hipLaunchKernelGGL(block_gather_kernel,dim3(nb),dim3(256),0,st,(const cplx<double> *) T,
+                       (const uint64_t *) tab,(int64_t) ne,(int64_t) C,arr,real);
+  return launch_check("block map kernel");
+}
+
+int fftwpp_amd_internal_herm_expand(int op, const void *a, void *b, size_t H, size_t C,
+                                    int acc, void *stream)
+{
+  if(!a || !b || H == 0) return fail(FFTWPP_AMD_ERR_ARG,"bad argument");
+  const size_t n=(op == 0 ? 2*H-1 : H)*C;
+  hipLaunchKernelGGL(herm_expand_kernel,dim3((unsigned) ((n+255)/256)),dim3(256),0,
+                     (hipStream_t) stream,op,(const cplx<double> *) a,(cplx<double> *) b,
+                     (int64_t) H,(int64_t) C,acc);
+  return launch_check("herm_expand_kernel");
+}
+
+int fftwpp_amd_internal_row_items(void *items, size_t rows, size_t rs, size_t L,
+                                  void *stream)
+{
+  if(!items) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  if(rows == 0) return FFTWPP_AMD_OK;
+  hipLaunchKernelGGL(row_items_kernel,dim3((unsigned) ((rows+255)/256)),dim3(256),0,
+                     (hipStream_t) stream,(BatchItem *) items,(int64_t) rows,(int64_t) rs,
+                     (int64_t) L);
+  return launch_check("row_items_kernel");
+}
 
 const char *fftwpp_amd_last_error(void) { return last_error; }
 

@@ -130,6 +130,17 @@ public:
     D(D), I(I), maxthreads(parent.maxthreads) { check(); }
 };
 
+// Device staging buffers shared by the convolution drivers.
+class DeviceWork {
+public:
+  std::vector<void *> bufs;
+  std::vector<size_t> sizes;
+  std::vector<std::vector<size_t>> keys;  // what a slot holds (empty: nothing known), e.g. an item table
+  void *get(size_t slot, size_t bytes);
+  ~DeviceWork();
+
+};
+
 static const struct centeredConstructor {} noOptimize={};
 
 class fftBase : public ThreadBase {
@@ -204,17 +215,37 @@ public:
   virtual bool realInput() const { return false; }
 
   // Residue block r of the padded transform of DEVICE data f (stride S,
-  // count C) into F, laid out as the reference's routines lay it out:
-  // F[S*k+c] holds the padded transform at index(r,k), k < noutputs(r).
+  // count C; Hermitian: C columns of ceil(L/2) modes) into F, laid out as
+  // the reference's routines lay it out (convolve.cc:771-5447, read by
+  // tests/hybrid.cc:60-110 and tests/hybridh.cc:60-100):
+  //   complex and centred: F[S*k+c] holds the padded transform at
+  //     index(r,k), k < noutputs(r), conjugate pairs of D > 1 blocks included;
+  //   Hermitian: copy d < (r == 0 ? D0 : D) of the block holds the real
+  //     padded transform at index(r, k + noutputs(0) d) in real word C*k+c
+  //     of (double *) (F + b*d);
+  //   overwrite (in-place) plans: forward(f,F,0) computes every block at
+  //     once, block 0 over f, the middle blocks over f + b, the last in F.
+  // Runs on the default stream.
   void forward(Complex *f, Complex *F, size_t r=0, Complex *W=NULL);
-  // Adjoint of forward: the r=0 block writes f, later blocks accumulate.
+  // Adjoint of forward over the same layout: the r=0 block writes f, later
+  // blocks accumulate (overwrite plans: every block at r = 0).
   void backward(Complex *F, Complex *f, size_t r=0, Complex *W=NULL);
   void pad(Complex *) {}
 
 protected:
   void initDevice();
-  void blockResidues(size_t r, std::vector<size_t> &res,
-                     std::vector<uint32_t> &map);
+  // The reference-layout face: entries of block r (padded position, array,
+  // word offset), their residues and device tables, built once per block.
+  struct FaceBlock {
+    bool ready=false;
+    std::vector<size_t> res;        // residues of the block, by slot
+    void *tab[3]={NULL,NULL,NULL};  // per array: {slot m + s, word} pairs
+    size_t ne[3]={0,0,0};
+  };
+  std::vector<FaceBlock> faceBlocks;
+  fftwpp_amd_dim *hdev=NULL;        // centred twin of a Hermitian dimension
+  DeviceWork faceWork;
+  FaceBlock &faceBlock(size_t r);
 };
 
 class fftPad : public fftBase {
@@ -310,15 +341,6 @@ private:
   void record();
 };
 
-// Device staging buffers shared by the convolution drivers.
-class DeviceWork {
-public:
-  std::vector<void *> bufs;
-  std::vector<size_t> sizes;
-  std::vector<std::vector<size_t>> keys;  // what a slot holds (empty: nothing known), e.g. an item table
-  void *get(size_t slot, size_t bytes);
-  ~DeviceWork();
-};
 
 // A convolution with a user multiplier on contiguous complex device data of
 // nd dimensions (dims outermost first, complex or centred): every input

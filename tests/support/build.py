@@ -1,5 +1,6 @@
 """Build tests/support/libtestmult.so: the user device multipliers of
-multipliers.hip and the C++ FFT face of fftface.cc, linked against the in-tree libfftwpp_amd.so (found next to
+multipliers.hip, the C++ FFT face of fftface.cc and the reference transform
+tests of residues.cc, linked against the in-tree libfftwpp_amd.so (found next to
 the package at run time through $ORIGIN)."""
 import os
 import subprocess
@@ -16,6 +17,7 @@ def build(verbose=False):
     pkg = os.path.join(ROOT, "fftwpp_amd")
     cmd = [b.HIPCC] + b.HIP_FLAGS + ["-shared", os.path.join(HERE, "multipliers.hip"),
                                      os.path.join(HERE, "fftface.cc"),
+                                     os.path.join(HERE, "residues.cc"),
                                      "-L", pkg, "-lfftwpp_amd",
                                      "-Wl,-rpath,$ORIGIN/../../fftwpp_amd",
                                      "-o", LIB + ".tmp"]
