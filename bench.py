@@ -71,6 +71,9 @@ STAGE_UNITS = {
     "x forward f": 1 + 2, "x forward g": 1 + 2,
     "y forward f": 2 + 4, "y forward g": 2 + 4,
     "z rows": 8 + 4, "y backward": 4 + 2, "x backward": 2 + 1,
+    # one launch for y forward, z rows and y backward of every x-plane: reads
+    # both x-transformed arrays, writes the plane results
+    "y-z planes": 4 + 2,
 }
 CONV_UNITS = 15  # (A+B)(1 + 2 rho_x), A=2, B=1, rho_x=2
 
@@ -114,6 +117,43 @@ def roofline_of(stages, E, L3):
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / PEAK_GBS, 4), "traffic": None, "kernel": name,
             "bytes_alg_per_launch": bytes_alg, "avg_launch_ms": round(stages[name], 4)}
+
+
+def bench_plane_engine(torch, f, g, L, H64, steps, warmup, stream):
+    """The 3D workload under a memory limit that selects the plane kernel for
+    the y-z stage (2 x-transformed arrays plus its L2 exchange areas instead
+    of 2 + 2 q_y arrays): conv/s, work bytes of both engines, and the
+    difference from the default engine's FP64 result."""
+    import numpy as np
+    from fftwpp_amd.torch_conv import DeviceConvolution
+    dev = torch.device("cuda:0")
+    out = {}
+    for key, dt, E in (("f64", torch.complex128, 16), ("f32", torch.complex64, 8)):
+        a = torch.tensor(f, device=dev, dtype=dt)
+        b = torch.tensor(g, device=dev, dtype=dt)
+        o = torch.empty_like(a)
+        plan = DeviceConvolution((L, L, L), dtype=dt, M=[2 * L] * 3)
+        full, _ = plan.work_bytes()
+        plan.set_memory_limit(3 * 2 * L * L * L * E)
+        need, plane = plan.work_bytes()
+        if not plane:
+            return {"note": "plane kernel not available on this device"}
+        ms, st = time_plan(torch, plan, o, a, b, steps, warmup, stream)
+        H = o.cpu().numpy().astype(np.complex128)
+        err = float(np.linalg.norm((H - H64).ravel()) / np.linalg.norm(H64.ravel()))
+        pk = st.get("y-z planes", 0.0)
+        out[key] = {"conv_per_s": round(1000.0 / ms, 3), "ms_per_conv": round(ms, 4),
+                    "work_bytes": need, "work_bytes_default_engine": full,
+                    "plane_kernel_ms": round(pk, 4),
+                    "plane_kernel_alg_GBs": round(STAGE_UNITS["y-z planes"] * E * L ** 3 / (pk * 1e-3) / 1e9, 1)
+                    if pk else None,
+                    "rel_l2_vs_default_fp64": err,
+                    "stages_ms": {k: round(v, 4) for k, v in st.items()}}
+        del a, b, o, plan
+        torch.cuda.empty_cache()
+    out["note"] = ("memory-capped engine: HBM traffic of the 15-unit model, about a third of "
+                   "the work memory; slower than the default engine on one MI355X")
+    return out
 
 
 def sampled_direct_error(f, g, H, npts=6, seed=7):
@@ -183,7 +223,7 @@ def cpu_library_baseline(f, g, L):
 # Kernel of each stage of the 3D plan, as named in rocprofv3 records.
 STAGE_KERNEL = {"z rows": "rowconv256p_kernel", "x forward": "colfwd256_kernel",
                 "y forward": "colfwd256_kernel", "y backward": "colbwd256_kernel",
-                "x backward": "colbwd256_kernel"}
+                "x backward": "colbwd256_kernel", "y-z planes": "plane256_kernel"}
 COPY_BYTES = 512 << 20
 
 
@@ -285,6 +325,12 @@ def bench_batched(torch, dev, steps, warmup, per_call=True):
     res = {"workload": "4096 1D complex convolutions, L log-uniform in [64,16384] "
                        f"(sum L = {total}), M = 2L, multBinary, uniform[-1,1] seed {SEED + 4}"}
     outs = {}
+    import tempfile
+    cache = tempfile.NamedTemporaryFile(prefix="fftwpp_amd_tune_", suffix=".json", delete=False)
+    cache.close()
+    os.unlink(cache.name)
+    saved = os.environ.get("FFTWPP_AMD_TUNE_CACHE")
+    os.environ["FFTWPP_AMD_TUNE_CACHE"] = cache.name   # a fresh cache: the first plan tunes
     for dt, key, E in [(torch.complex128, "f64", 16), (torch.complex64, "f32", 8)]:
         f = torch.tensor(fh, dtype=dt, device=dev)
         g = torch.tensor(gh, dtype=dt, device=dev)
@@ -292,6 +338,10 @@ def bench_batched(torch, dev, steps, warmup, per_call=True):
         t0 = time.perf_counter()
         plan = BatchedConvolution(Ls, M=2 * Ls, dtype=dt, tune=True, graph=True)
         tune_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        again = BatchedConvolution(Ls, M=2 * Ls, dtype=dt, tune=True, graph=True)
+        cached_s = time.perf_counter() - t0
+        del again
         ms = _time_calls(torch, lambda: plan(f, g, out), steps, warmup)
         outs[key] = out.clone()
         eager = BatchedConvolution(Ls, M=2 * Ls, dtype=dt, tune=False, graph=False)
@@ -301,6 +351,7 @@ def bench_batched(torch, dev, steps, warmup, per_call=True):
              "achieved_GBs": round(bytes_alg / (ms * 1e-3) / 1e9, 1),
              "frac_of_hbm_peak": round(bytes_alg / (ms * 1e-3) / 1e9 / PEAK_GBS, 4),
              "bytes_alg": bytes_alg, "tune_s": round(tune_s, 2),
+             "create_from_tune_cache_s": round(cached_s, 4),
              "untuned_eager_ms_per_batch": round(ms_e, 4),
              "classes": plan.describe().strip().splitlines()}
         if per_call and key == "f64":
@@ -322,6 +373,12 @@ def bench_batched(torch, dev, steps, warmup, per_call=True):
         res[key] = r
         del f, g, out
         torch.cuda.empty_cache()
+    if saved is None:
+        del os.environ["FFTWPP_AMD_TUNE_CACHE"]
+    else:
+        os.environ["FFTWPP_AMD_TUNE_CACHE"] = saved
+    if os.path.exists(cache.name):
+        os.unlink(cache.name)
     a, b = outs["f64"], outs["f32"].to(torch.complex128)
     res["f32"]["rel_l2_vs_f64"] = float(torch.linalg.norm(a - b) / torch.linalg.norm(a))
     return res
@@ -597,6 +654,8 @@ def main():
         torch.cuda.empty_cache()
 
     if not args.no_extra:
+        result["plane_engine"] = bench_plane_engine(torch, f, g, L, H64, min(args.steps, 10),
+                                                    2, stream)
         result["batched"] = bench_batched(torch, dev, args.steps, args.warmup)
         result["conv1d"] = bench_conv1d(torch, dev, args.steps)
         result["hconv2d"] = bench_hconv2d(torch, dev, args.steps, args.warmup)

@@ -25,6 +25,10 @@ SIGNATURES = {
     "fftwpp_amd_forward_many": (I, [P, I, P, Z, Z, Z, Z, P, Z, Z, Z, Z, P]),
     "fftwpp_amd_backward_many": (I, [P, I, P, Z, Z, P, Z, Z, Z, Z, Z, Z, I, D, P]),
     "fftwpp_amd_conv_rows": (I, [P, I, I, P, P, Z, Z, P, Z, D, P]),
+    "fftwpp_amd_plane_work": (Z, [I]),
+    "fftwpp_amd_plane_supported": (I, [P, P]),
+    "fftwpp_amd_plane_conv": (I, [P, P, I, I, P, P, P, Z, Z, Z, P, P]),
+    "fftwpp_amd_plane_status": (I, [P, P]),
     "fftwpp_amd_hconv_rows_long_part": (Z, [P, Z]),
     "fftwpp_amd_hconv_rows_long": (I, [P, I, P, P, Z, Z, P, Z, P, D, P]),
     "fftwpp_amd_batch_rows_lds": (Z, [P, I, Z, Z]),
@@ -53,6 +57,8 @@ SIGNATURES = {
     # convolution layer
     "fftwpp_amd_conv_create": (I, [ctypes.POINTER(P), I, I, ZP, ZP, ZP, I]),
     "fftwpp_amd_conv_destroy": (I, [P]),
+    "fftwpp_amd_conv_set_memory_limit": (I, [P, Z]),
+    "fftwpp_amd_conv_work_bytes": (I, [P, ZP, ctypes.POINTER(ctypes.c_int)]),
     "fftwpp_amd_conv_execute": (I, [P, P, P, P]),
     "fftwpp_amd_conv_adjoint": (I, [P, P, P, P, P, P, P]),
     "fftwpp_amd_conv_geometry": (I, [P, ZP, ZP, ZP]),

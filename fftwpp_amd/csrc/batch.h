@@ -141,6 +141,15 @@ template<typename T> struct BatchV { static constexpr int V=sizeof(T) == 8 ? 8 :
 // inside fftr() see the same calls from every thread.  Dynamic LDS: M
 // complex twiddles + slots * 2 RPlan<M,V>::PAD scalar words (one-round
 // exchanges).
+//
+// D = 0 selects the sequence mode: a work unit is one residue r of an item
+// and writes its inverse-transformed, conjugate-twiddled sequence
+// Z_r(sigma) = w_N^{-sigma r} IFFT_r(sigma), sigma < M, to part[i q M + r M]
+// (N words per item instead of partial sums of the unfolded outputs);
+// batch_unfold_kernel then forms out(sigma + M b) = sum_r w_q^{-b r} Z_r(sigma),
+// the backward fold over the blocks b of the reference's backward transforms
+// (convolve.cc:1468-1613, 1765-1965), so no output word is read and written
+// once per residue.
 template<typename T, int M>
 __global__ void __launch_bounds__(512)
 batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
@@ -158,6 +167,8 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
   const int slots=blockDim.x/TPS;
   const int slot=threadIdx.x/TPS, t=threadIdx.x-slot*TPS;
   T *xb=reinterpret_cast<T *>(tw+M)+slot*2*P::PAD;
+  const bool zm=D == 0;
+  if(zm) D=1;
   const int64_t G=(d.q+D-1)/D;
   const int64_t units=count*G;
   for(int64_t ub=(int64_t) blockIdx.x*slots; ub < units; ub += (int64_t) gridDim.x*slots) {
@@ -224,6 +235,12 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
 #pragma unroll
         for(int k=0; k < V; ++k) X[k]=mulc(X[k],zN[(t+TPS*k)*r]);
       }
+      if(zm) {
+        cplx<T> *z=part+(ii*qi+r)*(int64_t) M;
+#pragma unroll
+        for(int k=0; k < V; ++k) z[t+TPS*k]=X[k];
+        continue;
+      }
       const bool first=r == rbeg, last=r == rend-1;
       for(int jb=0, eb=0; jb < L; jb += M) {
         const cplx<T> wb=zN[eb*M];
@@ -238,6 +255,33 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
         }
         eb += r;
         if(eb >= qi) eb -= qi;
+      }
+    }
+  }
+}
+
+// out[off + sigma + M b] = scale * sum_{r < q} w_q^{-b r} Z[i q M + r M + sigma]
+// for the blocks b of each item (sequence mode of batch_reg_kernel); one
+// block per item (grid-stride), threads over sigma.
+template<typename T>
+__global__ void __launch_bounds__(256)
+batch_unfold_kernel(const BatchItem *items, int64_t count, const cplx<T> *Z,
+                    int q, int M, const cplx<T> *zN, cplx<T> *out, T scale)
+{
+  for(int64_t i=blockIdx.x; i < count; i += gridDim.x) {
+    const BatchItem it=items[i];
+    const cplx<T> *zi=Z+i*(int64_t) q*M;
+    const int L=(int) it.L;
+    for(int sg=threadIdx.x; sg < M; sg += blockDim.x) {
+      for(int b=0, jb=0; jb < L; ++b, jb += M) {
+        if(jb+sg >= L) break;
+        cplx<T> v=zi[sg];
+        for(int r=1, e=b; r < q; ++r) {
+          v=v+mulc(zi[r*M+sg],zN[(int64_t) e*M]);    // w_q^{-b r}, e = b r mod q
+          e += b;
+          if(e >= q) e -= q;
+        }
+        out[it.off+jb+sg]=scale*v;
       }
     }
   }

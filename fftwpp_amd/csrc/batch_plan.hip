@@ -21,11 +21,16 @@
 // the reference's CPU timing optimizer (convolve.cc:177-470).  Without it a
 // fixed rule is used: m = min(2^ceil(log2 L), max_m).
 #include <hip/hip_runtime.h>
+#include <sys/stat.h>
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <map>
+#include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -129,7 +134,18 @@ void free_class(Cls &c)
   c.dim=nullptr;
 }
 
-size_t groups(const Cls &c, const Tile &t) { return ceilquot(c.q,t.D); }
+// Work units per item: residue groups of D residues, or q single residues in
+// the sequence mode (D = 0).
+size_t groups(const Cls &c, const Tile &t) { return t.D ? ceilquot(c.q,t.D) : c.q; }
+
+// Words of the class's partial buffer for tile t: G slices of the outputs, or
+// q m per item in the sequence mode; 0 when the tile writes out directly.
+size_t part_words(const Cls &c, const Tile &t)
+{
+  if(t.D == 0) return c.ids.size()*c.q*c.m;
+  size_t G=groups(c,t);
+  return G > 1 ? G*c.ptotal : 0;
+}
 
 // Group items by (m, q) for the sub-FFT sizes `mof`; creates twiddle tables
 // and device item lists.
@@ -186,7 +202,7 @@ Tile make_tile(const fftwpp_amd_batch &h, const Cls &c, size_t TR,
   Tile t;
   t.TR=TR;
   t.threads=threads;
-  t.D=std::min(D,c.q);
+  t.D=std::min(D,c.q);   // D = 0: the sequence mode
   t.lacc=fftwpp_amd_batch_rows_lds(c.dim,h.dtype,TR,c.Lmax) ? c.Lmax : 0;
   size_t units=ceilquot(c.ids.size(),TR)*groups(c,t);
   // the register path runs threads/tps work units per block at once
@@ -200,14 +216,16 @@ Tile make_tile(const fftwpp_amd_batch &h, const Cls &c, size_t TR,
   return t;
 }
 
-// Residue group sizes tried: q, q/2, q/4, q/8 (rounded up), at most 8 groups.
-std::vector<size_t> group_sizes(size_t q)
+// Residue group sizes tried: q, q/2, q/4, q/8 (rounded up), at most 8 groups,
+// and for the register kernel with q > 1 the sequence mode (D = 0).
+std::vector<size_t> group_sizes(size_t q, bool reg)
 {
   std::vector<size_t> v;
   for(size_t G=1; G <= 8 && G <= q; G *= 2) {
     size_t D=ceilquot(q,G);
     if(std::find(v.begin(),v.end(),D) == v.end()) v.push_back(D);
   }
+  if(reg && q > 1) v.push_back(0);
   return v;
 }
 
@@ -245,14 +263,15 @@ int launch_class(const fftwpp_amd_batch &h, const Cls &c, const Tile &t,
                                1.0/(double) (c.m*c.q),s);
 }
 
-// Partial-sum buffer for the class's chosen tile (none when G = 1).
+// Partial-sum (or sequence) buffer for the class's chosen tile (none when
+// the tile writes out directly).
 int alloc_part(const fftwpp_amd_batch &h, Cls &c)
 {
   if(c.part) { (void) hipFree(c.part); c.part=nullptr; }
-  size_t G=groups(c,c.tile);
-  if(G <= 1) return FFTWPP_AMD_OK;
+  size_t n=part_words(c,c.tile);
+  if(n == 0) return FFTWPP_AMD_OK;
   size_t elt=h.dtype == FFTWPP_AMD_F32 ? 8 : 16;
-  return bhip(hipMalloc(&c.part,G*c.ptotal*elt),"hipMalloc partial sums");
+  return bhip(hipMalloc(&c.part,n*elt),"hipMalloc partial sums");
 }
 
 // Average time of one launch (ms), or a negative value on failure.
@@ -281,7 +300,10 @@ double tune_class(const fftwpp_amd_batch &h, Cls &c, void *part, void *out,
 {
   double best=-1;
   Tile bt=default_tile(h,c);
-  for(size_t D : group_sizes(c.q)) {
+  const bool reg=fftwpp_amd_batch_rows_tps(c.dim,h.dtype) != 0;
+  for(size_t D : group_sizes(c.q,reg)) {
+    // the timing arrays hold 8 slices of the batch for partial results
+    if(D == 0 && c.ids.size()*c.q*c.m > 8*h.total) continue;
     for(size_t TR=1; TR <= 64; TR *= 2) {
       if(TR > 1 && TR*c.m > 16384) break;
       if(TR > 1 && TR > c.ids.size()) break;
@@ -415,6 +437,137 @@ int tune_geometry(fftwpp_amd_batch &h, TuneArrays &ta)
   return rc;
 }
 
+// ------------------------------------------------------------------------
+// Persisted tuning results (the role of FFTW wisdom, fftw++.cc:31-72, for the
+// batch autotuner): a JSON file of entries keyed by a hash of (precision,
+// multiplier, device architecture and CU count, the item lengths L and
+// padded lengths M), each holding the chosen sub-FFT size per length bucket
+// and the tile of every class.  A tuned plan whose key is present is built
+// from the entry without timing; otherwise the tuner runs and appends its
+// result.  The file is FFTWPP_AMD_TUNE_CACHE when set (empty: no cache),
+// else $XDG_CACHE_HOME/fftwpp_amd/batch_tune.json or
+// ~/.cache/fftwpp_amd/batch_tune.json.  One entry per line:
+//   {"key":"<16 hex>","buckets":[[b,m],...],"tiles":[[m,q,TR,threads,D,nblocks,lacc],...]}
+// ------------------------------------------------------------------------
+std::string cache_path()
+{
+  if(const char *e=getenv("FFTWPP_AMD_TUNE_CACHE")) return e;
+  std::string base;
+  if(const char *x=getenv("XDG_CACHE_HOME")) base=x;
+  else if(const char *hm=getenv("HOME")) base=std::string(hm)+"/.cache";
+  else return "";
+  return base+"/fftwpp_amd/batch_tune.json";
+}
+
+uint64_t fnv1a(uint64_t hsh, const void *data, size_t n)
+{
+  const unsigned char *p=(const unsigned char *) data;
+  for(size_t i=0; i < n; ++i) { hsh ^= p[i]; hsh *= 1099511628211ull; }
+  return hsh;
+}
+
+std::string tune_key(const fftwpp_amd_batch &h)
+{
+  uint64_t k=1469598103934665603ull;
+  int dev=0;
+  hipDeviceProp_t prop;
+  memset(&prop,0,sizeof(prop));
+  if(hipGetDevice(&dev) == hipSuccess) (void) hipGetDeviceProperties(&prop,dev);
+  const int vals[3]={h.dtype,h.mult,h.cus};
+  k=fnv1a(k,vals,sizeof(vals));
+  k=fnv1a(k,prop.gcnArchName,strnlen(prop.gcnArchName,sizeof(prop.gcnArchName)));
+  k=fnv1a(k,&h.n,sizeof(h.n));
+  k=fnv1a(k,h.L.data(),h.n*sizeof(size_t));
+  k=fnv1a(k,h.M.data(),h.n*sizeof(size_t));
+  char buf[24];
+  snprintf(buf,sizeof(buf),"%016llx",(unsigned long long) k);
+  return buf;
+}
+
+struct TuneEntry {
+  std::map<size_t,size_t> buckets;        // ceilpow2(L) -> m
+  std::vector<std::vector<size_t>> tiles; // m, q, TR, threads, D, nblocks, lacc
+};
+
+std::mutex cache_mu;
+
+// Integers of the bracketed list that follows `name` in line (nested lists
+// flattened), or false when absent.
+bool json_ints(const std::string &line, const char *name, std::vector<std::vector<size_t>> &out)
+{
+  size_t at=line.find(std::string("\"")+name+"\"");
+  if(at == std::string::npos) return false;
+  at=line.find('[',at);
+  if(at == std::string::npos) return false;
+  int depth=0;
+  std::vector<size_t> cur;
+  std::string num;
+  for(size_t i=at; i < line.size(); ++i) {
+    const char ch=line[i];
+    if(ch >= '0' && ch <= '9') { num += ch; continue; }
+    if(!num.empty()) { cur.push_back(strtoull(num.c_str(),nullptr,10)); num.clear(); }
+    if(ch == '[') { ++depth; if(depth == 2) cur.clear(); }
+    else if(ch == ']') {
+      if(depth == 2) out.push_back(cur);
+      if(--depth == 0) return true;
+    }
+  }
+  return false;
+}
+
+bool cache_lookup(const std::string &key, TuneEntry &e)
+{
+  const std::string path=cache_path();
+  if(path.empty()) return false;
+  std::lock_guard<std::mutex> lock(cache_mu);
+  std::ifstream in(path);
+  std::string line, want="\"key\":\""+key+"\"";
+  while(std::getline(in,line)) {
+    if(line.find(want) == std::string::npos) continue;
+    std::vector<std::vector<size_t>> b, t;
+    if(!json_ints(line,"buckets",b) || !json_ints(line,"tiles",t)) continue;
+    e=TuneEntry();
+    for(auto &v : b) if(v.size() == 2) e.buckets[v[0]]=v[1];
+    for(auto &v : t) if(v.size() == 7) e.tiles.push_back(v);
+    return true;
+  }
+  return false;
+}
+
+void cache_store(const std::string &key, const fftwpp_amd_batch &h)
+{
+  const std::string path=cache_path();
+  if(path.empty()) return;
+  std::ostringstream os;
+  os << "{\"key\":\"" << key << "\",\"buckets\":[";
+  std::map<size_t,size_t> b;
+  for(size_t i=0; i < h.n; ++i) b[ceilpow2(h.L[i])]=h.mof[i];
+  bool first=true;
+  for(auto &kv : b) {
+    os << (first ? "" : ",") << "[" << kv.first << "," << kv.second << "]";
+    first=false;
+  }
+  os << "],\"tiles\":[";
+  first=true;
+  for(const Cls &c : h.cls) {
+    const Tile &t=c.tile;
+    os << (first ? "" : ",") << "[" << c.m << "," << c.q << "," << t.TR << "," << t.threads
+       << "," << t.D << "," << t.nblocks << "," << t.lacc << "]";
+    first=false;
+  }
+  os << "]}\n";
+  std::lock_guard<std::mutex> lock(cache_mu);
+  const size_t slash=path.rfind('/');
+  if(slash != std::string::npos) {
+    // create the directory chain (errors surface as a failed open below)
+    std::string dir=path.substr(0,slash);
+    for(size_t i=1; i <= dir.size(); ++i)
+      if(i == dir.size() || dir[i] == '/') (void) mkdir(dir.substr(0,i).c_str(),0755);
+  }
+  std::ofstream out(path,std::ios::app);
+  if(out) out << os.str();
+}
+
 } // namespace
 
 extern "C" {
@@ -472,23 +625,57 @@ int fftwpp_amd_batch_create(fftwpp_amd_batch **out, size_t n, const size_t *L,
   for(size_t i=0; i < n; ++i) h->mof[i]=std::min(ceilpow2(h->L[i]),maxm);
   const bool tuned=flags & FFTWPP_AMD_BATCH_TUNE;
   TuneArrays ta;
+  bool timing=false;                      // ta holds the timing arrays
+  // a stored tuning result for this batch, if any
+  std::string key;
+  TuneEntry entry;
+  bool cached=false;
   if(tuned) {
-    rc=ta.init(*h);
-    if(!rc) rc=tune_geometry(*h,ta);
-    if(rc) { ta.release(*h); destroy(h); return rc; }
+    key=tune_key(*h);
+    cached=cache_lookup(key,entry);
+    for(size_t i=0; i < n && cached; ++i) {
+      auto it=entry.buckets.find(ceilpow2(h->L[i]));
+      if(it == entry.buckets.end() || it->second == 0 || it->second > maxm) cached=false;
+      else h->mof[i]=it->second;
+    }
+    if(!cached) {
+      for(size_t i=0; i < n; ++i) h->mof[i]=std::min(ceilpow2(h->L[i]),maxm);
+      rc=ta.init(*h);
+      timing=true;
+      if(!rc) rc=tune_geometry(*h,ta);
+      if(rc) { ta.release(*h); destroy(h); return rc; }
+    }
   }
   std::vector<size_t> all(n);
   for(size_t i=0; i < n; ++i) all[i]=i;
   rc=build_classes(*h,all,h->mof,h->cls);
   std::sort(h->cls.begin(),h->cls.end(),[](const Cls &a, const Cls &b) { return a.work > b.work; });
+  bool complete=true;                     // every class's tile came from the cache
   for(Cls &c : h->cls) {
     if(rc) break;
+    bool have=false;
+    if(tuned && cached) {
+      for(const auto &v : entry.tiles) {
+        if(v[0] != c.m || v[1] != c.q) continue;
+        Tile t=make_tile(*h,c,std::max<size_t>(1,v[2]),v[3],v[4]);
+        if(v[5]) t.nblocks=v[5];
+        c.tile=t;
+        c.ms=-1;
+        have=true;
+        break;
+      }
+    }
     if(!tuned) c.tile=default_tile(*h,c);
-    else if(ta.time(*h,c) < 0) rc=bfail(FFTWPP_AMD_ERR_HIP,"batch tuning found no working launch");
+    else if(!have) {
+      complete=false;
+      if(!timing) { rc=ta.init(*h); timing=true; }
+      if(!rc && ta.time(*h,c) < 0) rc=bfail(FFTWPP_AMD_ERR_HIP,"batch tuning found no working launch");
+    }
     if(!rc) rc=alloc_part(*h,c);
   }
-  if(tuned) ta.release(*h);
+  if(timing) ta.release(*h);
   if(rc) { destroy(h); return rc; }
+  if(tuned && !(cached && complete)) cache_store(key,*h);
   *out=h;
   return FFTWPP_AMD_OK;
 }

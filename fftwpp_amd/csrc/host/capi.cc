@@ -568,6 +568,31 @@ int fftwpp_amd_conv_create(fftwpp_amd_conv **h, int dim, int hermitian,
   return FFTWPP_AMD_OK;
 }
 
+int fftwpp_amd_conv_set_memory_limit(fftwpp_amd_conv *c, size_t bytes)
+{
+  if(!c) return FFTWPP_AMD_ERR_ARG;
+  if(!c->c3 || c->hermitian) {
+    fftwpp_amd_internal_set_error("fftwpp_amd_conv_set_memory_limit: 3D complex plans only");
+    return FFTWPP_AMD_ERR_UNSUPPORTED;
+  }
+  c->c3->convolve3->memoryLimit=bytes;
+  return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_conv_work_bytes(const fftwpp_amd_conv *c, size_t *bytes, int *plane)
+{
+  if(!c || !bytes) return FFTWPP_AMD_ERR_ARG;
+  if(!c->c3 || c->hermitian) {
+    fftwpp_amd_internal_set_error("fftwpp_amd_conv_work_bytes: 3D complex plans only");
+    return FFTWPP_AMD_ERR_UNSUPPORTED;
+  }
+  const Convolution3 *k=c->c3->convolve3;
+  const bool p=k->usesPlanes(c->dtype);
+  *bytes=k->workBytes(c->dtype,p);
+  if(plane) *plane=p ? 1 : 0;
+  return FFTWPP_AMD_OK;
+}
+
 int fftwpp_amd_conv_destroy(fftwpp_amd_conv *c)
 {
   if(!c) return FFTWPP_AMD_OK;

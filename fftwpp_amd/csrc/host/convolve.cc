@@ -1187,9 +1187,34 @@ Convolution3::Convolution3(fftBase *fftx, fftBase *ffty, fftBase *fftz,
   if(Sx < Ly*Sy) throw Error(FFTWPP_AMD_ERR_ARG,"Sx cannot be less than Ly*Sy");
   if(fftx->C != (contiguous() ? Ly*Sy : Lz)) throw Error(FFTWPP_AMD_ERR_ARG,"fftx->C is invalid");
   if(fftz->C != 1) throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner transform with C > 1");
+  // y and z both m = 256, q = 2, complex, rows of lz words back to back: the
+  // y-z stage of every x-plane runs as one plane-kernel launch
+  planes=contiguous() && Sx == Ly*Sy && !fftx->realInput() &&
+    fftwpp_amd_plane_supported(ffty->dev,fftz->dev);
 }
 
 Convolution3::~Convolution3() {}
+
+size_t Convolution3::workBytes(int dtype, bool plane) const
+{
+  const size_t E=eltsize(dtype);
+  const size_t X=fftx->q*fftx->m*Ly*lz*E;              // one x-transformed array
+  if(plane) return 2*X+fftwpp_amd_plane_work(dtype);
+  return 2*X+2*ffty->q*X;
+}
+
+// The plane kernel runs when it applies and either FFTWPP_AMD_PLANE=1 asks for
+// it or the memory limit excludes the separate y, z and y launches (which
+// are faster on one MI355X: their stages stream through HBM at near copy
+// speed, while the plane kernel's team barriers cost more than the HBM
+// traffic it saves).
+bool Convolution3::usesPlanes(int dtype) const
+{
+  if(!planes) return false;
+  const char *env=getenv("FFTWPP_AMD_PLANE");
+  if(env && env[0] == '1') return true;
+  return memoryLimit != 0 && workBytes(dtype,false) > memoryLimit;
+}
 
 size_t Convolution3::arraySize() const { return Sx*(Lx-1)+Sy*(Ly-1)+lz; }
 
@@ -1220,17 +1245,39 @@ void Convolution3::execComplex(void *out, const void *f0, const void *f1, bool n
   double s=normalize ? scale : 1.0;
   size_t E=eltsize(dtype);
   size_t qx=fftx->q, mx=fftx->m, qy=ffty->q, my=ffty->m;
-  size_t planes=qx*mx;           // x-transformed planes of Ly x lz
+  size_t nplanes=qx*mx;          // x-transformed planes of Ly x lz
   size_t plane=Ly*lz;
   size_t yplane=qy*my*lz;        // y-transformed plane
-  void *X0=work.get(0,planes*plane*E);
-  void *X1=work.get(1,planes*plane*E);
-  void *Y0=work.get(2,planes*yplane*E);
-  void *Y1=work.get(3,planes*yplane*E);
-  // x forward: columns (y,z); non-contiguous y strides become batches
-  if(timeline) timeline->begin(stream);
+  void *X0=work.get(0,nplanes*plane*E);
+  void *X1=work.get(1,nplanes*plane*E);
   const void *in[2]={f0,f1};
   void *X[2]={X0,X1};
+  const bool viaPlanes=code != FFTWPP_AMD_MULT_REAL_BINARY && usesPlanes(dtype);
+  if(memoryLimit && workBytes(dtype,viaPlanes) > memoryLimit)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"3D convolution: work buffers exceed the memory limit");
+  if(viaPlanes) {
+    // x forward (f, g) -> y-z plane convolutions (one launch) -> x backward
+    void *P=work.get(5,fftwpp_amd_plane_work(dtype));
+    if(timeline) timeline->begin(stream);
+    for(int a=0; a < 2; ++a) {
+      if(contiguous())
+        check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,plane,1,0,X[a],plane,0,0,qx,stream),"x forward");
+      else
+        check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,lz,Ly,Sy,X[a],plane,lz,0,qx,stream),"x forward");
+      if(timeline) timeline->mark(a == 0 ? "x forward f" : "x forward g");
+    }
+    check(fftwpp_amd_plane_conv(ffty->dev,fftz->dev,dtype,code,X0,X1,X0,nplanes,Ly,lz,P,stream),
+          "y-z planes");
+    if(timeline) timeline->mark("y-z planes");
+    check(fftwpp_amd_backward_many(fftx->dev,dtype,X0,plane,0,out,Sx,plane,1,0,0,qx,0,s,stream),
+          "x backward");
+    if(timeline) timeline->mark("x backward");
+    return;
+  }
+  void *Y0=work.get(2,nplanes*yplane*E);
+  void *Y1=work.get(3,nplanes*yplane*E);
+  // x forward: columns (y,z); non-contiguous y strides become batches
+  if(timeline) timeline->begin(stream);
   void *Y[2]={Y0,Y1};
   for(int a=0; a < 2; ++a) {
     if(contiguous())
@@ -1238,12 +1285,12 @@ void Convolution3::execComplex(void *out, const void *f0, const void *f1, bool n
     else
       check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,lz,Ly,Sy,X[a],plane,lz,0,qx,stream),"x forward");
     if(timeline) timeline->mark(a == 0 ? "x forward f" : "x forward g");
-    check(fftwpp_amd_forward_many(ffty->dev,dtype,X[a],lz,lz,planes,plane,Y[a],lz,yplane,0,qy,stream),"y forward");
+    check(fftwpp_amd_forward_many(ffty->dev,dtype,X[a],lz,lz,nplanes,plane,Y[a],lz,yplane,0,qy,stream),"y forward");
     if(timeline) timeline->mark(a == 0 ? "y forward f" : "y forward g");
   }
-  fusedRows(fftz,work,4,dtype,code,Y0,Y1,planes*qy*my,lz,Y0,lz,1.0,stream,"z rows");
+  fusedRows(fftz,work,4,dtype,code,Y0,Y1,nplanes*qy*my,lz,Y0,lz,1.0,stream,"z rows");
   if(timeline) timeline->mark("z rows");
-  check(fftwpp_amd_backward_many(ffty->dev,dtype,Y0,lz,yplane,X0,lz,lz,planes,plane,0,qy,0,1.0,stream),"y backward");
+  check(fftwpp_amd_backward_many(ffty->dev,dtype,Y0,lz,yplane,X0,lz,lz,nplanes,plane,0,qy,0,1.0,stream),"y backward");
   if(timeline) timeline->mark("y backward");
   if(contiguous())
     check(fftwpp_amd_backward_many(fftx->dev,dtype,X0,plane,0,out,Sx,plane,1,0,0,qx,0,s,stream),"x backward");

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 #include <algorithm>
 #include <mutex>
@@ -29,6 +30,7 @@
 #include "kernels256.h"
 #include "batch.h"
 #include "rowsreg.h"
+#include "plane256.h"
 #include "../../include/fftwpp_amd.h"
 
 using namespace fftwpp_amd;
@@ -1091,6 +1093,13 @@ static int batch_reg_t(const DimArgs &a, const BatchLaunch &b, int mult,
   default: return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"register FFT size");
   }
   if(rc) return rc;
+  if(b.D == 0) {
+    const size_t nb=b.count < 4096 ? b.count : 4096;
+    hipLaunchKernelGGL(batch_unfold_kernel<T>,dim3((unsigned) nb),dim3(256),0,st,
+                       b.items,(int64_t) b.count,(const cplx<T> *) b.part,(int) a.q,(int) a.m,
+                       (const cplx<T> *) a.zN,(cplx<T> *) b.out,(T) b.scale);
+    return launch_check("batch_unfold_kernel");
+  }
   const size_t G=(a.q+b.D-1)/b.D;
   if(G > 1) {
     const size_t nb=b.count < 4096 ? b.count : 4096;
@@ -1327,6 +1336,63 @@ static int hrows_long_t(const fftwpp_amd_dim *d, const void *f, const void *g,
   }
 }
 
+// ------------------------------------------------------------------------
+// The y-z plane stage of 3D convolutions (plane256.h): one cooperative
+// launch of 8 teams of W = CUs/8 blocks.
+// ------------------------------------------------------------------------
+static bool plane_dim(const fftwpp_amd_dim *d)
+{
+  return d && d->m == 256 && d->q == 2 && d->kind == FFTWPP_AMD_KIND_COMPLEX && d->L == 256;
+}
+
+static size_t plane_ctl_bytes() { return 8192; }
+static_assert(sizeof(PlaneCtl) <= 8192,"plane control block");
+
+template<typename T>
+static size_t plane_work_t() { return plane_ctl_bytes()+8*PlaneSub<T>::n*PLANE_E*sizeof(cplx<T>); }
+
+template<typename T, int MULT>
+static int plane_launch(const cplx<T> *Xf, const cplx<T> *Xg, cplx<T> *Xh,
+                        int64_t planes, const cplx<T> *zN, void *work, hipStream_t st)
+{
+  int dev=0, cus=0, per=0;
+  int rc=hipcheck(hipGetDevice(&dev),"hipGetDevice");
+  if(!rc) rc=hipcheck(hipDeviceGetAttribute(&cus,hipDeviceAttributeMultiprocessorCount,dev),
+                      "device attribute");
+  if(rc) return rc;
+  const size_t lds=plane_lds<T>();
+  if((rc=allow_lds(plane256_kernel<T,MULT>,lds))) return rc;
+  if((rc=hipcheck(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per,(const void *) plane256_kernel<T,MULT>,PLANE_THREADS,lds),
+                  "occupancy"))) return rc;
+  if(per < PlaneSub<T>::n || cus != 8*PLANE_W)
+    return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"plane kernel: needs 8 XCDs of 32 CUs, two resident blocks per CU");
+  PlaneCtl *ctl=(PlaneCtl *) work;
+  cplx<T> *E=(cplx<T> *) ((char *) work+plane_ctl_bytes());
+  if((rc=hipcheck(hipMemsetAsync(ctl,0,sizeof(PlaneCtl),st),"clear plane control"))) return rc;
+  void *args[]={(void *) &Xf,(void *) &Xg,(void *) &Xh,(void *) &planes,(void *) &zN,
+                (void *) &E,(void *) &ctl};
+  return hipcheck(hipLaunchCooperativeKernel((const void *) plane256_kernel<T,MULT>,
+                                             dim3((unsigned) (PlaneSub<T>::n*cus)),dim3(PLANE_THREADS),args,
+                                             (unsigned) lds,st),
+                  "plane256_kernel cooperative launch");
+}
+
+template<typename T>
+static int plane_conv_t(const fftwpp_amd_dim *dz, int mult, const void *Xf,
+                        const void *Xg, void *Xh, size_t planes, void *work,
+                        hipStream_t st)
+{
+  const cplx<T> *zN=(const cplx<T> *) (sizeof(T) == 8 ? dz->a64.zN : dz->a32.zN);
+  const cplx<T> *f=(const cplx<T> *) Xf, *g=(const cplx<T> *) Xg;
+  cplx<T> *h=(cplx<T> *) Xh;
+  if(mult == FFTWPP_AMD_MULT_CORRELATION)
+    return plane_launch<T,3>(f,g,h,(int64_t) planes,zN,work,st);
+  if(mult == FFTWPP_AMD_MULT_NONE)
+    return plane_launch<T,0>(f,g,h,(int64_t) planes,zN,work,st);
+  return plane_launch<T,1>(f,g,h,(int64_t) planes,zN,work,st);
+}
+
 extern "C" {
 
 int fftwpp_amd_internal_block_map(int scatter, const void *T, const void *tab,
@@ -1397,6 +1463,96 @@ int fftwpp_amd_internal_engine(const fftwpp_amd_dim *d, int stage)
   if(fast256(d)) return 1;
   if(stage == 0) return cols_reg(d) ? 2 : 0;
   return rows_reg(d) ? 2 : 0;
+}
+
+size_t fftwpp_amd_plane_work(int dtype)
+{
+  return dtype == FFTWPP_AMD_F32 ? plane_work_t<float>() : plane_work_t<double>();
+}
+
+// Whether the plane kernel runs on the current device for dimensions dy, dz:
+// the geometry rule, and once per device a launch with no planes that forms
+// the teams and checks that every XCD holds blocks of the grid (this
+// synchronises the device the first time).
+int fftwpp_amd_plane_supported(const fftwpp_amd_dim *dy, const fftwpp_amd_dim *dz)
+{
+  if(!plane_dim(dy) || !plane_dim(dz) || dy->N != dz->N) return 0;
+  const char *env=getenv("FFTWPP_AMD_PLANE");
+  if(env && env[0] == '0') return 0;
+  static std::mutex mu;
+  static std::vector<std::pair<int,int>> known;   // (device, usable)
+  int dev=0;
+  if(hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> lock(mu);
+  for(const auto &e : known) if(e.first == dev) return e.second;
+  int ok=0;
+  void *work=nullptr;
+  // both precisions must keep their sub-teams resident (checked here so that
+  // a plan never meets the restriction at execution time)
+  int per64=0, per32=0;
+  if(allow_lds(plane256_kernel<double,1>,plane_lds<double>()) ||
+     allow_lds(plane256_kernel<float,1>,plane_lds<float>()) ||
+     hipOccupancyMaxActiveBlocksPerMultiprocessor(&per64,(const void *) plane256_kernel<double,1>,
+                                                  PLANE_THREADS,plane_lds<double>()) != hipSuccess ||
+     hipOccupancyMaxActiveBlocksPerMultiprocessor(&per32,(const void *) plane256_kernel<float,1>,
+                                                  PLANE_THREADS,plane_lds<float>()) != hipSuccess ||
+     per64 < PlaneSub<double>::n || per32 < PlaneSub<float>::n) {
+    (void) hipGetLastError();
+    known.push_back({dev,0});
+    return 0;
+  }
+  if(hipMalloc(&work,plane_ctl_bytes()) == hipSuccess) {
+    int rc=plane_conv_t<double>(dz,FFTWPP_AMD_MULT_BINARY,nullptr,nullptr,nullptr,0,work,0);
+    PlaneCtl c;
+    memset(&c,0,sizeof(c));
+    if(!rc) rc=hipcheck(hipDeviceSynchronize(),"plane probe sync");
+    if(!rc) rc=hipcheck(hipMemcpy(&c,work,sizeof(c),hipMemcpyDeviceToHost),"plane probe copy");
+    if(!rc) {
+      ok=c.abort_ == 0;
+      for(int i=0; i < 8; ++i) if(c.tk[i] != (unsigned) (PlaneSub<double>::n*PLANE_W)) ok=0;
+      if(!ok) {
+        char buf[256];
+        snprintf(buf,sizeof(buf),"plane probe: abort %u, blocks per XCD %u %u %u %u %u %u %u %u",
+                 c.abort_,c.tk[0],c.tk[1],c.tk[2],c.tk[3],c.tk[4],c.tk[5],c.tk[6],c.tk[7]);
+        fail(FFTWPP_AMD_ERR_UNSUPPORTED,buf);
+      }
+    }
+    (void) hipFree(work);
+  }
+  (void) hipGetLastError();
+  known.push_back({dev,ok});
+  return ok;
+}
+
+int fftwpp_amd_plane_conv(const fftwpp_amd_dim *dy, const fftwpp_amd_dim *dz,
+                          int dtype, int mult, const void *Xf, const void *Xg,
+                          void *Xh, size_t planes, size_t Ly, size_t lz,
+                          void *work, void *stream)
+{
+  if(!Xf || !Xg || !Xh || !work) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  if(!plane_dim(dy) || !plane_dim(dz) || dy->N != dz->N)
+    return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"plane kernel: y and z need m = 256, q = 2, L = 256");
+  if(mult != FFTWPP_AMD_MULT_BINARY && mult != FFTWPP_AMD_MULT_CORRELATION &&
+     mult != FFTWPP_AMD_MULT_NONE)
+    return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"plane kernel: multNone, multBinary or multcorrelation");
+  if(Ly != dy->L || lz != dz->L || planes == 0)
+    return fail(FFTWPP_AMD_ERR_ARG,"plane kernel: shape does not match the dimensions");
+  if(planes*Ly*lz > ((size_t) 1 << 40)) return fail(FFTWPP_AMD_ERR_ARG,"plane kernel: too many planes");
+  hipStream_t st=(hipStream_t) stream;
+  return dtype == FFTWPP_AMD_F32 ?
+    plane_conv_t<float>(dz,mult,Xf,Xg,Xh,planes,work,st) :
+    plane_conv_t<double>(dz,mult,Xf,Xg,Xh,planes,work,st);
+}
+
+// Abort flag of the last plane launch on `work` (0: completed; 1: a team
+// wait timed out; 2: an XCD held no block).  Synchronises the stream.
+int fftwpp_amd_plane_status(void *work, void *stream)
+{
+  if(!work) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  PlaneCtl c;
+  int rc=hipcheck(hipStreamSynchronize((hipStream_t) stream),"plane status sync");
+  if(!rc) rc=hipcheck(hipMemcpy(&c,work,sizeof(c),hipMemcpyDeviceToHost),"plane status copy");
+  return rc ? rc : (int) c.abort_;
 }
 
 int fftwpp_amd_dim_create(fftwpp_amd_dim **out, size_t L, size_t m,
@@ -1566,7 +1722,8 @@ int fftwpp_amd_batch_rows(const fftwpp_amd_dim *d, int dtype, int mult,
                           size_t ptotal, double scale, void *stream)
 {
   if(!d || !items || !f || !g || !out) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
-  if(D == 0) return fail(FFTWPP_AMD_ERR_ARG,"residue group size must be positive");
+  if(D == 0 && !batch_tps(d->m,dtype == FFTWPP_AMD_F32 ? sizeof(cplx<float>) : sizeof(cplx<double>)))
+    return fail(FFTWPP_AMD_ERR_ARG,"the sequence mode (D = 0) needs a register-FFT sub-FFT size");
   if(D < d->q && !part) return fail(FFTWPP_AMD_ERR_ARG,"residue groups need a partial-sum buffer");
   if(count == 0) return FFTWPP_AMD_OK;
   if(d->kind != FFTWPP_AMD_KIND_COMPLEX)
@@ -1582,6 +1739,7 @@ int fftwpp_amd_batch_rows(const fftwpp_amd_dim *d, int dtype, int mult,
   b.items=(const BatchItem *) items; b.count=count; b.f=f; b.g=g; b.out=out;
   b.part=part; b.TR=TR; b.threads=threads; b.nblocks=nblocks; b.lacc=lacc;
   b.D=D < d->q ? D : d->q; b.ptotal=ptotal; b.ld=0; b.lds=0; b.scale=scale;
+  if(D == 0 && d->q == 1) b.D=1;   // one residue: the sequence is the output
   return dtype == FFTWPP_AMD_F32 ? batch_rows_t<float>(d,mult,b,st) :
     batch_rows_t<double>(d,mult,b,st);
 }

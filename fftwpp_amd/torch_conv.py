@@ -110,6 +110,20 @@ class DeviceConvolution:
         out = a.clone()
         return self.convolve_(out, b)
 
+    def set_memory_limit(self, nbytes):
+        """Keep the plan's device work buffers within nbytes (3D complex
+        plans; 0 = no limit): a limit below the separate y, z and y stages'
+        need selects the plane kernel for the y-z stage."""
+        check(lib().fftwpp_amd_conv_set_memory_limit(self._h, int(nbytes)))
+
+    def work_bytes(self):
+        """(bytes, plane): device work bytes of the next execution and
+        whether it takes the plane kernel (3D complex plans)."""
+        n = ctypes.c_size_t()
+        p = ctypes.c_int()
+        check(lib().fftwpp_amd_conv_work_bytes(self._h, ctypes.byref(n), ctypes.byref(p)))
+        return n.value, bool(p.value)
+
     def _plane(self):
         """Self-conjugate plane (Hx, Hy, x0, y0, Sx, Sy) of a Hermitian plan's
         stored array (the arguments of fftwpp_amd_hermitian_plane)."""

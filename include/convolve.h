@@ -491,9 +491,20 @@ public:
 
   Timeline *timeline=NULL;  // when set, every launch is timed
 
+  // Device bytes the plan's work buffers may hold (0: no limit).  The
+  // separate y, z and y launches need 2 + 2 q_y x-transformed arrays; the
+  // plane kernel (y and z with m = 256, q = 2, L = 256) needs two plus
+  // fftwpp_amd_plane_work(), and is taken when the limit excludes the first.
+  size_t memoryLimit=0;
+  // Work bytes of a complex execution in dtype, plane engine or not.
+  size_t workBytes(int dtype, bool plane) const;
+  // Whether an execution in dtype takes the plane kernel.
+  bool usesPlanes(int dtype) const;
+
 protected:
   int multCode;
   size_t lz;  // stored length of a z row (Lz or ceil(Lz/2))
+  bool planes=false;  // the y-z stage can run as one plane-kernel launch
   DeviceWork work;
   void run(Complex **f, size_t offset, bool normalize);
   // The complex device convolution (execDevice on complex arrays).

@@ -104,6 +104,31 @@ int fftwpp_amd_conv_rows(const fftwpp_amd_dim *d, int dtype, int mult,
                          size_t rowstride, void *out, size_t outstride,
                          double scale, void *stream);
 
+/* The y-z sub-convolution of every x-plane of a 3D convolution in one
+ * persistent cooperative launch: planes of Ly x lz complex words (y rows of
+ * lz contiguous words, plane p at p*Ly*lz) of Xf and Xg are convolved over
+ * their y (dy) and z (dz) dimensions -- forward residues, `mult`, backward
+ * residues summed -- and written to Xh (which may equal Xf), unscaled.  This
+ * is the x-plane loop of Convolution3::convolveRaw (convolve.h:1812-1879)
+ * with its inner Convolution2, for dy and dz complex with m = 256, q = 2,
+ * L = 256 and mult multNone, multBinary or multcorrelation.  The padded
+ * y-transformed planes stay in the L2 cache of the XCD whose blocks share
+ * them; `work` holds fftwpp_amd_plane_work(dtype) bytes (exchange areas and
+ * a control block) and must not be shared by launches that overlap; the
+ * launch is not capturable into a graph (its control block is cleared by a
+ * memset).  fftwpp_amd_plane_supported() says whether the geometry and the
+ * device take the kernel (8 XCDs of 32 CUs; the first call on a device
+ * synchronises it); fftwpp_amd_plane_status() synchronises `stream` and
+ * returns the launch's abort code (0 = completed, 1 = a team wait timed
+ * out, 2 = an XCD held another number of blocks than the kernel needs). */
+size_t fftwpp_amd_plane_work(int dtype);
+int fftwpp_amd_plane_supported(const fftwpp_amd_dim *dy, const fftwpp_amd_dim *dz);
+int fftwpp_amd_plane_conv(const fftwpp_amd_dim *dy, const fftwpp_amd_dim *dz,
+                          int dtype, int mult, const void *Xf, const void *Xg,
+                          void *Xh, size_t planes, size_t Ly, size_t lz,
+                          void *work, void *stream);
+int fftwpp_amd_plane_status(void *work, void *stream);
+
 /* Hermitian rows longer than one block of m (H = ceil(L/2) > m): the
  * fused row convolution of fftwpp_amd_conv_rows with realMultBinary, for
  * power-of-two m in [16, 4096].  Each residue pair (fftPadHermitian's D = 2
@@ -127,7 +152,10 @@ int fftwpp_amd_hconv_rows_long(const fftwpp_amd_dim *d, int dtype, const void *f
  * LDS rows of lacc >= max L words, lacc = 0 in the destination.  D residues
  * per work unit: D >= q writes out directly; D < q writes G = ceil(q/D)
  * partial sums into part (G slices of ptotal words, item i at poff) and adds
- * them into out in a second launch.  Each item follows the residue loop of
+ * them into out in a second launch; D = 0 (register path only) makes each
+ * residue a work unit that writes its inverse-transformed sequence (q m
+ * words per item in part, item i at i q m), and the second launch folds the
+ * sequences back onto the item's blocks.  Each item follows the residue loop of
  * Convolution::convolveRaw (convolve.cc:7513-7575); the batching itself has
  * no reference counterpart.  fftwpp_amd_batch_rows_lds gives the dynamic
  * LDS bytes of a tile, 0 when it does not fit. */
@@ -229,6 +257,19 @@ int fftwpp_amd_conv_create(fftwpp_amd_conv **h, int dim, int hermitian,
                            const size_t *L, const size_t *M,
                            const size_t *m, int dtype);
 int fftwpp_amd_conv_destroy(fftwpp_amd_conv *h);
+
+/* Memory-bounded execution of 3D complex plans (no reference counterpart:
+ * the reference bounds its buffers by processing D residues per pass,
+ * convolve.h:445-451).  The plan's device work buffers stay within `bytes`
+ * (0 = no limit): when the separate y, z and y stages would need more, the
+ * y-z stage runs as the plane kernel (two x-transformed arrays plus
+ * fftwpp_amd_plane_work() bytes instead of 2 + 2 q_y arrays); an execution
+ * that fits neither fails with FFTWPP_AMD_ERR_UNSUPPORTED.  Results are the
+ * same either way.  fftwpp_amd_conv_work_bytes reports the bytes the next
+ * execution in the plan's precision holds and whether it takes the plane
+ * kernel. */
+int fftwpp_amd_conv_set_memory_limit(fftwpp_amd_conv *h, size_t bytes);
+int fftwpp_amd_conv_work_bytes(const fftwpp_amd_conv *h, size_t *bytes, int *plane);
 
 /* Device pointers: a <- a (*) b on `stream`. */
 int fftwpp_amd_conv_execute(fftwpp_amd_conv *h, void *a, const void *b,

@@ -185,3 +185,99 @@ def test_batch_workload_every_item(gpu, oracle, dtype_name, tol):
         torch.cuda.synchronize()
     worst = _check_items(oracle, plan, out, fs, gs, tol)
     print(f"batched workload {dtype_name}: worst item rel L2 {worst:.3e}", flush=True)
+
+
+@pytest.mark.parametrize("dtype_name,m,lengths,tol", [
+    ("complex128", 4096, [12289, 13000, 16000, 16384, 14001], 1e-12),
+    ("complex64", 4096, [12289, 13000, 16000, 16384, 14001], 1e-5),
+    ("complex128", 256, [129, 200, 256, 131, 255, 190, 170, 140, 250], 1e-12),
+    ("complex128", 2048, [4097, 5000, 6144], 1e-12),
+])
+def test_batch_rows_sequence_mode(gpu, oracle, dtype_name, m, lengths, tol):
+    """The sequence mode of the batched kernel (D = 0: one residue per work
+    unit writing its inverse-transformed sequence, then the backward fold over
+    the blocks in a second launch) called through the kernel-layer ABI for
+    items that share (m, q), checked item by item against explicit padding."""
+    import ctypes
+    import torch
+    from fftwpp_amd import _lib
+    lib = _lib.lib()
+    dtype = getattr(torch, dtype_name)
+    dcode = 0 if dtype == torch.complex128 else 1
+
+    def res(L, M):
+        p = -(-L // m)
+        P = 1 if p == 2 else p
+        n = -(-M // (P * m))
+        return P * n
+    qs = {res(L, 2 * L) for L in lengths}
+    assert len(qs) == 1
+    q = qs.pop()
+    fs, gs = _inputs(lengths, 31 + m)
+    offs = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+    items = np.zeros((len(lengths), 3), dtype=np.int64)
+    items[:, 0] = offs[:-1]
+    items[:, 1] = lengths
+    items[:, 2] = offs[:-1]
+    d = ctypes.c_void_p()
+    _lib.check(lib.fftwpp_amd_dim_create(ctypes.byref(d), max(lengths), m, q, 0))
+    try:
+        tps = lib.fftwpp_amd_batch_rows_tps(d, dcode)
+        assert tps > 0
+        f = torch.tensor(np.concatenate(fs), dtype=dtype, device=gpu)
+        g = torch.tensor(np.concatenate(gs), dtype=dtype, device=gpu)
+        out = torch.zeros_like(f)
+        it = torch.tensor(items.ravel(), device=gpu)
+        part = torch.empty(len(lengths) * q * m, dtype=dtype, device=gpu)
+        threads = max(tps, 256) if tps <= 64 else tps
+        units = len(lengths) * q
+        nblocks = -(-units // 8) * 8 if tps > 64 else -(-units // (threads // tps))
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _lib.check(lib.fftwpp_amd_batch_rows(d, dcode, 1, it.data_ptr(), len(lengths),
+                                             f.data_ptr(), g.data_ptr(), out.data_ptr(),
+                                             1, threads, nblocks, 0, 0, part.data_ptr(),
+                                             int(offs[-1]), 1.0 / (m * q), stream))
+        torch.cuda.synchronize()
+        h = out.cpu().numpy()
+        for i, L in enumerate(lengths):
+            e = _expected(oracle, fs[i], gs[i])
+            err = oracle.rel_l2(h[offs[i]:offs[i + 1]].astype(np.complex128), e)
+            assert err < tol, f"item {i} (L={L}, m={m}, q={q}): rel L2 {err:.3e}"
+    finally:
+        lib.fftwpp_amd_dim_destroy(d)
+
+
+def test_batch_tune_cache(gpu, oracle, tmp_path, monkeypatch):
+    """The autotuner's choices persist (the analogue of FFTW wisdom): the
+    second tuned plan of the same batch is built from the stored entry
+    without timing (well under 50 ms), has the same classes and tiles, and
+    still matches the oracle; another batch appends its own entry."""
+    import time
+    import torch
+    from fftwpp_amd.batch import BatchedConvolution
+    path = tmp_path / "tune.json"
+    monkeypatch.setenv("FFTWPP_AMD_TUNE_CACHE", str(path))
+    lengths = [int(v) for v in oracle.batched_lengths(256, 64, 16384, seed=9)]
+    M = [2 * L for L in lengths]
+    t0 = time.perf_counter()
+    p1 = BatchedConvolution(lengths, M=M, dtype=torch.complex128, tune=True)
+    t1 = time.perf_counter()
+    assert path.exists() and len(path.read_text().splitlines()) == 1
+    p2 = BatchedConvolution(lengths, M=M, dtype=torch.complex128, tune=True)
+    t2 = time.perf_counter()
+    print(f"tuned creation {t1 - t0:.3f} s, from the cache {t2 - t1:.3f} s", flush=True)
+    assert t2 - t1 < 0.05
+
+    def tiles(plan):
+        return [line.split(" ms=")[0] for line in plan.describe().splitlines()]
+    assert tiles(p2) == tiles(p1)
+    assert len(path.read_text().splitlines()) == 1
+    fs, gs = _inputs(lengths, 19)
+    out = p2(p2.pack(fs), p2.pack(gs))
+    torch.cuda.synchronize()
+    got = p2.unpack(out)
+    for i in range(0, len(lengths), 16):
+        assert oracle.rel_l2(got[i], _expected(oracle, fs[i], gs[i])) < 1e-12
+    other = lengths[:100]
+    BatchedConvolution(other, M=M[:100], dtype=torch.complex128, tune=True)
+    assert len(path.read_text().splitlines()) == 2
