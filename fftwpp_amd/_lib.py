@@ -34,6 +34,7 @@ SIGNATURES = {
     "fftwpp_amd_batch_rows_lds": (Z, [P, I, Z, Z]),
     "fftwpp_amd_batch_rows_tps": (Z, [P, I]),
     "fftwpp_amd_batch_rows": (I, [P, I, I, P, Z, P, P, P, Z, Z, Z, Z, Z, P, Z, D, P]),
+    "fftwpp_amd_batch_rows_noalloc": (I, [P, I, I, P, Z, P, P, P, Z, Z, Z, Z, Z, Z, D, P]),
     "fftwpp_amd_real_balance_bytes": (Z, []),
     "fftwpp_amd_real_balance": (I, [I, P, P, Z, P, P]),
     "fftwpp_amd_real_pack": (I, [I, P, P, P, Z, P, P]),

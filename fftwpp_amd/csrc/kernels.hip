@@ -1744,6 +1744,55 @@ int fftwpp_amd_batch_rows(const fftwpp_amd_dim *d, int dtype, int mult,
     batch_rows_t<double>(d,mult,b,st);
 }
 
+// Scratch of fftwpp_amd_batch_rows_noalloc: a file-scope device array, one
+// copy per GPU.  Host code never passes the array's name to a kernel (on the
+// host it only stands for the symbol); the device address comes from
+// hipGetSymbolAddress, looked up once per device on the entry point's first
+// call there.
+static __device__ __align__(256) unsigned char scratch[FFTWPP_AMD_NOALLOC_SCRATCH_BYTES];
+
+static int noalloc_scratch(void **ptr)
+{
+  static std::mutex mu;
+  static std::vector<std::pair<int,void *>> known;   // (device, address)
+  int dev=0;
+  int rc=hipcheck(hipGetDevice(&dev),"hipGetDevice");
+  if(rc) return rc;
+  std::lock_guard<std::mutex> lock(mu);
+  for(const auto &e : known)
+    if(e.first == dev) { *ptr=e.second; return FFTWPP_AMD_OK; }
+  void *p=nullptr;
+  rc=hipcheck(hipGetSymbolAddress(&p,HIP_SYMBOL(scratch)),"hipGetSymbolAddress(scratch)");
+  if(rc) return rc;
+  known.push_back({dev,p});
+  *ptr=p;
+  return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_batch_rows_noalloc(const fftwpp_amd_dim *d, int dtype, int mult,
+                                  const void *items, size_t count, const void *f,
+                                  const void *g, void *out, size_t TR,
+                                  size_t threads, size_t nblocks, size_t lacc,
+                                  size_t D, size_t ptotal, double scale,
+                                  void *stream)
+{
+  if(!d) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  const size_t elt=dtype == FFTWPP_AMD_F32 ? sizeof(cplx<float>) : sizeof(cplx<double>);
+  size_t words=0;
+  if(D == 0) words=count*d->q*d->m;
+  else if(D < d->q) words=((d->q+D-1)/D)*ptotal;
+  if(words > FFTWPP_AMD_NOALLOC_SCRATCH_BYTES/elt)
+    return fail(FFTWPP_AMD_ERR_ARG,"fftwpp_amd_batch_rows_noalloc: partial results exceed the "
+                "16 MiB scratch array (use fftwpp_amd_batch_rows with a caller buffer)");
+  void *part=nullptr;
+  if(words) {
+    int rc=noalloc_scratch(&part);
+    if(rc) return rc;
+  }
+  return fftwpp_amd_batch_rows(d,dtype,mult,items,count,f,g,out,TR,threads,nblocks,lacc,D,
+                               part,ptotal,scale,stream);
+}
+
 size_t fftwpp_amd_real_balance_bytes(void)
 {
   return (2*REAL_NORM_BLOCKS+1)*sizeof(double);   // partials, then e and the zero flag

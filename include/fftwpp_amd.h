@@ -171,6 +171,25 @@ int fftwpp_amd_batch_rows(const fftwpp_amd_dim *d, int dtype, int mult,
                           const void *g, void *out, size_t TR, size_t threads,
                           size_t nblocks, size_t lacc, size_t D, void *part,
                           size_t ptotal, double scale, void *stream);
+/* fftwpp_amd_batch_rows with its partial sums (or sequences) in a fixed
+ * device array of the library instead of a caller's `part` buffer.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc); it calls no allocating routine.  The array holds
+ * FFTWPP_AMD_NOALLOC_SCRATCH_BYTES (16 MiB) per GPU: launches needing more
+ * (G ptotal words for D < q, count q m words for D = 0, times 16 bytes in
+ * FP64 or 8 in FP32) are refused with FFTWPP_AMD_ERR_ARG before anything is
+ * enqueued.  Every call on a GPU shares its array, so calls that overlap on
+ * one GPU (two streams, two threads) race; serialise them.  Nothing in the
+ * array needs clearing between calls (every word read was written by the
+ * same launch). */
+#define FFTWPP_AMD_NOALLOC_SCRATCH_BYTES ((size_t) 16 << 20)
+int fftwpp_amd_batch_rows_noalloc(const fftwpp_amd_dim *d, int dtype, int mult,
+                                  const void *items, size_t count, const void *f,
+                                  const void *g, void *out, size_t TR,
+                                  size_t threads, size_t nblocks, size_t lacc,
+                                  size_t D, size_t ptotal, double scale,
+                                  void *stream);
 
 /* Real convolutions (fftPadReal, convolve.h:805-980; tests/hybridconvr*.cc):
  * z[i] = f0[i] + i 2^e f1[i] for n real words of f0 and f1, and out[i] =
