@@ -8,7 +8,12 @@ The same workload is then timed in FP32 (FP64-generated twiddles) and its
 relative L2 error against the FP64 result is reported.
 
 Prints ONE JSON line.  Per-kernel durations come from HIP events recorded by
-the library after each of its launches on the benchmark stream.
+the library after each of its launches on the benchmark stream.  The batched
+workload runs in a child process of its own before this process touches the
+GPU: after the 3D legs' multi-GB buffers have been allocated and freed, the
+batch measured in the same process runs 10-25% slower than in a fresh one
+(not a thermal effect: a 20 s pause does not remove it), and a user of the
+batch API does not carry that history.
 """
 import argparse
 import ctypes
@@ -25,6 +30,11 @@ if ROOT not in sys.path:
 METRIC = "3D conv/s at 256³→512³ FP32 vs FP64 (rel. L2 err) + HBM GB/s % of peak"
 PEAK_GBS = 8000.0  # MI355X HBM3E
 SEED = 20261202
+
+
+def progress(msg):
+    """One line per finished leg on stderr (the JSON result stays alone on stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
@@ -44,6 +54,7 @@ def parse():
                          "stays null when rocprofv3 is missing or a pass fails)")
     ap.add_argument("--no-traffic", dest="traffic", action="store_false",
                     help="skip the counter passes (roofline.traffic null)")
+    ap.add_argument("--batched-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--traffic-child", default=None, choices=["f64", "f32"],
                     help=argparse.SUPPRESS)
     ap.add_argument("--calibrate", action="store_true",
@@ -371,6 +382,7 @@ def bench_batched(torch, dev, steps, warmup, per_call=True):
             r["rel_l2_vs_per_call"] = float(torch.linalg.norm(outs[key] - ref) / torch.linalg.norm(ref))
             del plans
         res[key] = r
+        progress(f"batched {key}: {ms:.4f} ms per batch")
         del f, g, out
         torch.cuda.empty_cache()
     if saved is None:
@@ -550,6 +562,11 @@ def main():
     if args.traffic_child:
         traffic_child(args.traffic_child, args.size)
         return
+    if args.batched_child:
+        import torch
+        print(json.dumps(bench_batched(torch, torch.device("cuda:0"), args.steps, args.warmup)),
+              flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "one GPU only: --gpus N>1 is not supported (the batched "
                                    "items would shard across GPUs with no exchange, the 3D and "
@@ -566,8 +583,22 @@ def main():
         # child processes first: this process has not touched the GPU yet
         try:
             traffic = measure_traffic(args.size)
+            progress("traffic counters done")
         except Exception as e:  # no rocprofv3, no records: the field stays null
             print(f"traffic measurement skipped: {e}", file=sys.stderr, flush=True)
+    batched = None
+    if not args.no_extra:
+        # the batched workload in a fresh process (see the module docstring)
+        import subprocess
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--batched-child",
+                            "--steps", str(args.steps), "--warmup", str(args.warmup)],
+                           stdout=subprocess.PIPE, text=True)
+        lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+        if r.returncode != 0 or not lines:
+            print(f"batched child failed with status {r.returncode}", file=sys.stderr, flush=True)
+            sys.exit(1)
+        batched = json.loads(lines[-1])
+        progress("batched workload done (child process)")
     import numpy as np
     import torch
     from fftwpp_amd.torch_conv import DeviceConvolution
@@ -594,6 +625,7 @@ def main():
     ms64, st64 = time_plan(torch, plan, out, a, b, args.steps, args.warmup, stream)
     H64 = out.cpu().numpy()
     err64 = sampled_direct_error(f, g, H64)
+    progress(f"3D FP64 done: {ms64:.4f} ms")
 
     result = {
         "metric": METRIC,
@@ -650,21 +682,28 @@ def main():
             "stages_ms": {k: round(v, 4) for k, v in st32.items()},
         }
         result["rel_l2"]["fp32_vs_fp64"] = err32
+        progress(f"3D FP32 done: {ms32:.4f} ms")
         del a, b, out, plan
         torch.cuda.empty_cache()
 
     if not args.no_extra:
         result["plane_engine"] = bench_plane_engine(torch, f, g, L, H64, min(args.steps, 10),
                                                     2, stream)
-        result["batched"] = bench_batched(torch, dev, args.steps, args.warmup)
+        progress("plane engine done")
+        result["batched"] = batched
         result["conv1d"] = bench_conv1d(torch, dev, args.steps)
+        progress("1D done")
         result["hconv2d"] = bench_hconv2d(torch, dev, args.steps, args.warmup)
+        progress("2D Hermitian done")
         result["real3d"] = bench_real3d(torch, dev, args.steps, args.warmup)
+        progress("3D real done")
         result["user_multiplier"] = bench_user_multiplier(torch, dev, min(args.steps, 5), 1)
+        progress("user multiplier done")
         result["fft"] = bench_fft(torch, dev, args.steps, args.warmup)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(f, g, L)
+        progress("CPU baseline (port) done")
         lib_base, href = cpu_library_baseline(f, g, L)
         result["cpu_baseline_library"] = lib_base
         result["rel_l2"]["fp64_vs_explicit_padding"] = float(

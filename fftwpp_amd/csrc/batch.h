@@ -150,7 +150,7 @@ template<typename T> struct BatchV { static constexpr int V=sizeof(T) == 8 ? 8 :
 // the backward fold over the blocks b of the reference's backward transforms
 // (convolve.cc:1468-1613, 1765-1965), so no output word is read and written
 // once per residue.
-template<typename T, int M>
+template<typename T, int M, bool ZM=false>
 __global__ void __launch_bounds__(512)
 batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
                  const cplx<T> *f, const cplx<T> *g, cplx<T> *out, T scale,
@@ -167,7 +167,7 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
   const int slots=blockDim.x/TPS;
   const int slot=threadIdx.x/TPS, t=threadIdx.x-slot*TPS;
   T *xb=reinterpret_cast<T *>(tw+M)+slot*2*P::PAD;
-  const bool zm=D == 0;
+  constexpr bool zm=ZM;                  // the sequence mode (the host passes D = 0)
   if(zm) D=1;
   const int64_t G=(d.q+D-1)/D;
   const int64_t units=count*G;
@@ -235,7 +235,7 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
 #pragma unroll
         for(int k=0; k < V; ++k) X[k]=mulc(X[k],zN[(t+TPS*k)*r]);
       }
-      if(zm) {
+      if constexpr(zm) {
         cplx<T> *z=part+(ii*qi+r)*(int64_t) M;
 #pragma unroll
         for(int k=0; k < V; ++k) z[t+TPS*k]=X[k];

@@ -120,6 +120,11 @@ struct fftwpp_amd_batch {
   hipEvent_t fork=nullptr, join[NSIDE]={};
   std::vector<Graph> graphs;
   int cus=256;
+  // stream schedule of the classes: 0 = dealt round robin over the side
+  // streams; 1 = all on one stream in order; 2 = the classes with a
+  // persistent grid (fewer blocks than work units) in order on one stream,
+  // the others dealt over the remaining side streams
+  int sched=0;
 };
 
 namespace {
@@ -275,20 +280,43 @@ int alloc_part(const fftwpp_amd_batch &h, Cls &c)
 }
 
 // Average time of one launch (ms), or a negative value on failure.
+// A buffer larger than the Infinity Cache, overwritten before every timed
+// launch so that launches are timed with cold caches, as they run in the
+// batch (set while a plan is tuned).
+thread_local void *scrub_buf=nullptr;
+// Candidate set of tune_class: false = residue groups only (no sequence mode,
+// no persistent grids).
+thread_local bool tune_wide=true;
+thread_local size_t scrub_bytes=0;
+const size_t SCRUB_BYTES=(size_t) 320 << 20;
+
+// Per-class timings with (true) or without (false) the cache scrub.
+thread_local bool tune_cold=true;
+
+int scrub(hipStream_t s, int k)
+{
+  if(!scrub_buf || !tune_cold) return FFTWPP_AMD_OK;
+  return bhip(hipMemsetAsync(scrub_buf,k & 0xff,scrub_bytes,s),"scrub caches");
+}
+
 double time_launch(const fftwpp_amd_batch &h, const Cls &c, const Tile &t,
                    void *part, void *out, const void *f, const void *g,
                    hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
   if(launch_class(h,c,t,part,out,f,g,s)) return -1;
   const int reps=3;
-  if(hipEventRecord(e0,s) != hipSuccess) return -1;
-  for(int k=0; k < reps; ++k)
+  double total=0;
+  for(int k=0; k < reps; ++k) {
+    if(scrub(s,k)) return -1;
+    if(hipEventRecord(e0,s) != hipSuccess) return -1;
     if(launch_class(h,c,t,part,out,f,g,s)) return -1;
-  if(hipEventRecord(e1,s) != hipSuccess) return -1;
-  if(hipEventSynchronize(e1) != hipSuccess) return -1;
-  float ms=0;
-  if(hipEventElapsedTime(&ms,e0,e1) != hipSuccess) return -1;
-  return ms/reps;
+    if(hipEventRecord(e1,s) != hipSuccess) return -1;
+    if(hipEventSynchronize(e1) != hipSuccess) return -1;
+    float ms=0;
+    if(hipEventElapsedTime(&ms,e0,e1) != hipSuccess) return -1;
+    total += ms;
+  }
+  return total/reps;
 }
 
 // Best tile of a class by timing the candidates (items per block, threads,
@@ -301,7 +329,7 @@ double tune_class(const fftwpp_amd_batch &h, Cls &c, void *part, void *out,
   double best=-1;
   Tile bt=default_tile(h,c);
   const bool reg=fftwpp_amd_batch_rows_tps(c.dim,h.dtype) != 0;
-  for(size_t D : group_sizes(c.q,reg)) {
+  for(size_t D : group_sizes(c.q,reg && tune_wide)) {
     // the timing arrays hold 8 slices of the batch for partial results
     if(D == 0 && c.ids.size()*c.q*c.m > 8*h.total) continue;
     for(size_t TR=1; TR <= 64; TR *= 2) {
@@ -312,8 +340,19 @@ double tune_class(const fftwpp_amd_batch &h, Cls &c, void *part, void *out,
         if(threads > 64 && threads > TR*c.m/2) break;
         Tile t=make_tile(h,c,TR,threads,D);
         double ms=time_launch(h,c,t,part,out,f,g,s,e0,e1);
-        if(ms < 0) continue;
-        if(best < 0 || ms < best) { best=ms; bt=t; }
+        if(ms >= 0 && (best < 0 || ms < best)) { best=ms; bt=t; }
+        // persistent grids of 1, 2 or 4 blocks per CU for one-sequence blocks
+        // with several units per item: fewer items in flight per XCD keep an
+        // item's inputs in its L2 for the re-reads of its other residues
+        const size_t tps=fftwpp_amd_batch_rows_tps(c.dim,h.dtype);
+        if(!tune_wide || !reg || tps <= 64 || groups(c,t) < 2) continue;
+        for(size_t per=1; per <= 4; per *= 2) {
+          Tile tc=t;
+          tc.nblocks=per*(size_t) h.cus;
+          if(tc.nblocks >= t.nblocks) break;
+          const double mc=time_launch(h,c,tc,part,out,f,g,s,e0,e1);
+          if(mc >= 0 && (best < 0 || mc < best)) { best=mc; bt=tc; }
+        }
       }
     }
   }
@@ -330,13 +369,52 @@ int run_classes(fftwpp_amd_batch &h, void *out, const void *f, const void *g,
   if(rc) return rc;
   for(int k=0; k < NSIDE; ++k)
     if((rc=bhip(hipStreamWaitEvent(h.side[k],h.fork,0),"wait fork"))) return rc;
-  for(size_t i=0; i < h.cls.size(); ++i)
-    if((rc=launch_class(h,h.cls[i],h.cls[i].tile,h.cls[i].part,out,f,g,
-                        h.side[i % NSIDE]))) return rc;
+  size_t other=0;
+  for(size_t i=0; i < h.cls.size(); ++i) {
+    const Cls &c=h.cls[i];
+    int k;
+    if(h.sched == 1) k=0;
+    else if(h.sched == 2) {
+      const size_t tps=fftwpp_amd_batch_rows_tps(c.dim,h.dtype);
+      const size_t units=ceilquot(c.ids.size(),c.tile.TR)*groups(c,c.tile);
+      const bool persistent=tps > 64 && c.tile.nblocks < units;
+      k=persistent ? 0 : 1+(int) (other++ % (NSIDE-1));
+    } else k=(int) (i % NSIDE);
+    if((rc=launch_class(h,c,c.tile,c.part,out,f,g,h.side[k]))) return rc;
+  }
   for(int k=0; k < NSIDE; ++k) {
     if((rc=bhip(hipEventRecord(h.join[k],h.side[k]),"record join"))) return rc;
     if((rc=bhip(hipStreamWaitEvent(s,h.join[k],0),"wait join"))) return rc;
   }
+  return FFTWPP_AMD_OK;
+}
+
+// Stream schedule with the least time for the whole batch (its classes'
+// tiles already chosen), timed on the tuning arrays.
+int tune_schedule(fftwpp_amd_batch &h, void *out, const void *f, const void *g,
+                  hipEvent_t e0, hipEvent_t e1, double *tbest)
+{
+  double best=-1;
+  int bs=0;
+  for(int sc=0; sc < 3; ++sc) {
+    h.sched=sc;
+    int rc=run_classes(h,out,f,g,h.cap);
+    double total=0;
+    for(int k=0; k < 3 && !rc; ++k) {
+      rc=scrub(h.cap,k);
+      if(!rc) rc=bhip(hipEventRecord(e0,h.cap),"record");
+      if(!rc) rc=run_classes(h,out,f,g,h.cap);
+      if(!rc) rc=bhip(hipEventRecord(e1,h.cap),"record");
+      if(!rc) rc=bhip(hipEventSynchronize(e1),"sync");
+      float ms=0;
+      if(!rc) rc=bhip(hipEventElapsedTime(&ms,e0,e1),"elapsed");
+      total += ms;
+    }
+    if(rc) return rc;
+    if(best < 0 || total < best) { best=total; bs=sc; }
+  }
+  h.sched=bs;
+  *tbest=best;
   return FFTWPP_AMD_OK;
 }
 
@@ -387,6 +465,9 @@ struct TuneArrays {
     rc=bhip(hipMemsetAsync(buf,0,bytes,h.cap),"clear tuning arrays");
     if(!rc) rc=bhip(hipEventCreate(&e0),"event");
     if(!rc) rc=bhip(hipEventCreate(&e1),"event");
+    // the cache scrub buffer is optional: without it launches are timed warm
+    if(!rc && hipMalloc(&scrub_buf,SCRUB_BYTES) == hipSuccess) scrub_bytes=SCRUB_BYTES;
+    else { scrub_buf=nullptr; (void) hipGetLastError(); }
     return rc;
   }
   double time(const fftwpp_amd_batch &h, Cls &c) {
@@ -397,6 +478,9 @@ struct TuneArrays {
     if(e1) (void) hipEventDestroy(e1);
     (void) hipStreamSynchronize(h.cap);
     if(buf) (void) hipFree(buf);
+    if(scrub_buf) (void) hipFree(scrub_buf);
+    scrub_buf=nullptr;
+    scrub_bytes=0;
     buf=nullptr; e0=e1=nullptr;
   }
 };
@@ -447,7 +531,8 @@ int tune_geometry(fftwpp_amd_batch &h, TuneArrays &ta)
 // result.  The file is FFTWPP_AMD_TUNE_CACHE when set (empty: no cache),
 // else $XDG_CACHE_HOME/fftwpp_amd/batch_tune.json or
 // ~/.cache/fftwpp_amd/batch_tune.json.  One entry per line:
-//   {"key":"<16 hex>","buckets":[[b,m],...],"tiles":[[m,q,TR,threads,D,nblocks,lacc],...]}
+//   {"key":"<16 hex>","buckets":[[b,m],...],"tiles":[[m,q,TR,threads,D,nblocks,lacc],...],
+//    "sched":s}
 // ------------------------------------------------------------------------
 std::string cache_path()
 {
@@ -487,6 +572,7 @@ std::string tune_key(const fftwpp_amd_batch &h)
 struct TuneEntry {
   std::map<size_t,size_t> buckets;        // ceilpow2(L) -> m
   std::vector<std::vector<size_t>> tiles; // m, q, TR, threads, D, nblocks, lacc
+  int sched=0;                            // stream schedule
 };
 
 std::mutex cache_mu;
@@ -529,6 +615,8 @@ bool cache_lookup(const std::string &key, TuneEntry &e)
     e=TuneEntry();
     for(auto &v : b) if(v.size() == 2) e.buckets[v[0]]=v[1];
     for(auto &v : t) if(v.size() == 7) e.tiles.push_back(v);
+    const size_t at=line.find("\"sched\":");
+    if(at != std::string::npos) e.sched=atoi(line.c_str()+at+8);
     return true;
   }
   return false;
@@ -555,7 +643,7 @@ void cache_store(const std::string &key, const fftwpp_amd_batch &h)
        << "," << t.D << "," << t.nblocks << "," << t.lacc << "]";
     first=false;
   }
-  os << "]}\n";
+  os << "],\"sched\":" << h.sched << "}\n";
   std::lock_guard<std::mutex> lock(cache_mu);
   const size_t slash=path.rfind('/');
   if(slash != std::string::npos) {
@@ -673,6 +761,48 @@ int fftwpp_amd_batch_create(fftwpp_amd_batch **out, size_t n, const size_t *L,
     }
     if(!rc) rc=alloc_part(*h,c);
   }
+  if(tuned && cached && complete) h->sched=entry.sched;
+  else if(tuned && !rc) {
+    // The classes run concurrently in the batch, which per-class timings do
+    // not see (nor how warm the caches are then): the tiles are chosen four
+    // ways -- the wide candidate set or residue groups only, each timed with
+    // cold or warm caches -- and the whole batch is timed with each choice;
+    // the fastest is kept.
+    if(!timing) { rc=ta.init(*h); timing=true; }
+    double best=-1;
+    std::vector<Tile> bt;
+    std::vector<double> bms;
+    int bsched=0;
+    for(int way=0; way < 4 && !rc; ++way) {
+      tune_wide=way < 2;
+      tune_cold=way % 2 == 0;
+      if(way > 0) {                          // way 0: the tiles tuned above
+        for(Cls &c : h->cls) {
+          if(rc) break;
+          if(ta.time(*h,c) < 0) rc=bfail(FFTWPP_AMD_ERR_HIP,"batch tuning found no working launch");
+          if(!rc) rc=alloc_part(*h,c);
+        }
+      }
+      tune_cold=true;
+      double t=-1;
+      if(!rc) rc=tune_schedule(*h,ta.out,ta.f,ta.g,ta.e0,ta.e1,&t);
+      if(!rc && (best < 0 || t < best)) {
+        best=t;
+        bsched=h->sched;
+        bt.clear();
+        bms.clear();
+        for(const Cls &c : h->cls) { bt.push_back(c.tile); bms.push_back(c.ms); }
+      }
+    }
+    tune_wide=true;
+    tune_cold=true;
+    for(size_t i=0; i < h->cls.size() && !rc; ++i) {
+      h->cls[i].tile=bt[i];
+      h->cls[i].ms=bms[i];
+      rc=alloc_part(*h,h->cls[i]);
+    }
+    h->sched=bsched;
+  }
   if(timing) ta.release(*h);
   if(rc) { destroy(h); return rc; }
   if(tuned && !(cached && complete)) cache_store(key,*h);
@@ -723,6 +853,8 @@ int fftwpp_amd_batch_describe(const fftwpp_amd_batch *h, char *buf, size_t len)
              c.tile.lacc,c.ms);
     s += line;
   }
+  snprintf(line,sizeof(line),"schedule=%d\n",h->sched);
+  s += line;
   size_t k=std::min(len-1,s.size());
   memcpy(buf,s.data(),k);
   buf[k]=0;

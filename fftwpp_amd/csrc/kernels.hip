@@ -1060,19 +1060,27 @@ static void batch_reg_shape(size_t m, size_t elt, size_t threads,
   *lds=m*elt+(n/tps)*2*pad*(elt/2);
 }
 
-template<typename T, int M>
-static int batch_reg_launch(const DimArgs &a, const BatchLaunch &b, int mult,
-                            hipStream_t st)
+template<typename T, int M, bool ZM>
+static int batch_reg_launch_z(const DimArgs &a, const BatchLaunch &b, int mult,
+                              hipStream_t st)
 {
   size_t nt, lds;
   batch_reg_shape(M,sizeof(cplx<T>),b.threads,&nt,&lds);
-  int rc=allow_lds(batch_reg_kernel<T,M>,lds);
+  int rc=allow_lds(batch_reg_kernel<T,M,ZM>,lds);
   if(rc) return rc;
-  hipLaunchKernelGGL((batch_reg_kernel<T,M>),dim3((unsigned) b.nblocks),dim3((unsigned) nt),
+  hipLaunchKernelGGL((batch_reg_kernel<T,M,ZM>),dim3((unsigned) b.nblocks),dim3((unsigned) nt),
                      lds,st,a,b.items,(int64_t) b.count,(const cplx<T> *) b.f,
                      (const cplx<T> *) b.g,(cplx<T> *) b.out,(T) b.scale,mult,(int) b.D,
                      (cplx<T> *) b.part,(int64_t) b.ptotal);
   return launch_check("batch_reg_kernel");
+}
+
+template<typename T, int M>
+static int batch_reg_launch(const DimArgs &a, const BatchLaunch &b, int mult,
+                            hipStream_t st)
+{
+  return b.D == 0 ? batch_reg_launch_z<T,M,true>(a,b,mult,st)
+                  : batch_reg_launch_z<T,M,false>(a,b,mult,st);
 }
 
 template<typename T>
