@@ -55,7 +55,8 @@ def parse():
     ap.add_argument("--no-traffic", dest="traffic", action="store_false",
                     help="skip the counter passes (roofline.traffic null)")
     ap.add_argument("--batched-child", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--traffic-child", default=None, choices=["f64", "f32"],
+    ap.add_argument("--traffic-child", default=None,
+                    choices=["f64", "f32", "f64-plane", "f32-plane"],
                     help=argparse.SUPPRESS)
     ap.add_argument("--calibrate", action="store_true",
                     help="run a 512 MiB device-to-device copy first (the known-size "
@@ -249,12 +250,16 @@ def traffic_child(key, L):
     dst.copy_(src)
     torch.cuda.synchronize()
     del src, dst
-    dt = torch.complex128 if key == "f64" else torch.complex64
+    dt = torch.complex128 if key.startswith("f64") else torch.complex64
     f, g = inputs(L)
     a = torch.tensor(f, device=dev, dtype=dt)
     b = torch.tensor(g, device=dev, dtype=dt)
     out = torch.empty_like(a)
     plan = DeviceConvolution((L, L, L), dtype=dt, M=[2 * L] * 3)
+    if key.endswith("-plane"):
+        # the memory-capped engine: the y-z stage as the plane kernel
+        E = 16 if key.startswith("f64") else 8
+        plan.set_memory_limit(3 * 2 * L ** 3 * E)
     for _ in range(2):
         plan.convolve_out(out, a, b)
     torch.cuda.synchronize()
