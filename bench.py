@@ -479,8 +479,10 @@ def bench_hconv2d(torch, dev, steps, warmup):
 
 
 def bench_real3d(torch, dev, steps, warmup):
-    """3D real convolution 256^3 -> 512^3 (fftPadReal outer dimension), FP64,
-    checked against the complex path on the same real data."""
+    """3D real convolution 256^3 -> 512^3 (fftPadReal outer dimension), FP64
+    and FP32, checked against the complex path on the same real data.  The x
+    dimension runs on the real residue transforms: 257 kept x planes (N/2 + 1
+    of 512) go through the complex y and z stages."""
     import numpy as np
     from fftwpp_amd.torch_conv import DeviceConvolution
     L = 256
@@ -489,19 +491,30 @@ def bench_real3d(torch, dev, steps, warmup):
     g = torch.tensor(rng.uniform(-1, 1, (L, L, L)), device=dev)
     out = torch.empty_like(f)
     plan = DeviceConvolution((L, L, L), kind="real", dtype=torch.float64, M=[2 * L] * 3)
-    ms = _time_calls(torch, lambda: plan.convolve_out(out, f, g), steps, warmup)
+    stream = torch.cuda.current_stream().cuda_stream
+    ms, stages = time_plan(torch, plan, out, f, g, steps, warmup, stream)
     ref = DeviceConvolution((L, L, L), M=[2 * L] * 3)
     fc, gc = f.to(torch.complex128), g.to(torch.complex128)
     hc = torch.empty_like(fc)
     ref.convolve_out(hc, fc, gc)
     err = float(torch.linalg.norm(out - hc.real) / torch.linalg.norm(hc.real))
+    del ref, fc, gc
+    f32, g32 = f.float(), g.float()
+    out32 = torch.empty_like(f32)
+    plan32 = DeviceConvolution((L, L, L), kind="real", dtype=torch.float32, M=[2 * L] * 3)
+    ms32, stages32 = time_plan(torch, plan32, out32, f32, g32, steps, warmup, stream)
+    err32 = float(torch.linalg.norm(out32.double() - hc.real) / torch.linalg.norm(hc.real))
     bytes_alg = CONV_UNITS * 8 * L ** 3
     return {"workload": "3D real 256^3 -> 512^3 (M = 2L), FP64, uniform[-1,1] seed %d" % (SEED + 5),
             "conv_per_s": round(1000.0 / ms, 2), "ms_per_conv": round(ms, 4),
             "achieved_GBs": round(bytes_alg / (ms * 1e-3) / 1e9, 1),
             "frac_of_hbm_peak": round(bytes_alg / (ms * 1e-3) / 1e9 / PEAK_GBS, 4),
             "bytes_alg": bytes_alg, "rel_l2_vs_complex_path": err,
-            "note": "runs as the complex convolution of f + i g (twice the real model's bytes)"}
+            "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+            "fp32": {"conv_per_s": round(1000.0 / ms32, 2), "ms_per_conv": round(ms32, 4),
+                     "rel_l2_vs_complex_fp64": err32,
+                     "stages_ms": {k: round(v, 4) for k, v in stages32.items()}},
+            "note": "x on the real residue transforms (257 of 512 x planes kept)"}
 
 
 def bench_user_multiplier(torch, dev, steps, warmup):

@@ -24,6 +24,9 @@ SIGNATURES = {
     "fftwpp_amd_internal_rows_fit": (I, [P, I]),
     "fftwpp_amd_forward_many": (I, [P, I, P, Z, Z, Z, Z, P, Z, Z, Z, Z, P]),
     "fftwpp_amd_backward_many": (I, [P, I, P, Z, Z, P, Z, Z, Z, Z, Z, Z, I, D, P]),
+    "fftwpp_amd_real_planes": (Z, [P]),
+    "fftwpp_amd_forward_real_many": (I, [P, I, P, Z, Z, Z, Z, P, Z, Z, P]),
+    "fftwpp_amd_backward_real_many": (I, [P, I, P, Z, Z, P, Z, Z, Z, Z, D, P]),
     "fftwpp_amd_conv_rows": (I, [P, I, I, P, P, Z, Z, P, Z, D, P]),
     "fftwpp_amd_plane_work": (Z, [I]),
     "fftwpp_amd_plane_supported": (I, [P, P]),

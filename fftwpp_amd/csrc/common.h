@@ -90,4 +90,58 @@ __device__ __forceinline__ cplx<T> zeta(const DimArgs &d, int64_t e)
   return ztab<T>(d,r);
 }
 
+// Buffer access: a complex word at byte offset off (per lane) + soff
+// (uniform) of a raw buffer; one offset register serves every load of a
+// thread whose words differ by uniform strides.
+// Store of one complex word at byte offset off + soff of the buffer r.
+template<typename T>
+__device__ __forceinline__ void st_buf(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff,
+                                       cplx<T> v);
+template<>
+__device__ __forceinline__ void st_buf<double>(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                               unsigned soff, cplx<double> v)
+{
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  typedef unsigned u4 __attribute__((ext_vector_type(4)));
+  d2 w; w.x=v.x; w.y=v.y;
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4,w),r,off,soff,0);
+}
+template<>
+__device__ __forceinline__ void st_buf<float>(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                              unsigned soff, cplx<float> v)
+{
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  typedef unsigned u2 __attribute__((ext_vector_type(2)));
+  f2 w; w.x=v.x; w.y=v.y;
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2,w),r,off,soff,0);
+}
+// Plain (L1-cached) load of one complex word at byte offset off + soff of r.
+template<typename T>
+__device__ __forceinline__ cplx<T> ld_buf(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff);
+template<>
+__device__ __forceinline__ cplx<double> ld_buf<double>(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                                       unsigned soff)
+{
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const d2 v=__builtin_bit_cast(d2,__builtin_amdgcn_raw_buffer_load_b128(r,off,soff,0));
+  return mk(v.x,v.y);
+}
+template<>
+__device__ __forceinline__ cplx<float> ld_buf<float>(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                                     unsigned soff)
+{
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  const f2 v=__builtin_bit_cast(f2,__builtin_amdgcn_raw_buffer_load_b64(r,off,soff,0));
+  return mk(v.x,v.y);
+}
+
+// Raw buffer resource over `words` complex words from base (offsets at or
+// beyond the end read 0 and drop stores).
+template<typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const cplx<T> *base, size_t words)
+{
+  return __builtin_amdgcn_make_buffer_rsrc((void *) base,0,(int) (words*sizeof(cplx<T>)),
+                                           0x00020000);
+}
+
 } // namespace fftwpp_amd

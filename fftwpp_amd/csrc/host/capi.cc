@@ -675,7 +675,8 @@ int fftwpp_amd_conv_adjoint(fftwpp_amd_conv *c, const void *w, const void *a,
                             void *stream)
 {
   if(!c || !w) return FFTWPP_AMD_ERR_ARG;
-  size_t bytes=c->words*(c->dtype == FFTWPP_AMD_F32 ? 8 : 16);
+  // real plans hold real words, the others complex words
+  size_t bytes=c->words*(c->dtype == FFTWPP_AMD_F32 ? 8 : 16)/(c->hermitian == 3 ? 2 : 1);
   const void *ptrs[5]={w,a,b,grad_a,grad_b};
   for(const void *p : ptrs)
     if(p && !fftwpp_amd_is_device_pointer(p)) {
@@ -683,10 +684,6 @@ int fftwpp_amd_conv_adjoint(fftwpp_amd_conv *c, const void *w, const void *a,
       return FFTWPP_AMD_ERR_ARG;
     }
   if(c->hermitian == 1) return hermitian_adjoint(c,w,a,b,grad_a,grad_b,stream,bytes);
-  if(c->hermitian == 3) {
-    fftwpp_amd_internal_set_error("fftwpp_amd_conv_adjoint: no adjoint for real convolutions");
-    return FFTWPP_AMD_ERR_UNSUPPORTED;
-  }
   try {
     if(grad_a) {
       if(!b) return FFTWPP_AMD_ERR_ARG;

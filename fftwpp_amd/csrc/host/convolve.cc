@@ -750,6 +750,13 @@ static void realConvolution(DeviceWork &work, size_t n, int dtype, int code,
   check(fftwpp_amd_real_unpack(dtype,z,out,n,0.5,bal,stream),"real unpack");
 }
 
+// Whether the real residue transforms take the outer dimension of a real
+// convolution (else the f + i g packing of realConvolution runs).
+static bool realPlanes(fftBase *fft)
+{
+  return fft->realInput() && fftwpp_amd_real_planes(fft->dev) > 0;
+}
+
 static void requireBinary(size_t A, size_t B, int code)
 {
   if(code < 0)
@@ -965,7 +972,11 @@ void Convolution::execDevice(void *out, const void *f0, const void *f1, bool nor
 {
   int code=mult >= 0 ? mult : multCode;
   requireBinary(A,B,code);
-  if(fft->realInput())
+  if(realPlanes(fft)) {
+    if(code != FFTWPP_AMD_MULT_BINARY && code != FFTWPP_AMD_MULT_CORRELATION)
+      throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"real convolutions take multBinary or multcorrelation");
+    execComplex(out,f0,f1,normalize,dtype,stream,code,true);
+  } else if(fft->realInput())
     realConvolution(work,arraySize(),dtype,code,out,f0,f1,stream,[&](void *z) {
       execComplex(z,z,z,normalize,dtype,stream,code);
     });
@@ -974,13 +985,27 @@ void Convolution::execDevice(void *out, const void *f0, const void *f1, bool nor
 }
 
 void Convolution::execComplex(void *out, const void *f0, const void *f1, bool normalize,
-                              int dtype, void *stream, int code)
+                              int dtype, void *stream, int code, bool real)
 {
   if(fft->C != 1)
     throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"1D Convolution with C > 1");
   double s=normalize ? scale : 1.0;
   size_t n=fft->inputLength();
   if(timeline) timeline->begin(stream);
+  if(real) {
+    // kept planes of f and g, their products, the real backward transform
+    // (fftPadReal::forward/backward around Convolution::operate)
+    const size_t K=fftwpp_amd_real_planes(fft->dev), E=eltsize(dtype);
+    void *T0=work.get(0,K*E);
+    void *T1=work.get(1,K*E);
+    check(fftwpp_amd_forward_real_many(fft->dev,dtype,f0,1,1,1,0,T0,1,0,stream),"real forward");
+    check(fftwpp_amd_forward_real_many(fft->dev,dtype,f1,1,1,1,0,T1,1,0,stream),"real forward");
+    if(timeline) timeline->mark("real forward");
+    check(fftwpp_amd_multiply(dtype,code,T0,T1,K,stream),"multiply");
+    check(fftwpp_amd_backward_real_many(fft->dev,dtype,T0,1,0,out,1,1,1,0,s,stream),"real backward");
+    if(timeline) timeline->mark("real backward");
+    return;
+  }
   if(fft->kind() == FFTWPP_AMD_KIND_HERMITIAN) {
     fusedRows(fft,work,4,dtype,code,f0,f1,1,n,out,n,s,stream,"Hermitian rows");
     if(timeline) timeline->mark("rows");
@@ -1121,7 +1146,11 @@ void Convolution2::execDevice(void *out, const void *f0, const void *f1, bool no
 {
   int code=mult >= 0 ? mult : multCode;
   requireBinary(A,B,code);
-  if(fftx->realInput())
+  if(realPlanes(fftx)) {
+    if(code != FFTWPP_AMD_MULT_BINARY && code != FFTWPP_AMD_MULT_CORRELATION)
+      throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"real convolutions take multBinary or multcorrelation");
+    execComplex(out,f0,f1,normalize,dtype,stream,code,true);
+  } else if(fftx->realInput())
     realConvolution(work,arraySize(),dtype,code,out,f0,f1,stream,[&](void *z) {
       execComplex(z,z,z,normalize,dtype,stream,code);
     });
@@ -1130,25 +1159,34 @@ void Convolution2::execDevice(void *out, const void *f0, const void *f1, bool no
 }
 
 void Convolution2::execComplex(void *out, const void *f0, const void *f1, bool normalize,
-                               int dtype, void *stream, int code)
+                               int dtype, void *stream, int code, bool real)
 {
   if(!rowsSupported(ffty,dtype))
     throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner sub-FFT size beyond the fused row kernels");
   double s=normalize ? scale : 1.0;
   size_t E=eltsize(dtype);
   size_t qx=fftx->q, mx=fftx->m;
-  size_t rows=qx*mx;
+  // x-transformed rows: every residue, or the kept planes of real data
+  size_t rows=real ? fftwpp_amd_real_planes(fftx->dev) : qx*mx;
   void *F0=work.get(0,rows*ly*E);
   void *F1=work.get(1,rows*ly*E);
   if(timeline) timeline->begin(stream);
-  check(fftwpp_amd_forward_many(fftx->dev,dtype,f0,Sx,ly,1,0,F0,ly,0,0,qx,stream),"x forward");
-  if(timeline) timeline->mark("x forward f");
-  check(fftwpp_amd_forward_many(fftx->dev,dtype,f1,Sx,ly,1,0,F1,ly,0,0,qx,stream),"x forward");
-  if(timeline) timeline->mark("x forward g");
+  const void *in[2]={f0,f1};
+  void *F[2]={F0,F1};
+  for(int a=0; a < 2; ++a) {
+    if(real)
+      check(fftwpp_amd_forward_real_many(fftx->dev,dtype,in[a],Sx,ly,1,0,F[a],ly,0,stream),"x forward");
+    else
+      check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,ly,1,0,F[a],ly,0,0,qx,stream),"x forward");
+    if(timeline) timeline->mark(real ? (a == 0 ? "x real forward f" : "x real forward g") : (a == 0 ? "x forward f" : "x forward g"));
+  }
   fusedRows(ffty,work,2,dtype,code,F0,F1,rows,ly,F0,ly,1.0,stream,"y rows");
   if(timeline) timeline->mark("y rows");
-  check(fftwpp_amd_backward_many(fftx->dev,dtype,F0,ly,0,out,Sx,ly,1,0,0,qx,0,s,stream),"x backward");
-  if(timeline) timeline->mark("x backward");
+  if(real)
+    check(fftwpp_amd_backward_real_many(fftx->dev,dtype,F0,ly,0,out,Sx,ly,1,0,s,stream),"x backward");
+  else
+    check(fftwpp_amd_backward_many(fftx->dev,dtype,F0,ly,0,out,Sx,ly,1,0,0,qx,0,s,stream),"x backward");
+  if(timeline) timeline->mark(real ? "x real backward" : "x backward");
 }
 
 void Convolution2::run(Complex **f, size_t offset, bool normalize)
@@ -1198,7 +1236,8 @@ Convolution3::~Convolution3() {}
 size_t Convolution3::workBytes(int dtype, bool plane) const
 {
   const size_t E=eltsize(dtype);
-  const size_t X=fftx->q*fftx->m*Ly*lz*E;              // one x-transformed array
+  const size_t planesX=realPlanes(fftx) ? fftwpp_amd_real_planes(fftx->dev) : fftx->q*fftx->m;
+  const size_t X=planesX*Ly*lz*E;                     // one x-transformed array
   if(plane) return 2*X+fftwpp_amd_plane_work(dtype);
   return 2*X+2*ffty->q*X;
 }
@@ -1229,7 +1268,11 @@ void Convolution3::execDevice(void *out, const void *f0, const void *f1, bool no
 {
   int code=mult >= 0 ? mult : multCode;
   requireBinary(A,B,code);
-  if(fftx->realInput())
+  if(realPlanes(fftx)) {
+    if(code != FFTWPP_AMD_MULT_BINARY && code != FFTWPP_AMD_MULT_CORRELATION)
+      throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"real convolutions take multBinary or multcorrelation");
+    execComplex(out,f0,f1,normalize,dtype,stream,code,true);
+  } else if(fftx->realInput())
     realConvolution(work,arraySize(),dtype,code,out,f0,f1,stream,[&](void *z) {
       execComplex(z,z,z,normalize,dtype,stream,code);
     });
@@ -1238,21 +1281,23 @@ void Convolution3::execDevice(void *out, const void *f0, const void *f1, bool no
 }
 
 void Convolution3::execComplex(void *out, const void *f0, const void *f1, bool normalize,
-                               int dtype, void *stream, int code)
+                               int dtype, void *stream, int code, bool real)
 {
   if(!rowsSupported(fftz,dtype))
     throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner sub-FFT size beyond the fused row kernels");
   double s=normalize ? scale : 1.0;
   size_t E=eltsize(dtype);
   size_t qx=fftx->q, mx=fftx->m, qy=ffty->q, my=ffty->m;
-  size_t nplanes=qx*mx;          // x-transformed planes of Ly x lz
+  // x-transformed planes of Ly x lz: every residue, or the kept planes of
+  // real data
+  size_t nplanes=real ? fftwpp_amd_real_planes(fftx->dev) : qx*mx;
   size_t plane=Ly*lz;
   size_t yplane=qy*my*lz;        // y-transformed plane
   void *X0=work.get(0,nplanes*plane*E);
   void *X1=work.get(1,nplanes*plane*E);
   const void *in[2]={f0,f1};
   void *X[2]={X0,X1};
-  const bool viaPlanes=code != FFTWPP_AMD_MULT_REAL_BINARY && usesPlanes(dtype);
+  const bool viaPlanes=!real && code != FFTWPP_AMD_MULT_REAL_BINARY && usesPlanes(dtype);
   if(memoryLimit && workBytes(dtype,viaPlanes) > memoryLimit)
     throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"3D convolution: work buffers exceed the memory limit");
   if(viaPlanes) {
@@ -1280,11 +1325,15 @@ void Convolution3::execComplex(void *out, const void *f0, const void *f1, bool n
   if(timeline) timeline->begin(stream);
   void *Y[2]={Y0,Y1};
   for(int a=0; a < 2; ++a) {
-    if(contiguous())
+    if(real && contiguous())
+      check(fftwpp_amd_forward_real_many(fftx->dev,dtype,in[a],Sx,plane,1,0,X[a],plane,0,stream),"x forward");
+    else if(real)
+      check(fftwpp_amd_forward_real_many(fftx->dev,dtype,in[a],Sx,lz,Ly,Sy,X[a],plane,lz,stream),"x forward");
+    else if(contiguous())
       check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,plane,1,0,X[a],plane,0,0,qx,stream),"x forward");
     else
       check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,lz,Ly,Sy,X[a],plane,lz,0,qx,stream),"x forward");
-    if(timeline) timeline->mark(a == 0 ? "x forward f" : "x forward g");
+    if(timeline) timeline->mark(real ? (a == 0 ? "x real forward f" : "x real forward g") : (a == 0 ? "x forward f" : "x forward g"));
     check(fftwpp_amd_forward_many(ffty->dev,dtype,X[a],lz,lz,nplanes,plane,Y[a],lz,yplane,0,qy,stream),"y forward");
     if(timeline) timeline->mark(a == 0 ? "y forward f" : "y forward g");
   }
@@ -1292,11 +1341,15 @@ void Convolution3::execComplex(void *out, const void *f0, const void *f1, bool n
   if(timeline) timeline->mark("z rows");
   check(fftwpp_amd_backward_many(ffty->dev,dtype,Y0,lz,yplane,X0,lz,lz,nplanes,plane,0,qy,0,1.0,stream),"y backward");
   if(timeline) timeline->mark("y backward");
-  if(contiguous())
+  if(real && contiguous())
+    check(fftwpp_amd_backward_real_many(fftx->dev,dtype,X0,plane,0,out,Sx,plane,1,0,s,stream),"x backward");
+  else if(real)
+    check(fftwpp_amd_backward_real_many(fftx->dev,dtype,X0,plane,lz,out,Sx,lz,Ly,Sy,s,stream),"x backward");
+  else if(contiguous())
     check(fftwpp_amd_backward_many(fftx->dev,dtype,X0,plane,0,out,Sx,plane,1,0,0,qx,0,s,stream),"x backward");
   else
     check(fftwpp_amd_backward_many(fftx->dev,dtype,X0,plane,lz,out,Sx,lz,Ly,Sy,0,qx,0,s,stream),"x backward");
-  if(timeline) timeline->mark("x backward");
+  if(timeline) timeline->mark(real ? "x real backward" : "x backward");
 }
 
 void Convolution3::run(Complex **f, size_t offset, bool normalize)

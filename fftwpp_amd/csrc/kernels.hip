@@ -891,6 +891,82 @@ static int colreg_t(const DimArgs &a, bool fwd, const void *in, size_t s_in,
 #undef FFTWPP_AMD_COLREG
 }
 
+// Real columns (colfwd_real_kernel, colbwd_real_kernel): power-of-two m in
+// [16, 4096], p = ceil(L/m) <= 2, non-centred data.
+static bool cols_real(const fftwpp_amd_dim *d)
+{
+  const size_t m=d->m;
+  if(m < 16 || m > 4096 || (m & (m-1))) return false;
+  if(d->kind != FFTWPP_AMD_KIND_COMPLEX) return false;
+  return d->L <= 2*m && m*d->q < ((size_t) 1 << 30);
+}
+
+template<typename T, int M>
+static int colreal_launch(const DimArgs &a, bool fwd, const void *in,
+                          size_t s_in, size_t bsi, void *out, size_t s_out,
+                          size_t bso, size_t C, size_t nbatch, double scale,
+                          size_t ics, size_t ocs, hipStream_t st)
+{
+  typedef ColReg<T,M> CR;
+  const size_t lds=M*sizeof(cplx<T>)+CR::TC*CR::CS*sizeof(T);
+  const dim3 grid((unsigned) ((C+CR::TC-1)/CR::TC),(unsigned) nbatch);
+  const bool p2=a.L > M;
+  int rc;
+  if(fwd) {
+    if(p2) {
+      if((rc=allow_lds(colfwd_real_kernel<T,M,2>,lds))) return rc;
+      hipLaunchKernelGGL((colfwd_real_kernel<T,M,2>),grid,dim3(CR::NT),lds,st,a,
+                         (const T *) in,(int64_t) s_in,(int64_t) C,(int64_t) bsi,
+                         (cplx<T> *) out,(int64_t) s_out,(int64_t) bso,(int64_t) ics,
+                         (int64_t) ocs);
+    } else {
+      if((rc=allow_lds(colfwd_real_kernel<T,M,1>,lds))) return rc;
+      hipLaunchKernelGGL((colfwd_real_kernel<T,M,1>),grid,dim3(CR::NT),lds,st,a,
+                         (const T *) in,(int64_t) s_in,(int64_t) C,(int64_t) bsi,
+                         (cplx<T> *) out,(int64_t) s_out,(int64_t) bso,(int64_t) ics,
+                         (int64_t) ocs);
+    }
+    return launch_check("colfwd_real_kernel");
+  }
+  if(p2) {
+    if((rc=allow_lds(colbwd_real_kernel<T,M,2>,lds))) return rc;
+    hipLaunchKernelGGL((colbwd_real_kernel<T,M,2>),grid,dim3(CR::NT),lds,st,a,
+                       (const cplx<T> *) in,(int64_t) s_in,(int64_t) bsi,(T *) out,
+                       (int64_t) s_out,(int64_t) C,(int64_t) bso,(T) scale,(int64_t) ics,
+                       (int64_t) ocs);
+  } else {
+    if((rc=allow_lds(colbwd_real_kernel<T,M,1>,lds))) return rc;
+    hipLaunchKernelGGL((colbwd_real_kernel<T,M,1>),grid,dim3(CR::NT),lds,st,a,
+                       (const cplx<T> *) in,(int64_t) s_in,(int64_t) bsi,(T *) out,
+                       (int64_t) s_out,(int64_t) C,(int64_t) bso,(T) scale,(int64_t) ics,
+                       (int64_t) ocs);
+  }
+  return launch_check("colbwd_real_kernel");
+}
+
+// Single columns of many batch entries become the columns of one launch, as
+// in fwd_many_t.
+template<typename T>
+static int colreal_t(const fftwpp_amd_dim *d, bool fwd, const void *in, size_t s_in,
+                     size_t bsi, void *out, size_t s_out, size_t bso, size_t C,
+                     size_t nbatch, double scale, hipStream_t st)
+{
+  const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
+  size_t ics=1, ocs=1;
+  if(C == 1 && nbatch > 1) {
+    C=nbatch; ics=bsi; ocs=bso;
+    nbatch=1; bsi=bso=0;
+  }
+#define FFTWPP_AMD_COLREAL(MM) case MM: return colreal_launch<T,MM>(a,fwd,in,s_in,bsi,out,s_out,bso,C,nbatch,scale,ics,ocs,st);
+  switch(a.m) {
+  FFTWPP_AMD_COLREAL(16) FFTWPP_AMD_COLREAL(32) FFTWPP_AMD_COLREAL(64)
+  FFTWPP_AMD_COLREAL(128) FFTWPP_AMD_COLREAL(256) FFTWPP_AMD_COLREAL(512)
+  FFTWPP_AMD_COLREAL(1024) FFTWPP_AMD_COLREAL(2048) FFTWPP_AMD_COLREAL(4096)
+  default: return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"real residue transforms: m");
+  }
+#undef FFTWPP_AMD_COLREAL
+}
+
 template<typename T>
 static int fwd_many_t(const fftwpp_amd_dim *d, const void *f, size_t S,
                       size_t C, size_t nbatch, size_t bsi, void *F,
@@ -1660,6 +1736,44 @@ int fftwpp_amd_backward_many(const fftwpp_amd_dim *d, int dtype,
     bwd_many_t<double>(d,F,SF,bstride_in,f,S,C,nbatch,bstride_out,r0,nres,accumulate,scale,st);
 }
 
+
+size_t fftwpp_amd_real_planes(const fftwpp_amd_dim *d)
+{
+  if(!d || !cols_real(d)) return 0;
+  return (size_t) RealKept::planes((int64_t) d->m,(int64_t) d->q);
+}
+
+int fftwpp_amd_forward_real_many(const fftwpp_amd_dim *d, int dtype,
+                                 const void *f, size_t S, size_t C,
+                                 size_t nbatch, size_t bstride_in, void *F,
+                                 size_t SF, size_t bstride_out, void *stream)
+{
+  if(!d || !f || !F) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  if(!cols_real(d))
+    return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"real residue transforms take power-of-two m in [16, 4096] and L <= 2m");
+  if(C == 0 || nbatch == 0) return fail(FFTWPP_AMD_ERR_ARG,"bad shape");
+  if(S < C || SF < C) return fail(FFTWPP_AMD_ERR_ARG,"stride smaller than count");
+  hipStream_t st=(hipStream_t) stream;
+  return dtype == FFTWPP_AMD_F32 ?
+    colreal_t<float>(d,true,f,S,bstride_in,F,SF,bstride_out,C,nbatch,1.0,st) :
+    colreal_t<double>(d,true,f,S,bstride_in,F,SF,bstride_out,C,nbatch,1.0,st);
+}
+
+int fftwpp_amd_backward_real_many(const fftwpp_amd_dim *d, int dtype,
+                                  const void *F, size_t SF, size_t bstride_in,
+                                  void *f, size_t S, size_t C, size_t nbatch,
+                                  size_t bstride_out, double scale, void *stream)
+{
+  if(!d || !f || !F) return fail(FFTWPP_AMD_ERR_ARG,"null argument");
+  if(!cols_real(d))
+    return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"real residue transforms take power-of-two m in [16, 4096] and L <= 2m");
+  if(C == 0 || nbatch == 0) return fail(FFTWPP_AMD_ERR_ARG,"bad shape");
+  if(S < C || SF < C) return fail(FFTWPP_AMD_ERR_ARG,"stride smaller than count");
+  hipStream_t st=(hipStream_t) stream;
+  return dtype == FFTWPP_AMD_F32 ?
+    colreal_t<float>(d,false,F,SF,bstride_in,f,S,bstride_out,C,nbatch,scale,st) :
+    colreal_t<double>(d,false,F,SF,bstride_in,f,S,bstride_out,C,nbatch,scale,st);
+}
 
 int fftwpp_amd_conv_rows(const fftwpp_amd_dim *d, int dtype, int mult,
                          const void *f, const void *g, size_t rows,

@@ -164,55 +164,6 @@ inline size_t plane_lds()
   return (size_t) 768*sizeof(cplx<T>)+(size_t) plane_xwords<T>()*sizeof(T)+16;
 }
 
-// Store of one complex word at byte offset off + soff of the buffer r.
-template<typename T>
-__device__ __forceinline__ void st_buf(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff,
-                                       cplx<T> v);
-template<>
-__device__ __forceinline__ void st_buf<double>(__amdgpu_buffer_rsrc_t r, unsigned off,
-                                               unsigned soff, cplx<double> v)
-{
-  typedef double d2 __attribute__((ext_vector_type(2)));
-  typedef unsigned u4 __attribute__((ext_vector_type(4)));
-  d2 w; w.x=v.x; w.y=v.y;
-  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4,w),r,off,soff,0);
-}
-template<>
-__device__ __forceinline__ void st_buf<float>(__amdgpu_buffer_rsrc_t r, unsigned off,
-                                              unsigned soff, cplx<float> v)
-{
-  typedef float f2 __attribute__((ext_vector_type(2)));
-  typedef unsigned u2 __attribute__((ext_vector_type(2)));
-  f2 w; w.x=v.x; w.y=v.y;
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2,w),r,off,soff,0);
-}
-// Plain (L1-cached) load of one complex word at byte offset off + soff of r.
-template<typename T>
-__device__ __forceinline__ cplx<T> ld_buf(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff);
-template<>
-__device__ __forceinline__ cplx<double> ld_buf<double>(__amdgpu_buffer_rsrc_t r, unsigned off,
-                                                       unsigned soff)
-{
-  typedef double d2 __attribute__((ext_vector_type(2)));
-  const d2 v=__builtin_bit_cast(d2,__builtin_amdgcn_raw_buffer_load_b128(r,off,soff,0));
-  return mk(v.x,v.y);
-}
-template<>
-__device__ __forceinline__ cplx<float> ld_buf<float>(__amdgpu_buffer_rsrc_t r, unsigned off,
-                                                     unsigned soff)
-{
-  typedef float f2 __attribute__((ext_vector_type(2)));
-  const f2 v=__builtin_bit_cast(f2,__builtin_amdgcn_raw_buffer_load_b64(r,off,soff,0));
-  return mk(v.x,v.y);
-}
-
-template<typename T>
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const cplx<T> *base, size_t words)
-{
-  return __builtin_amdgcn_make_buffer_rsrc((void *) base,0,(int) (words*sizeof(cplx<T>)),
-                                           0x00020000);
-}
-
 // Planes of 256 x 256 words (Ly = lz = 256).  Sub-team s of XCD x takes
 // the planes x + 8 (2 i + s); its block k owns z columns [8k, 8k+8) in
 // phases A and C and rows [16k, 16k+16) of its E in phase B.
@@ -259,7 +210,7 @@ plane256_kernel(const cplx<T> *Xf, const cplx<T> *Xg, cplx<T> *Xh,
   unsigned *cA0=&ctl->ctr[3*team][0];
   unsigned *cA1=&ctl->ctr[3*team+1][0];
   unsigned *cB=&ctl->ctr[3*team+2][0];
-  const __amdgpu_buffer_rsrc_t re=plane_rsrc(E+(size_t) team*PLANE_E,PLANE_E);
+  const __amdgpu_buffer_rsrc_t re=buf_rsrc(E+(size_t) team*PLANE_E,PLANE_E);
 
   // column phases: group g = array a in A, residue ry in C; lane (c, j)
   const int g=tid >> 7, t=tid & 127;
@@ -282,7 +233,7 @@ plane256_kernel(const cplx<T> *Xf, const cplx<T> *Xg, cplx<T> *Xh,
   const int64_t p0=x+8*sub, pstep=8*SUB;
   if(p0 >= planes) return;
   {
-    const __amdgpu_buffer_rsrc_t rx=plane_rsrc((g == 0 ? Xf : Xg)+p0*(int64_t) PW,PW);
+    const __amdgpu_buffer_rsrc_t rx=buf_rsrc((g == 0 ? Xf : Xg)+p0*(int64_t) PW,PW);
 #pragma unroll
     for(int kk=0; kk < 16; ++kk) x0[kk]=ld_buf<T>(rx,lcol,4096u*EW*kk);
   }
@@ -344,7 +295,7 @@ plane256_kernel(const cplx<T> *Xf, const cplx<T> *Xg, cplx<T> *Xh,
     {
       // (the last plane loads itself again: x0 is then dead through phase B)
       const int64_t pn=p+pstep < planes ? p+pstep : p;
-      const __amdgpu_buffer_rsrc_t rx=plane_rsrc((g == 0 ? Xf : Xg)+pn*(int64_t) PW,PW);
+      const __amdgpu_buffer_rsrc_t rx=buf_rsrc((g == 0 ? Xf : Xg)+pn*(int64_t) PW,PW);
 #pragma unroll
       for(int kk=0; kk < 16; ++kk) x0[kk]=ld_buf<T>(rx,lcol,4096u*EW*kk);
     }
@@ -368,7 +319,7 @@ plane256_kernel(const cplx<T> *Xf, const cplx<T> *Xg, cplx<T> *Xh,
 #pragma unroll
       for(int kk=0; kk < 16; ++kk) sum[(ry*16+kk)*128+t]=v[kk];
       __syncthreads();
-      const __amdgpu_buffer_rsrc_t rh=plane_rsrc(Xh+p*(int64_t) PW,PW);
+      const __amdgpu_buffer_rsrc_t rh=buf_rsrc(Xh+p*(int64_t) PW,PW);
 #pragma unroll
       for(int s8=0; s8 < 8; ++s8) {
         // word i = kk 128 + t' of the block's 16 x 128 outputs: y = j' + 16 kk

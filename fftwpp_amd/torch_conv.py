@@ -150,8 +150,9 @@ class DeviceConvolution:
 
     def adjoint(self, w, a, b, need_a=True, need_b=True):
         """Cotangents of (a, b) for output cotangent w (PyTorch convention
-        grad = dL/d conj(z)).  Complex kinds: grad_a = w (x) b, grad_b =
-        w (x) a (dealiased correlations).  Hermitian kind: the cotangents
+        grad = dL/d conj(z)).  Complex and real kinds: grad_a = w (x) b,
+        grad_b = w (x) a (dealiased correlations; real plans whose outer
+        dimension runs on the real residue transforms).  Hermitian kind: the cotangents
         of h = conv(S a, S b) with S the symmetrizer of the self-conjugate
         plane, with respect to the stored modes."""
         self._check(w)

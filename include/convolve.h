@@ -285,10 +285,14 @@ public:
 
 // Real data (fftPadReal, convolve.h:805-980 of the reference): the outer (or
 // only) dimension of a real convolution, with the reference's constructors.
-// On the device a binary convolution of real f and g is the imaginary part of
-// the complex self-convolution of z = f + i g, halved ((f + i g) * (f + i g)
-// = f*f - g*g + 2 i f*g), so the dimension is planned and transformed as a
-// complex fftPad and the drivers pack and unpack the real arrays.
+// The dimension is planned as a complex fftPad.  When its geometry has a
+// power-of-two m in [16, 4096] and L <= 2m, the drivers run the real residue
+// transforms (fftwpp_amd_forward_real_many / backward_real_many): N/2 + 1
+// kept complex planes, over which the inner dimensions convolve as complex
+// data.  Other geometries take the binary convolution of real f and g as the
+// imaginary part of the complex self-convolution of z = f + i g, halved
+// ((f + i g) * (f + i g) = f*f - g*g + 2 i f*g), with the drivers packing and
+// unpacking the real arrays.
 class fftPadReal : public fftPad {
 public:
   fftPadReal(size_t L, size_t M, Application &app, size_t C=1, size_t S=0,
@@ -395,9 +399,10 @@ protected:
   int multCode;
   DeviceWork work;
   void run(Complex **f, size_t offset, bool normalize);
-  // The complex device convolution (execDevice on complex arrays).
+  // The device convolution of complex arrays, or with real = true of real
+  // arrays through the real residue transforms of the outer dimension.
   void execComplex(void *out, const void *f0, const void *f1, bool normalize,
-                   int dtype, void *stream, int code);
+                   int dtype, void *stream, int code, bool real=false);
 };
 
 void HermitianSymmetrize(Complex *f);
@@ -448,9 +453,10 @@ protected:
   size_t ly;  // stored length of a y row (Ly or ceil(Ly/2))
   DeviceWork work;
   void run(Complex **f, size_t offset, bool normalize);
-  // The complex device convolution (execDevice on complex arrays).
+  // The device convolution of complex arrays, or with real = true of real
+  // arrays through the real residue transforms of the outer dimension.
   void execComplex(void *out, const void *f0, const void *f1, bool normalize,
-                   int dtype, void *stream, int code);
+                   int dtype, void *stream, int code, bool real=false);
 };
 
 class Convolution3 : public ThreadBase {
@@ -507,9 +513,10 @@ protected:
   bool planes=false;  // the y-z stage can run as one plane-kernel launch
   DeviceWork work;
   void run(Complex **f, size_t offset, bool normalize);
-  // The complex device convolution (execDevice on complex arrays).
+  // The device convolution of complex arrays, or with real = true of real
+  // arrays through the real residue transforms of the outer dimension.
   void execComplex(void *out, const void *f0, const void *f1, bool normalize,
-                   int dtype, void *stream, int code);
+                   int dtype, void *stream, int code, bool real=false);
 };
 
 } // namespace fftwpp

@@ -94,6 +94,31 @@ int fftwpp_amd_backward_many(const fftwpp_amd_dim *d, int dtype,
                              size_t bstride_out, size_t r0, size_t nres,
                              int accumulate, double scale, void *stream);
 
+/* Real residue transforms (fftPadReal::forward / backward and their Many,
+ * Inner and Explicit variants, convolve.cc:5449-7483, declared at
+ * convolve.h:805-980) for a non-centred dimension with power-of-two m in
+ * [16, 4096] and L <= 2m.  Real data carry conjugate transforms at padded
+ * indices l and N - l, so only fftwpp_amd_real_planes(d) = N/2 + 1 complex
+ * planes are kept, residue-major: residue 0 at s = 0..m/2, residues
+ * 0 < r < q/2 at s = 0..m-1, and (q even) residue q/2 at s = 0..m/2-1; kept
+ * plane base(r) + s holds sum_j f_j exp(2 pi i j (q s + r)/N).
+ * fftwpp_amd_real_planes returns 0 for geometries these kernels do not take.
+ *
+ * Forward: f real (element j of column c of batch b at f[b bstride_in + c +
+ * j S], real words), F complex (plane k at F[b bstride_out + c + k SF]).
+ * Backward: f_j = scale * sum over kept l of c_l Re(F_l exp(-2 pi i j l/N)),
+ * c_l = 1 for the self-conjugate planes l = 0 and l = N/2, else 2 (a kept
+ * plane stands for itself and its conjugate partner), written for j < L. */
+size_t fftwpp_amd_real_planes(const fftwpp_amd_dim *d);
+int fftwpp_amd_forward_real_many(const fftwpp_amd_dim *d, int dtype,
+                                 const void *f, size_t S, size_t C,
+                                 size_t nbatch, size_t bstride_in, void *F,
+                                 size_t SF, size_t bstride_out, void *stream);
+int fftwpp_amd_backward_real_many(const fftwpp_amd_dim *d, int dtype,
+                                  const void *F, size_t SF, size_t bstride_in,
+                                  void *f, size_t S, size_t C, size_t nbatch,
+                                  size_t bstride_out, double scale, void *stream);
+
 /* Fused innermost-dimension convolution of `rows` contiguous rows: every
  * residue forward for both inputs, the multiplier, backward, accumulate.
  * Row i of f/g starts at i*rowstride, of out at i*outstride (out may equal
@@ -297,6 +322,8 @@ int fftwpp_amd_conv_execute(fftwpp_amd_conv *h, void *a, const void *b,
 /* Adjoint of the bilinear map (a,b) -> a (*) b: given the cotangent w of the
  * output, grad_a = w (x) b and grad_b = w (x) a (dealiased correlations,
  * multcorrelation of convolve.cc:85-110 on the same padded residues).
+ * Real plans take it when their outer dimension runs on the real residue
+ * transforms (fftwpp_amd_real_planes > 0; else FFTWPP_AMD_ERR_UNSUPPORTED).
  * Pointers are device pointers; either gradient pointer may be NULL. */
 int fftwpp_amd_conv_adjoint(fftwpp_amd_conv *h, const void *w, const void *a,
                             const void *b, void *grad_a, void *grad_b,

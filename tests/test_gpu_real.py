@@ -90,15 +90,50 @@ def test_real2d_3d(gpu, oracle):
 
 
 def test_real_rejects_adjoint_and_complex_dtype(gpu, oracle):
+    """Complex dtypes are refused; so is the adjoint of a geometry that runs
+    as the packed complex self-convolution (m = 8 here)."""
     import torch
     from fftwpp_amd import FFTWPPError
     from fftwpp_amd.torch_conv import DeviceConvolution
     with pytest.raises(TypeError):
         DeviceConvolution((16,), kind="real", dtype=torch.complex128)
-    plan = DeviceConvolution((16,), kind="real", dtype=torch.float64)
-    a = torch.ones(16, dtype=torch.float64, device=gpu)
+    plan = DeviceConvolution((8,), kind="real", dtype=torch.float64, M=[16], m=[8])
+    a = torch.ones(8, dtype=torch.float64, device=gpu)
     with pytest.raises(FFTWPPError):
         plan.adjoint(a, a, a)
+
+
+def _correlate(w, b):
+    import scipy.signal as ss
+    # c[i] = sum_j w[i + j] b[j] over the stored box
+    full = ss.correlate(w, b, mode="full")
+    return full[tuple(slice(s - 1, 2 * s - 1) for s in w.shape)]
+
+
+@pytest.mark.parametrize("shape", [(1000,), (64, 48), (33, 64, 40)])
+def test_real_adjoint(gpu, oracle, shape):
+    """The adjoint of a real convolution on the real residue transforms:
+    grad_a = w (x) b and grad_b = w (x) a (correlations, scipy.signal), and
+    the adjoint identity <grad_a, d> = <w, d (*) b>."""
+    import torch
+    from fftwpp_amd.torch_conv import DeviceConvolution
+    rng = np.random.default_rng(len(shape) + 40)
+    a, b, w, dd = (rng.uniform(-1, 1, shape) for _ in range(4))
+    want_a, want_b = _correlate(w, b), _correlate(w, a)
+    for dtype, tol in ((torch.float64, TOL64), (torch.float32, TOL32)):
+        plan = DeviceConvolution(shape, kind="real", dtype=dtype)
+        N, m, q = plan.geometry()
+        assert 16 <= m[0] <= 4096 and m[0] & (m[0] - 1) == 0 and shape[0] <= 2 * m[0]
+        T = lambda x: torch.tensor(x, dtype=dtype, device=gpu)
+        ga, gb = plan.adjoint(T(w), T(a), T(b))
+        h = plan(T(dd), T(b))
+        torch.cuda.synchronize()
+        ga = ga.cpu().numpy().astype(np.float64)
+        assert oracle.rel_l2(ga, want_a) < tol, dtype
+        assert oracle.rel_l2(gb.cpu().numpy().astype(np.float64), want_b) < tol, dtype
+        lhs = float(np.sum(ga * dd))
+        rhs = float(np.sum(w * h.cpu().numpy().astype(np.float64)))
+        assert abs(lhs - rhs) <= tol * np.linalg.norm(w) * np.linalg.norm(dd) * np.linalg.norm(b), dtype
 
 
 def _ratio_inputs(shape, ratio, seed):
