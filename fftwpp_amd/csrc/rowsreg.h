@@ -229,143 +229,6 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
 }
 
 
-// Hermitian rows with H <= M for M = 512 and 1024: one wave per row, lane t
-// holding sigma = t + 64 k, k < V = M/64, so every exchange stays inside the
-// wave.  The forward transforms of f and g share one complex FFT per residue
-// r of the fold of f + i g,
-//   U_r(sigma) = w_N^{sigma r} [u(sigma) + w_q^{-r} v(sigma)],
-//   u(sigma) = f(sigma) + i g(sigma)                    (sigma < H),
-//   v(sigma) = conj f(M - sigma) + i conj g(M - sigma)  (M - sigma < H),
-// (the wavenumbers k = sigma and sigma - M of the block, f(-k) = conj f(k)),
-// whose DFT is F_r + i G_r with F_r and G_r real, since the folds of
-// Hermitian data are Hermitian: P_r = F_r G_r is realMultBinary's product
-// (fftPadHermitian::forward2 and the multiplier, convolve.cc:4517-4609,
-// 26-110).  Residues are paired for the inverse, P_{r1} + i P_{r2}, and
-// unpacked with the mirror element Z(M - sigma) as in rowreg_kernel
-// (backward2, convolve.cc:4749-4843).  A row costs q forward and ceil(q/2)
-// inverse transforms (rowreg_kernel: 3 ceil(q/2)).
-//
-// Rows are read with buffer loads whose per-lane offset is the same for all k
-// (the k-dependent part is a uniform offset), and the uniform residue
-// twiddles w_N^{64 k r} come from a small LDS table, so neither addresses nor
-// twiddles occupy registers across the residue loop.  The block's 8 waves
-// hold 8 rows; dynamic LDS: M - 1 complex FFT twiddles, q (V + 1) complex
-// residue twiddles, then 8 exchange areas of 2 RPlan<M,V>::PAD scalar words.
-template<int M> struct HRowP {
-  static_assert(M == 512 || M == 1024, "one wave per row of 8 or 16 values a lane");
-  static constexpr int V=M/64;
-  static constexpr int ROWS=8;                  // rows (waves) per block
-  template<typename T>
-  static size_t lds(int q)
-  {
-    return (size_t) (M+q*(V+1))*sizeof(cplx<T>)+(size_t) ROWS*2*RPlan<M,V>::PAD*sizeof(T);
-  }
-};
-
-// X[k] = U_r(t + 64 k); tr = the LDS twiddles w_N^{64 k r}, k <= V, of r.
-template<typename T, int M>
-__device__ __forceinline__ void hrowp_fold(cplx<T> *X, __amdgpu_buffer_rsrc_t rf,
-                                           __amdgpu_buffer_rsrc_t rg, const cplx<T> *tr,
-                                           cplx<T> b, int t, int r, int H)
-{
-  constexpr int V=HRowP<M>::V;
-  constexpr unsigned E=sizeof(cplx<T>);
-  const cplx<T> c=conjg(tr[V]);                 // w_q^{-r}
-  // sigma = t + 64 k; M - sigma = (64 - t) + 64 (V - 1 - k)
-  const unsigned o1=t*E, o2=(64-t)*E;
-#pragma unroll
-  for(int k=0; k < V; ++k) {
-    const int sigma=t+64*k;
-    cplx<T> u=mk((T) 0,(T) 0), v=u;
-    if(sigma < H) {
-      const cplx<T> fv=ld_buf<T>(rf,o1,64u*k*E), gv=ld_buf<T>(rg,o1,64u*k*E);
-      u=mk(fv.x-gv.y,fv.y+gv.x);
-    }
-    if(M-sigma < H) {
-      const cplx<T> fv=ld_buf<T>(rf,o2,64u*(V-1-k)*E), gv=ld_buf<T>(rg,o2,64u*(V-1-k)*E);
-      v=mk(fv.x+gv.y,gv.x-fv.y);
-    }
-    cplx<T> a=u+c*v;
-    if(r) a=(b*tr[k])*a;
-    X[k]=a;
-  }
-}
-
-template<typename T, int M>
-__global__ void __launch_bounds__(512)
-hrowp_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
-             int64_t rs, cplx<T> *out, int64_t os, T scale)
-{
-  constexpr int V=HRowP<M>::V, ROWS=HRowP<M>::ROWS;
-  typedef RPlan<M,V> P;
-  constexpr unsigned E=sizeof(cplx<T>);
-  extern __shared__ char smem_raw[];
-  const int q=(int) d.q, H=(int) d.H;
-  cplx<T> *tw=reinterpret_cast<cplx<T> *>(smem_raw);
-  cplx<T> *tq=tw+M;                             // tq[r (V + 1) + k] = w_N^{64 k r}
-  const cplx<T> *zN=(const cplx<T> *) d.zN;
-  fill_twr<T,M,V>(tw,zN,q,threadIdx.x,blockDim.x);
-  for(int e=threadIdx.x; e < q*(V+1); e += blockDim.x) {
-    const int r=e/(V+1), k=e-r*(V+1);
-    tq[e]=zN[64*k*r];                           // 64 k r <= M r < N
-  }
-  __syncthreads();
-  const int slot=threadIdx.x >> 6, t=threadIdx.x & 63;
-  T *xb=reinterpret_cast<T *>(tq+q*(V+1))+slot*2*P::PAD;
-  for(int64_t rb=(int64_t) blockIdx.x*ROWS; rb < rows; rb += (int64_t) gridDim.x*ROWS) {
-    const int64_t row=rb+slot;
-    if(row >= rows) break;                      // uniform over the wave
-    const __amdgpu_buffer_rsrc_t rf=buf_rsrc(f+row*rs,H), rg=buf_rsrc(g+row*rs,H);
-    cplx<T> acc[V];
-#pragma unroll
-    for(int k=0; k < V; ++k) acc[k]=mk((T) 0,(T) 0);
-    for(int r1=0; r1 < q; r1 += 2) {
-      const int r2=r1+1 < q ? r1+1 : -1;
-      const cplx<T> b1=zN[t*r1];
-      const cplx<T> b2=r2 >= 0 ? zN[t*r2] : mk((T) 0,(T) 0);
-      cplx<T> X[V];
-      T p1[V];
-      hrowp_fold<T,M>(X,rf,rg,tq+r1*(V+1),b1,t,r1,H);
-      fftr<T,M,V,1,true>(X,xb,tw,t);
-#pragma unroll
-      for(int k=0; k < V; ++k) p1[k]=X[k].x*X[k].y;
-      if(r2 >= 0) {
-        hrowp_fold<T,M>(X,rf,rg,tq+r2*(V+1),b2,t,r2,H);
-        fftr<T,M,V,1,true>(X,xb,tw,t);
-#pragma unroll
-        for(int k=0; k < V; ++k) X[k]=mk(p1[k],X[k].x*X[k].y);
-      } else {
-#pragma unroll
-        for(int k=0; k < V; ++k) X[k]=mk(p1[k],(T) 0);
-      }
-      fftr<T,M,V,-1,true>(X,xb,tw,t);
-      // mirror elements Z(M - sigma) through LDS
-#pragma unroll
-      for(int k=0; k < V; ++k) {
-        const int e=xidx<T,M,V>(t+64*k);
-        xb[e]=X[k].x;
-        xb[P::PAD+e]=X[k].y;
-      }
-      rsync<64>();
-      const cplx<T> *t1=tq+r1*(V+1), *t2=tq+(r2 >= 0 ? r2 : 0)*(V+1);
-#pragma unroll
-      for(int k=0; k < V; ++k) {
-        const int e=xidx<T,M,V>((M-(t+64*k)) & (M-1));
-        const cplx<T> z=X[k], zc=mk(xb[e],-xb[P::PAD+e]);
-        cplx<T> v=mulc((T) 0.5*(z+zc),b1*t1[k]);
-        if(r2 >= 0) v=v+mulc(muli((T) 0.5*(z-zc),-1),b2*t2[k]);
-        acc[k]=acc[k]+v;
-      }
-      rsync<64>();
-    }
-    const __amdgpu_buffer_rsrc_t ro=buf_rsrc(out+row*os,H);
-#pragma unroll
-    for(int k=0; k < V; ++k)
-      if(t+64*k < H) st_buf<T>(ro,t*E,64u*k*E,scale*acc[k]);
-  }
-}
-
-
 // Hermitian rows longer than one block (H > M): the residue-pair passes of a
 // row (fftPadHermitian's D = 2 pairs, convolve.cc:4517-5012) run as
 // independent work units (row, pass), each on one slot of TPS threads.  A
