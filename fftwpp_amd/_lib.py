@@ -28,6 +28,7 @@ SIGNATURES = {
     "fftwpp_amd_forward_real_many": (I, [P, I, P, Z, Z, Z, Z, P, Z, Z, P]),
     "fftwpp_amd_backward_real_many": (I, [P, I, P, Z, Z, P, Z, Z, Z, Z, D, P]),
     "fftwpp_amd_conv_rows": (I, [P, I, I, P, P, Z, Z, P, Z, D, P]),
+    "fftwpp_amd_dim_kernel_args": (P, [P, I]),
     "fftwpp_amd_plane_work": (Z, [I]),
     "fftwpp_amd_plane_supported": (I, [P, P]),
     "fftwpp_amd_plane_conv": (I, [P, P, I, I, P, P, P, Z, Z, Z, P, P]),
@@ -61,6 +62,7 @@ SIGNATURES = {
     # convolution layer
     "fftwpp_amd_conv_create": (I, [ctypes.POINTER(P), I, I, ZP, ZP, ZP, I]),
     "fftwpp_amd_conv_destroy": (I, [P]),
+    "fftwpp_amd_conv_execute_fused": (I, [P, P, P, P, P, P, P]),
     "fftwpp_amd_conv_set_memory_limit": (I, [P, Z]),
     "fftwpp_amd_conv_work_bytes": (I, [P, ZP, ctypes.POINTER(ctypes.c_int)]),
     "fftwpp_amd_conv_execute": (I, [P, P, P, P]),

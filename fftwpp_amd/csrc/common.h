@@ -48,6 +48,36 @@ __device__ __forceinline__ cplx<T> muli(cplx<T> a, int sign)
   return sign > 0 ? mk(-a.y,a.x) : mk(a.y,-a.x);
 }
 
+// The pointwise products of the fused row kernels.  op(F, G, row, l) returns
+// the product of the two transformed inputs at padded index l = q s + r of
+// row `row` of the row stage; op(u, v, row, l) on two reals is the product of
+// one residue of Hermitian rows, whose transforms are real (realMultBinary).
+// LibMult<MULT>: multNone (0), multBinary (1), multcorrelation (3) fixed at
+// compile time; RunMult: the same chosen at run time.  A caller's functor with
+// these two members runs in the same kernels (fused user multipliers).
+template<int MULT> struct LibMult {
+  template<typename T>
+  __device__ __forceinline__ cplx<T> operator()(cplx<T> a, cplx<T> b, int64_t, int) const
+  {
+    if constexpr(MULT == 1) return a*b;
+    else if constexpr(MULT == 3) return mulc(a,b);
+    else return a;
+  }
+  template<typename T>
+  __device__ __forceinline__ T operator()(T a, T b, int64_t, int) const { return a*b; }
+};
+
+struct RunMult {
+  int mult;
+  template<typename T>
+  __device__ __forceinline__ cplx<T> operator()(cplx<T> a, cplx<T> b, int64_t, int) const
+  {
+    return mult == 1 ? a*b : (mult == 3 ? mulc(a,b) : a);
+  }
+  template<typename T>
+  __device__ __forceinline__ T operator()(T a, T b, int64_t, int) const { return a*b; }
+};
+
 // Geometry of one padded dimension as seen by a kernel.
 constexpr int MAXSTAGES=24;
 

@@ -389,6 +389,15 @@ struct fftwpp_amd_conv {
     if(c3) c3->convolve3->timeline=t;
     if(h3) h3->convolve3->timeline=t;
   }
+  void setRowHook(const fftwpp::RowHook *r) {
+    if(c1) c1->convolve->rowHook=r;
+    if(h1) h1->convolve->rowHook=r;
+    if(k1) k1->convolve->rowHook=r;
+    if(c2) c2->convolve2->rowHook=r;
+    if(h2) h2->convolve2->rowHook=r;
+    if(c3) c3->convolve3->rowHook=r;
+    if(h3) h3->convolve3->rowHook=r;
+  }
   void exec(void *out, const void *a, const void *b, void *stream, int mult) {
     bool norm=true;
     if(c1) c1->convolve->execDevice(out,a,b,norm,dtype,stream,mult);
@@ -634,6 +643,28 @@ int fftwpp_amd_conv_execute_out(fftwpp_amd_conv *c, void *out, const void *a,
     c->exec(out,a,b,stream,-1);
   } catch(const std::exception &e) {
     return conv_status("fftwpp_amd_conv_execute_out",e);
+  }
+  return FFTWPP_AMD_OK;
+}
+
+int fftwpp_amd_conv_execute_fused(fftwpp_amd_conv *c, void *out, const void *a,
+                                  const void *b, fftwpp_amd_rows_hook hook,
+                                  void *user, void *stream)
+{
+  if(!c || !out || !a || !b || !hook) return FFTWPP_AMD_ERR_ARG;
+  if(!fftwpp_amd_is_device_pointer(out) || !fftwpp_amd_is_device_pointer(a) ||
+     !fftwpp_amd_is_device_pointer(b)) {
+    fftwpp_amd_internal_set_error("fftwpp_amd_conv_execute_fused: arguments must be device pointers");
+    return FFTWPP_AMD_ERR_ARG;
+  }
+  const fftwpp::RowHook rh={hook,user};
+  try {
+    c->setRowHook(&rh);
+    c->exec(out,a,b,stream,-1);
+    c->setRowHook(NULL);
+  } catch(const std::exception &e) {
+    c->setRowHook(NULL);
+    return conv_status("fftwpp_amd_conv_execute_fused",e);
   }
   return FFTWPP_AMD_OK;
 }

@@ -921,6 +921,49 @@ static bool rowsSupported(fftBase *fft, int dtype)
 // cover the row, else the long Hermitian path for Hermitian rows of a
 // power-of-two m (residue passes in parallel, partial sums in work slot
 // `slot`), else the LDS row kernel.
+// The row stage handed to a caller's hook (fftwpp_amd_conv_execute_fused):
+// `outer` carries the outer layout of the rows, the rest is filled here.
+static void hookRows(const RowHook *hook, fftBase *fft, int dtype, const void *f,
+                     const void *g, size_t rows, size_t rs, void *out, size_t os,
+                     double s, void *stream, fftwpp_amd_rows_args outer, const char *what)
+{
+  outer.dim=fft->dev;
+  outer.dtype=dtype;
+  outer.hermitian=fft->kind() == FFTWPP_AMD_KIND_HERMITIAN;
+  outer.f=f;
+  outer.g=g;
+  outer.out=out;
+  outer.rows=rows;
+  outer.rowstride=rs;
+  outer.outstride=os;
+  outer.scale=s;
+  outer.stream=stream;
+  const int rc=hook->hook(&outer,hook->user);
+  if(rc)
+    throw Error(rc < 0 ? rc : FFTWPP_AMD_ERR_UNSUPPORTED,
+                std::string(what)+": the fused row stage refused the geometry or failed to launch");
+}
+
+// Outer layout of the rows of a 2D (one outer dimension) or 3D (two) row stage.
+static fftwpp_amd_rows_args outerLayout(fftBase *x, size_t nx, bool realx, fftBase *y=NULL)
+{
+  fftwpp_amd_rows_args a;
+  memset(&a,0,sizeof(a));
+  a.nouter=y ? 2 : 1;
+  a.om[0]=x->m;
+  a.oq[0]=x->q;
+  a.ocount[0]=nx;
+  a.oreal[0]=realx;
+  a.ostride[0]=y ? y->q*y->m : 1;
+  if(y) {
+    a.om[1]=y->m;
+    a.oq[1]=y->q;
+    a.ocount[1]=y->q*y->m;
+    a.ostride[1]=1;
+  }
+  return a;
+}
+
 static void fusedRows(fftBase *fft, DeviceWork &work, size_t slot, int dtype,
                       int code, const void *f, const void *g, size_t rows,
                       size_t rs, void *out, size_t os, double s, void *stream,
@@ -972,6 +1015,8 @@ void Convolution::execDevice(void *out, const void *f0, const void *f1, bool nor
 {
   int code=mult >= 0 ? mult : multCode;
   requireBinary(A,B,code);
+  if(rowHook && fft->realInput() && !realPlanes(fft))
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"a fused multiplier needs the real residue transforms (power-of-two m in [16, 4096], L <= 2m)");
   if(realPlanes(fft)) {
     if(code != FFTWPP_AMD_MULT_BINARY && code != FFTWPP_AMD_MULT_CORRELATION)
       throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"real convolutions take multBinary or multcorrelation");
@@ -992,6 +1037,16 @@ void Convolution::execComplex(void *out, const void *f0, const void *f1, bool no
   double s=normalize ? scale : 1.0;
   size_t n=fft->inputLength();
   if(timeline) timeline->begin(stream);
+  if(rowHook) {
+    // the whole 1D array is the one row of the stage
+    if(real)
+      throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"a 1D real convolution has no row stage for a fused multiplier");
+    fftwpp_amd_rows_args outer;
+    memset(&outer,0,sizeof(outer));
+    hookRows(rowHook,fft,dtype,f0,f1,1,n,out,n,s,stream,outer,"rows");
+    if(timeline) timeline->mark("rows");
+    return;
+  }
   if(real) {
     // kept planes of f and g, their products, the real backward transform
     // (fftPadReal::forward/backward around Convolution::operate)
@@ -1146,6 +1201,8 @@ void Convolution2::execDevice(void *out, const void *f0, const void *f1, bool no
 {
   int code=mult >= 0 ? mult : multCode;
   requireBinary(A,B,code);
+  if(rowHook && fftx->realInput() && !realPlanes(fftx))
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"a fused multiplier needs the real residue transforms (power-of-two m in [16, 4096], L <= 2m)");
   if(realPlanes(fftx)) {
     if(code != FFTWPP_AMD_MULT_BINARY && code != FFTWPP_AMD_MULT_CORRELATION)
       throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"real convolutions take multBinary or multcorrelation");
@@ -1180,7 +1237,10 @@ void Convolution2::execComplex(void *out, const void *f0, const void *f1, bool n
       check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,ly,1,0,F[a],ly,0,0,qx,stream),"x forward");
     if(timeline) timeline->mark(real ? (a == 0 ? "x real forward f" : "x real forward g") : (a == 0 ? "x forward f" : "x forward g"));
   }
-  fusedRows(ffty,work,2,dtype,code,F0,F1,rows,ly,F0,ly,1.0,stream,"y rows");
+  if(rowHook)
+    hookRows(rowHook,ffty,dtype,F0,F1,rows,ly,F0,ly,1.0,stream,outerLayout(fftx,rows,real),"y rows");
+  else
+    fusedRows(ffty,work,2,dtype,code,F0,F1,rows,ly,F0,ly,1.0,stream,"y rows");
   if(timeline) timeline->mark("y rows");
   if(real)
     check(fftwpp_amd_backward_real_many(fftx->dev,dtype,F0,ly,0,out,Sx,ly,1,0,s,stream),"x backward");
@@ -1268,6 +1328,8 @@ void Convolution3::execDevice(void *out, const void *f0, const void *f1, bool no
 {
   int code=mult >= 0 ? mult : multCode;
   requireBinary(A,B,code);
+  if(rowHook && fftx->realInput() && !realPlanes(fftx))
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"a fused multiplier needs the real residue transforms (power-of-two m in [16, 4096], L <= 2m)");
   if(realPlanes(fftx)) {
     if(code != FFTWPP_AMD_MULT_BINARY && code != FFTWPP_AMD_MULT_CORRELATION)
       throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"real convolutions take multBinary or multcorrelation");
@@ -1297,7 +1359,7 @@ void Convolution3::execComplex(void *out, const void *f0, const void *f1, bool n
   void *X1=work.get(1,nplanes*plane*E);
   const void *in[2]={f0,f1};
   void *X[2]={X0,X1};
-  const bool viaPlanes=!real && code != FFTWPP_AMD_MULT_REAL_BINARY && usesPlanes(dtype);
+  const bool viaPlanes=!real && !rowHook && code != FFTWPP_AMD_MULT_REAL_BINARY && usesPlanes(dtype);
   if(memoryLimit && workBytes(dtype,viaPlanes) > memoryLimit)
     throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"3D convolution: work buffers exceed the memory limit");
   if(viaPlanes) {
@@ -1337,7 +1399,11 @@ void Convolution3::execComplex(void *out, const void *f0, const void *f1, bool n
     check(fftwpp_amd_forward_many(ffty->dev,dtype,X[a],lz,lz,nplanes,plane,Y[a],lz,yplane,0,qy,stream),"y forward");
     if(timeline) timeline->mark(a == 0 ? "y forward f" : "y forward g");
   }
-  fusedRows(fftz,work,4,dtype,code,Y0,Y1,nplanes*qy*my,lz,Y0,lz,1.0,stream,"z rows");
+  if(rowHook)
+    hookRows(rowHook,fftz,dtype,Y0,Y1,nplanes*qy*my,lz,Y0,lz,1.0,stream,
+             outerLayout(fftx,nplanes,real,ffty),"z rows");
+  else
+    fusedRows(fftz,work,4,dtype,code,Y0,Y1,nplanes*qy*my,lz,Y0,lz,1.0,stream,"z rows");
   if(timeline) timeline->mark("z rows");
   check(fftwpp_amd_backward_many(ffty->dev,dtype,Y0,lz,yplane,X0,lz,lz,nplanes,plane,0,qy,0,1.0,stream),"y backward");
   if(timeline) timeline->mark("y backward");

@@ -31,6 +31,7 @@
 #include "batch.h"
 #include "rowsreg.h"
 #include "plane256.h"
+#include "rowsfused.h"
 #include "../../include/fftwpp_amd.h"
 
 using namespace fftwpp_amd;
@@ -1220,125 +1221,27 @@ static bool rows_reg(const fftwpp_amd_dim *d)
   return d->L <= m;
 }
 
-// RP residue lanes per row when rows are too few to fill the chip (under 512)
-// and the row has at least two residue passes (residues, or residue pairs for
-// Hermitian rows).
-template<typename T, int M, bool HERM, int RP>
-static int rowreg_launch_rp(const DimArgs &a, const void *f, const void *g,
-                            size_t rows, size_t rs, void *out, size_t os,
-                            double scale, int mult, hipStream_t st)
-{
-  constexpr int V=RowV::V;
-  constexpr int TPS=M/V;
-  const size_t nt=RP > 1 ? RP*TPS : (TPS > 64 ? TPS : 256);
-  const size_t slots=nt/TPS;
-  const size_t lds=M*sizeof(cplx<T>)+slots*2*RPlan<M,V>::PAD*sizeof(T);
-  int rc=allow_lds(rowreg_kernel<T,M,HERM,RP>,lds);
-  if(rc) return rc;
-  const size_t rpb=RP > 1 ? 1 : slots;
-  size_t nb=(rows+rpb-1)/rpb;
-  // persistent grid: the twiddle table is filled once per resident block
-  nb=std::min(nb,resident_blocks(rowreg_kernel<T,M,HERM,RP>,(int) nt,lds));
-  hipLaunchKernelGGL((rowreg_kernel<T,M,HERM,RP>),dim3((unsigned) nb),dim3((unsigned) nt),lds,st,
-                     a,(const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
-                     (cplx<T> *) out,(int64_t) os,(T) scale,mult);
-  return launch_check("rowreg_kernel");
-}
-
-template<typename T, int M, bool HERM>
-static int rowreg_launch(const DimArgs &a, const void *f, const void *g,
-                         size_t rows, size_t rs, void *out, size_t os,
-                         double scale, int mult, hipStream_t st)
-{
-  constexpr int TPS=M/RowV::V;
-  const int64_t passes=HERM ? (a.q+1)/2 : a.q;
-  if(TPS <= 256 && rows < 512 && passes >= 2)
-    return rowreg_launch_rp<T,M,HERM,2>(a,f,g,rows,rs,out,os,scale,mult,st);
-  return rowreg_launch_rp<T,M,HERM,1>(a,f,g,rows,rs,out,os,scale,mult,st);
-}
-
-template<typename T, bool HERM>
-static int rowreg_t(const DimArgs &a, const void *f, const void *g, size_t rows,
-                    size_t rs, void *out, size_t os, double scale, int mult,
-                    hipStream_t st)
-{
-  switch(a.m) {
-  case 16: return rowreg_launch<T,16,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
-  case 32: return rowreg_launch<T,32,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
-  case 64: return rowreg_launch<T,64,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
-  case 128: return rowreg_launch<T,128,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
-  case 256: return rowreg_launch<T,256,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
-  case 512: return rowreg_launch<T,512,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
-  case 1024: return rowreg_launch<T,1024,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
-  case 2048: return rowreg_launch<T,2048,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
-  case 4096: return rowreg_launch<T,4096,HERM>(a,f,g,rows,rs,out,os,scale,mult,st);
-  default: return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"register FFT size");
-  }
-}
-
 template<typename T>
 static int conv_rows_t(const fftwpp_amd_dim *d, int mult, const void *f,
                        const void *g, size_t rows, size_t rs, void *out,
                        size_t os, double scale, hipStream_t st)
 {
   const DimArgs &a=sizeof(T) == 8 ? d->a64 : d->a32;
-  // q = 2: both residues of a row in one wave (each row read once)
-  if(fast256(d,rs,os) && d->q == 2) {
-    constexpr int WPB=4;
-    const size_t need=(rows+2*WPB-1)/(2*WPB);
-    const cplx<T> *zt=(const cplx<T> *) a.zN;
-    size_t lds=row256_lds<T>(4*WPB,2);
+  // the register row kernels (rowsfused.h), the m = 256 ones with the
+  // multiplier fixed at compile time
+  {
+    const bool hm=d->kind == FFTWPP_AMD_KIND_HERMITIAN;
     int rc;
-    // persistent grid: at most the blocks that are resident at once
-    if(mult == FFTWPP_AMD_MULT_CORRELATION) {
-      if((rc=allow_lds(rowconv256p_kernel<T,3,WPB>,lds))) return rc;
-      dim3 grid((unsigned) std::min(need,resident_blocks(rowconv256p_kernel<T,3,WPB>,64*WPB,lds)));
-      hipLaunchKernelGGL((rowconv256p_kernel<T,3,WPB>),grid,dim3(64*WPB),lds,st,
-                         (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
-                         (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt);
-    } else if(mult == FFTWPP_AMD_MULT_NONE) {
-      if((rc=allow_lds(rowconv256p_kernel<T,0,WPB>,lds))) return rc;
-      dim3 grid((unsigned) std::min(need,resident_blocks(rowconv256p_kernel<T,0,WPB>,64*WPB,lds)));
-      hipLaunchKernelGGL((rowconv256p_kernel<T,0,WPB>),grid,dim3(64*WPB),lds,st,
-                         (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
-                         (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt);
-    } else {
-      if((rc=allow_lds(rowconv256p_kernel<T,1,WPB>,lds))) return rc;
-      dim3 grid((unsigned) std::min(need,resident_blocks(rowconv256p_kernel<T,1,WPB>,64*WPB,lds)));
-      hipLaunchKernelGGL((rowconv256p_kernel<T,1,WPB>),grid,dim3(64*WPB),lds,st,
-                         (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
-                         (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt);
-    }
-    return launch_check("rowconv256p_kernel");
-  }
-  if(fast256(d,rs,os)) {
-    constexpr int NR=rows256<T>();
-    dim3 grid((unsigned) ((rows+NR-1)/NR));
-    const cplx<T> *zt=(const cplx<T> *) a.zN;
-    size_t lds=row256_lds<T>(NR,d->q);
-    int rc;
-    if(mult == FFTWPP_AMD_MULT_CORRELATION) {
-      if((rc=allow_lds(rowconv256_kernel<T,3,NR>,lds))) return rc;
-      hipLaunchKernelGGL((rowconv256_kernel<T,3,NR>),grid,dim3(16*NR),lds,st,
-                         (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
-                         (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt,(int64_t) d->q);
-    } else if(mult == FFTWPP_AMD_MULT_NONE) {
-      if((rc=allow_lds(rowconv256_kernel<T,0,NR>,lds))) return rc;
-      hipLaunchKernelGGL((rowconv256_kernel<T,0,NR>),grid,dim3(16*NR),lds,st,
-                         (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
-                         (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt,(int64_t) d->q);
-    } else {
-      if((rc=allow_lds(rowconv256_kernel<T,1,NR>,lds))) return rc;
-      hipLaunchKernelGGL((rowconv256_kernel<T,1,NR>),grid,dim3(16*NR),lds,st,
-                         (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
-                         (cplx<T> *) out,(int64_t) os,(int) d->L,(T) scale,zt,(int64_t) d->q);
-    }
-    return launch_check("rowconv256_kernel");
-  }
-  if(rows_reg(d)) {
-    if(d->kind == FFTWPP_AMD_KIND_HERMITIAN)
-      return rowreg_t<T,true>(a,f,g,rows,rs,out,os,scale,mult,st);
-    return rowreg_t<T,false>(a,f,g,rows,rs,out,os,scale,mult,st);
+    if(rows256_take(a,hm,rs,os) && mult == FFTWPP_AMD_MULT_CORRELATION)
+      rc=rows_fused<T>(a,hm,f,g,rows,rs,out,os,scale,LibMult<3>(),st);
+    else if(rows256_take(a,hm,rs,os) && mult == FFTWPP_AMD_MULT_NONE)
+      rc=rows_fused<T>(a,hm,f,g,rows,rs,out,os,scale,LibMult<0>(),st);
+    else if(rows256_take(a,hm,rs,os))
+      rc=rows_fused<T>(a,hm,f,g,rows,rs,out,os,scale,LibMult<1>(),st);
+    else
+      rc=rows_fused<T>(a,hm,f,g,rows,rs,out,os,scale,RunMult{mult},st);
+    if(rc < 0) return hipcheck((hipError_t) -rc,"fused row kernel");
+    if(rc == 0) return FFTWPP_AMD_OK;
   }
   size_t elt=sizeof(cplx<T>);
   bool herm=d->kind == FFTWPP_AMD_KIND_HERMITIAN;
@@ -1736,6 +1639,12 @@ int fftwpp_amd_backward_many(const fftwpp_amd_dim *d, int dtype,
     bwd_many_t<double>(d,F,SF,bstride_in,f,S,C,nbatch,bstride_out,r0,nres,accumulate,scale,st);
 }
 
+
+const void *fftwpp_amd_dim_kernel_args(const fftwpp_amd_dim *d, int dtype)
+{
+  if(!d) return NULL;
+  return dtype == FFTWPP_AMD_F32 ? (const void *) &d->a32 : (const void *) &d->a64;
+}
 
 size_t fftwpp_amd_real_planes(const fftwpp_amd_dim *d)
 {

@@ -187,11 +187,11 @@ template<typename T> struct Row256 {
   static constexpr bool KEEP=sizeof(T) == 4;
   static constexpr int MINW=2;
 };
-template<typename T, int MULT, int NR>
+template<typename T, class Op, int NR>
 __global__ void __launch_bounds__(16*NR,Row256<T>::MINW)
 rowconv256_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows, int64_t rs,
                   cplx<T> *out, int64_t os, int L, T scale, const cplx<T> *zN,
-                  int64_t q)
+                  int64_t q, Op op)
 {
   extern __shared__ char smem_raw[];
   const int qi=(int) q;
@@ -248,10 +248,7 @@ rowconv256_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows, int64_t rs,
     fft256s<T,1,1>(a,row,j,0,tw16);
     fft256s<T,1,1>(b,row,j,0,tw16);
 #pragma unroll
-    for(int k=0; k < 16; ++k) {
-      if(MULT == 1) a[k]=a[k]*b[k];
-      else if(MULT == 3) a[k]=mulc(a[k],b[k]);
-    }
+    for(int k=0; k < 16; ++k) a[k]=op(a[k],b[k],row0+i,qi*(j+16*k)+r);
     fft256s<T,-1,1>(a,row,j,0,tw16);
 #pragma unroll
     for(int k=0; k < 16; ++k)
@@ -294,11 +291,11 @@ __device__ __forceinline__ double pair16_sum(double v)
 // persistent: each block fills its twiddle tables once and its waves, which
 // exchange only within themselves, loop over row pairs independently.
 // Dynamic LDS: row256_lds(4 WPB, 2).
-template<typename T, int MULT, int WPB>
+template<typename T, class Op, int WPB>
 __global__ void __launch_bounds__(64*WPB,2)
 rowconv256p_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows,
                    int64_t rs, cplx<T> *out, int64_t os, int L, T scale,
-                   const cplx<T> *zN)
+                   const cplx<T> *zN, Op op)
 {
   extern __shared__ char smem_raw[];
   cplx<T> *rtab=reinterpret_cast<cplx<T> *>(smem_raw);   // 2 x 256
@@ -333,10 +330,7 @@ rowconv256p_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows,
     fft256s<T,1,1,RS>(a,xrow,j,0,tw16);
     fft256s<T,1,1,RS>(b,xrow,j,0,tw16);
 #pragma unroll
-    for(int k=0; k < 16; ++k) {
-      if(MULT == 1) a[k]=a[k]*b[k];
-      else if(MULT == 3) a[k]=mulc(a[k],b[k]);
-    }
+    for(int k=0; k < 16; ++k) a[k]=op(a[k],b[k],row,2*(j+16*k)+r);
     fft256s<T,-1,1,RS>(a,xrow,j,0,tw16);
 #pragma unroll
     for(int k=0; k < 16; ++k) {

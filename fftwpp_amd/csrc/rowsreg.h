@@ -73,10 +73,10 @@ __device__ __forceinline__ void rowreg_combine(cplx<T> *acc, T *xbase, T *xb,
 // RP = 1: each slot owns a row (one slot per block when TPS > 64).  RP > 1:
 // the block holds one row and its RP slots take every RP-th residue (pair);
 // their sums are added through LDS before the row is written.
-template<typename T, int M, bool HERM, int RP>
+template<typename T, int M, bool HERM, int RP, class Op>
 __global__ void __launch_bounds__(512)
 rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
-              int64_t rs, cplx<T> *out, int64_t os, T scale, int mult)
+              int64_t rs, cplx<T> *out, int64_t os, T scale, Op op)
 {
   constexpr int V=RowV::V;
   typedef RPlan<M,V> P;
@@ -125,8 +125,7 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
         fftr<T,M,V,1,true>(X,xb,tw,t);
         fftr<T,M,V,1,true>(Y,xb,tw,t);
 #pragma unroll
-        for(int k=0; k < V; ++k)
-          X[k]=mult == 1 ? X[k]*Y[k] : (mult == 3 ? mulc(X[k],Y[k]) : X[k]);
+        for(int k=0; k < V; ++k) X[k]=op(X[k],Y[k],row,q*(t+TPS*k)+r);
         fftr<T,M,V,-1,true>(X,xb,tw,t);
 #pragma unroll
         for(int k=0; k < V; ++k) {
@@ -183,7 +182,10 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
         fftr<T,M,V,1,true>(X,xb,tw,t);
         fftr<T,M,V,1,true>(Y,xb,tw,t);
 #pragma unroll
-        for(int k=0; k < V; ++k) X[k]=mk(X[k].x*Y[k].x,X[k].y*Y[k].y);
+        for(int k=0; k < V; ++k) {
+          const int l=q*(t+TPS*k);
+          X[k]=mk(op(X[k].x,Y[k].x,row,l+r1),r2 >= 0 ? op(X[k].y,Y[k].y,row,l+r2) : (T) 0);
+        }
         fftr<T,M,V,-1,true>(X,xb,tw,t);
         // mirror elements Z(M - sigma) through LDS
         cplx<T> Zm[V];

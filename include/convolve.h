@@ -358,6 +358,14 @@ void generalConvolution(fftBase *const *dims, size_t nd, size_t A, size_t B,
                         DeviceWork &work, int dtype, void *const *f,
                         double scale, void *stream, const DeviceMultiplier &mult);
 
+// A caller-supplied inner row stage (fftwpp_amd_conv_execute_fused): when a
+// convolution's rowHook is set, its fused innermost-row stage is handed to
+// hook(args, user) instead of the library's row kernels.
+struct RowHook {
+  fftwpp_amd_rows_hook hook;
+  void *user;
+};
+
 class Convolution : public ThreadBase {
 public:
   fftBase *fft;
@@ -394,6 +402,7 @@ public:
   size_t arraySize() const;  // complex words per input array
 
   Timeline *timeline=NULL;  // when set, every launch is timed
+  const RowHook *rowHook=NULL;  // when set, the row stage runs through it
 
 protected:
   int multCode;
@@ -447,6 +456,7 @@ public:
   size_t arraySize() const;
 
   Timeline *timeline=NULL;  // when set, every launch is timed
+  const RowHook *rowHook=NULL;  // when set, the row stage runs through it
 
 protected:
   int multCode;
@@ -496,6 +506,7 @@ public:
   size_t arraySize() const;
 
   Timeline *timeline=NULL;  // when set, every launch is timed
+  const RowHook *rowHook=NULL;  // when set, the row stage runs through it
 
   // Device bytes the plan's work buffers may hold (0: no limit).  The
   // separate y, z and y launches need 2 + 2 q_y x-transformed arrays; the

@@ -397,6 +397,51 @@ int fftwpp_amd_conv_execute_general(fftwpp_amd_conv *h, void *const *f,
                                     void *stream);
 
 /* ------------------------------------------------------------------------
+ * Fused user multipliers: the multiplier runs inside the fused innermost-row
+ * kernels, as the library multipliers do (Convolution::operate inside the
+ * residue loop, convolve.h:1120-1135), instead of on expanded arrays.  The
+ * multiplier is device code compiled into those kernels by the caller
+ * (include/fftwpp_amd_fused.h, hipcc); the library runs every other stage of
+ * a plan made by fftwpp_amd_conv_create and hands the row stage to `hook`.
+ *
+ * Row i of the stage (f, g, out at i*rowstride, i*outstride; out may equal
+ * f) lies at the outer padded grid point (x_0, ..., x_{nouter-1}):
+ * i = sum_d j_d ostride[d] with j_d < ocount[d], and
+ * x_d = oq[d] (j_d mod om[d]) + j_d / om[d] (FFTWPP_AMD_GRID_POINT), except
+ * for a real outer dimension (oreal[d] = 1), whose j_d indexes the kept planes
+ * of fftwpp_amd_real_planes (x_d of plane base(r) + s is q s + r).  Within a
+ * row the multiplier sees padded index l = q s + r of the row dimension.
+ * The hook returns 0 after enqueueing the stage on `stream`, or nonzero (the
+ * row geometry is outside the register row kernels, or a HIP error). */
+typedef struct fftwpp_amd_rows_args {
+  const fftwpp_amd_dim *dim;   /* the row (innermost) dimension */
+  int dtype;                   /* FFTWPP_AMD_F64 or FFTWPP_AMD_F32 */
+  int hermitian;               /* Hermitian rows: real residue values */
+  const void *f, *g;           /* transformed inputs of the row stage */
+  void *out;
+  size_t rows, rowstride, outstride;
+  double scale;                /* applied to the row results */
+  void *stream;
+  int nouter;                  /* 0 (1D), 1 (2D), 2 (3D) */
+  size_t om[2], oq[2], ocount[2], ostride[2];
+  int oreal[2];
+} fftwpp_amd_rows_args;
+
+typedef int (*fftwpp_amd_rows_hook)(const fftwpp_amd_rows_args *args, void *user);
+
+/* conv_execute_out with the row stage handed to `hook` (plans of
+ * fftwpp_amd_conv_create; a 1D real plan has no row stage and is refused;
+ * the 3D plane kernel is not used). */
+int fftwpp_amd_conv_execute_fused(fftwpp_amd_conv *h, void *out, const void *a,
+                                  const void *b, fftwpp_amd_rows_hook hook,
+                                  void *user, void *stream);
+
+/* The kernel arguments of a dimension (a fftwpp_amd::DimArgs of
+ * fftwpp_amd/csrc/common.h, owned by d) for code that launches the row
+ * kernels itself; NULL for a bad dimension. */
+const void *fftwpp_amd_dim_kernel_args(const fftwpp_amd_dim *d, int dtype);
+
+/* ------------------------------------------------------------------------
  * Batches of independent 1D complex convolutions of uneven lengths.  No
  * reference counterpart: FFTW++ convolves one set of arrays per Convolution
  * object and call (convolve.h:982-1166); a batch of n items replaces n such

@@ -1,5 +1,6 @@
 """Build tests/support/libtestmult.so: the user device multipliers of
-multipliers.hip, the C++ FFT face of fftface.cc and the reference transform
+multipliers.hip, the fused multipliers of fusedmult.hip (compiled into the
+library's row kernels), the C++ FFT face of fftface.cc and the reference transform
 tests of residues.cc, linked against the in-tree libfftwpp_amd.so (found next to
 the package at run time through $ORIGIN)."""
 import os
@@ -15,7 +16,9 @@ def build(verbose=False):
     sys.path.insert(0, ROOT)
     from fftwpp_amd import build as b
     pkg = os.path.join(ROOT, "fftwpp_amd")
-    cmd = [b.HIPCC] + b.HIP_FLAGS + ["-shared", os.path.join(HERE, "multipliers.hip"),
+    cmd = [b.HIPCC] + b.HIP_FLAGS + ["-I", os.path.join(pkg, "csrc"),
+                                     "-shared", os.path.join(HERE, "multipliers.hip"),
+                                     os.path.join(HERE, "fusedmult.hip"),
                                      os.path.join(HERE, "fftface.cc"),
                                      os.path.join(HERE, "residues.cc"),
                                      "-L", pkg, "-lfftwpp_amd",
