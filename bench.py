@@ -552,10 +552,19 @@ def bench_user_multiplier(torch, dev, steps, warmup):
     out = torch.empty_like(f)
     stream = torch.cuda.current_stream().cuda_stream
 
+    fb = ctypes.CDLL(lib).fusedmult_binary
+    fb.restype = ctypes.c_int
+    fb.argtypes = [ctypes.c_void_p] * 5
     from fftwpp_amd import _lib
 
     def fused_call(c):
         _lib.check(fn(fused._h, out.data_ptr(), f.data_ptr(), g.data_ptr(), Ns, 3, c, stream))
+
+    def binary_call():
+        _lib.check(fb(fused._h, out.data_ptr(), f.data_ptr(), g.data_ptr(), stream))
+    ms_b = _time_calls(torch, binary_call, steps, warmup)
+    torch.cuda.synchronize()
+    same_b = bool(torch.equal(out, h))
     ms_f = _time_calls(torch, lambda: fused_call(1e-6), steps, warmup)
     fused_call(0.0)
     torch.cuda.synchronize()
@@ -569,8 +578,13 @@ def bench_user_multiplier(torch, dev, steps, warmup):
             "frac_of_hbm_peak": round(bytes_alg / (ms * 1e-3) / 1e9 / PEAK_GBS, 4),
             "bytes_alg": bytes_alg, "rel_l2_vs_fused": err,
             "fused_functor": {
-                "workload": "the same convolution with a device functor F G / (1 + 1e-6 |k|^2) "
-                            "compiled into the z-row kernel",
+                "workload": "the same multiplier (F G) as a caller's device functor compiled "
+                            "into the z-row kernel (include/fftwpp_amd_fused.h)",
+                "conv_per_s": round(1000.0 / ms_b, 2), "ms_per_conv": round(ms_b, 4),
+                "equals_library_multBinary": same_b},
+            "fused_functor_weighted": {
+                "workload": "a device functor F G / (1 + 1e-6 |k|^2), k the padded grid "
+                            "frequency in x, y, z",
                 "conv_per_s": round(1000.0 / ms_f, 2), "ms_per_conv": round(ms_f, 4),
                 "unit_weight_equals_library_multBinary": same}}
 

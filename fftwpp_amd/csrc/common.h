@@ -12,6 +12,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <utility>
 
 namespace fftwpp_amd {
 
@@ -52,6 +53,7 @@ __device__ __forceinline__ cplx<T> muli(cplx<T> a, int sign)
 // the product of the two transformed inputs at padded index l = q s + r of
 // row `row` of the row stage; op(u, v, row, l) on two reals is the product of
 // one residue of Hermitian rows, whose transforms are real (realMultBinary).
+// The kernels call it through bind_row (below) once per row.
 // LibMult<MULT>: multNone (0), multBinary (1), multcorrelation (3) fixed at
 // compile time; RunMult: the same chosen at run time.  A caller's functor with
 // these two members runs in the same kernels (fused user multipliers).
@@ -66,6 +68,34 @@ template<int MULT> struct LibMult {
   template<typename T>
   __device__ __forceinline__ T operator()(T a, T b, int64_t, int) const { return a*b; }
 };
+
+// A product functor bound to one row: op.at(row) when Op provides it (the
+// functor's work that depends on the row alone, done once per row), else a
+// wrapper that passes the row to every call.  Either way rop(F, G, l).
+template<class Op> struct RowBound {
+  Op op;
+  int64_t row;
+  template<typename T>
+  __device__ __forceinline__ cplx<T> operator()(cplx<T> a, cplx<T> b, int l) const
+  {
+    return op(a,b,row,l);
+  }
+  template<typename T>
+  __device__ __forceinline__ T operator()(T a, T b, int l) const { return op(a,b,row,l); }
+};
+
+template<class Op, class=void> struct HasAt { static constexpr bool value=false; };
+template<class Op>
+struct HasAt<Op,decltype((void) std::declval<const Op &>().at((int64_t) 0))> {
+  static constexpr bool value=true;
+};
+
+template<class Op>
+__device__ __forceinline__ auto bind_row(const Op &op, int64_t row)
+{
+  if constexpr(HasAt<Op>::value) return op.at(row);
+  else return RowBound<Op>{op,row};
+}
 
 struct RunMult {
   int mult;

@@ -247,8 +247,11 @@ rowconv256_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows, int64_t rs,
     }
     fft256s<T,1,1>(a,row,j,0,tw16);
     fft256s<T,1,1>(b,row,j,0,tw16);
+    {
+      const auto rop=bind_row(op,row0+i);
 #pragma unroll
-    for(int k=0; k < 16; ++k) a[k]=op(a[k],b[k],row0+i,qi*(j+16*k)+r);
+      for(int k=0; k < 16; ++k) a[k]=rop(a[k],b[k],qi*(j+16*k)+r);
+    }
     fft256s<T,-1,1>(a,row,j,0,tw16);
 #pragma unroll
     for(int k=0; k < 16; ++k)
@@ -329,8 +332,11 @@ rowconv256p_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows,
     }
     fft256s<T,1,1,RS>(a,xrow,j,0,tw16);
     fft256s<T,1,1,RS>(b,xrow,j,0,tw16);
+    {
+      const auto rop=bind_row(op,row);
 #pragma unroll
-    for(int k=0; k < 16; ++k) a[k]=op(a[k],b[k],row,2*(j+16*k)+r);
+      for(int k=0; k < 16; ++k) a[k]=rop(a[k],b[k],2*(j+16*k)+r);
+    }
     fft256s<T,-1,1,RS>(a,xrow,j,0,tw16);
 #pragma unroll
     for(int k=0; k < 16; ++k) {

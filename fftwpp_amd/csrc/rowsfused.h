@@ -127,21 +127,35 @@ inline int rowreg_launch(const DimArgs &a, const void *f, const void *g, size_t 
   return rowreg_launch_rp<T,M,HERM,1>(a,f,g,rows,rs,out,os,scale,op,st);
 }
 
-template<typename T, bool HERM, class Op>
+// Row sizes compiled: bit i of MMASK stands for m = 16 << i (16 ... 4096);
+// sizes left out are reported as not taken (1).
+constexpr unsigned ROWS_ALL_SIZES=0x1FFu;
+
+template<typename T, int M, bool HERM, unsigned MMASK, class Op>
+inline int rowreg_if(const DimArgs &a, const void *f, const void *g, size_t rows, size_t rs,
+                     void *out, size_t os, double scale, const Op &op, hipStream_t st)
+{
+  if constexpr((MMASK >> (__builtin_ctz(M)-4)) & 1u)
+    return rowreg_launch<T,M,HERM>(a,f,g,rows,rs,out,os,scale,op,st);
+  else
+    return 1;
+}
+
+template<typename T, bool HERM, unsigned MMASK, class Op>
 inline int rowreg_dispatch(const DimArgs &a, const void *f, const void *g, size_t rows,
                            size_t rs, void *out, size_t os, double scale, const Op &op,
                            hipStream_t st)
 {
   switch(a.m) {
-  case 16: return rowreg_launch<T,16,HERM>(a,f,g,rows,rs,out,os,scale,op,st);
-  case 32: return rowreg_launch<T,32,HERM>(a,f,g,rows,rs,out,os,scale,op,st);
-  case 64: return rowreg_launch<T,64,HERM>(a,f,g,rows,rs,out,os,scale,op,st);
-  case 128: return rowreg_launch<T,128,HERM>(a,f,g,rows,rs,out,os,scale,op,st);
-  case 256: return rowreg_launch<T,256,HERM>(a,f,g,rows,rs,out,os,scale,op,st);
-  case 512: return rowreg_launch<T,512,HERM>(a,f,g,rows,rs,out,os,scale,op,st);
-  case 1024: return rowreg_launch<T,1024,HERM>(a,f,g,rows,rs,out,os,scale,op,st);
-  case 2048: return rowreg_launch<T,2048,HERM>(a,f,g,rows,rs,out,os,scale,op,st);
-  case 4096: return rowreg_launch<T,4096,HERM>(a,f,g,rows,rs,out,os,scale,op,st);
+  case 16: return rowreg_if<T,16,HERM,MMASK>(a,f,g,rows,rs,out,os,scale,op,st);
+  case 32: return rowreg_if<T,32,HERM,MMASK>(a,f,g,rows,rs,out,os,scale,op,st);
+  case 64: return rowreg_if<T,64,HERM,MMASK>(a,f,g,rows,rs,out,os,scale,op,st);
+  case 128: return rowreg_if<T,128,HERM,MMASK>(a,f,g,rows,rs,out,os,scale,op,st);
+  case 256: return rowreg_if<T,256,HERM,MMASK>(a,f,g,rows,rs,out,os,scale,op,st);
+  case 512: return rowreg_if<T,512,HERM,MMASK>(a,f,g,rows,rs,out,os,scale,op,st);
+  case 1024: return rowreg_if<T,1024,HERM,MMASK>(a,f,g,rows,rs,out,os,scale,op,st);
+  case 2048: return rowreg_if<T,2048,HERM,MMASK>(a,f,g,rows,rs,out,os,scale,op,st);
+  case 4096: return rowreg_if<T,4096,HERM,MMASK>(a,f,g,rows,rs,out,os,scale,op,st);
   default: return 1;
   }
 }
@@ -164,17 +178,24 @@ inline bool rowreg_take(const DimArgs &a, bool herm)
   return herm ? a.H <= m : a.L <= m;
 }
 
-template<typename T, class Op>
+// HERMS: which row kinds are compiled (bit 0 complex or centred, bit 1
+// Hermitian).
+template<typename T, class Op, unsigned MMASK=ROWS_ALL_SIZES, unsigned HERMS=3u>
 inline int rows_fused(const DimArgs &a, bool herm, const void *f, const void *g, size_t rows,
                       size_t rs, void *out, size_t os, double scale, const Op &op,
                       hipStream_t st)
 {
   if(rows == 0) return 0;
-  if(rows256_take(a,herm,rs,os))
-    return rows256_launch<T>(a,f,g,rows,rs,out,os,scale,op,st);
-  if(rowreg_take(a,herm))
-    return herm ? rowreg_dispatch<T,true>(a,f,g,rows,rs,out,os,scale,op,st) :
-      rowreg_dispatch<T,false>(a,f,g,rows,rs,out,os,scale,op,st);
+  if(!((herm ? 2u : 1u) & HERMS)) return 1;
+  if constexpr((MMASK >> 4) & 1u)
+    if(rows256_take(a,herm,rs,os))
+      return rows256_launch<T>(a,f,g,rows,rs,out,os,scale,op,st);
+  if(rowreg_take(a,herm)) {
+    if constexpr((HERMS & 2u) != 0)
+      if(herm) return rowreg_dispatch<T,true,MMASK>(a,f,g,rows,rs,out,os,scale,op,st);
+    if constexpr((HERMS & 1u) != 0)
+      if(!herm) return rowreg_dispatch<T,false,MMASK>(a,f,g,rows,rs,out,os,scale,op,st);
+  }
   return 1;
 }
 

@@ -124,8 +124,11 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
         }
         fftr<T,M,V,1,true>(X,xb,tw,t);
         fftr<T,M,V,1,true>(Y,xb,tw,t);
+        {
+          const auto rop=bind_row(op,row);
 #pragma unroll
-        for(int k=0; k < V; ++k) X[k]=op(X[k],Y[k],row,q*(t+TPS*k)+r);
+          for(int k=0; k < V; ++k) X[k]=rop(X[k],Y[k],q*(t+TPS*k)+r);
+        }
         fftr<T,M,V,-1,true>(X,xb,tw,t);
 #pragma unroll
         for(int k=0; k < V; ++k) {
@@ -181,10 +184,13 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
         }
         fftr<T,M,V,1,true>(X,xb,tw,t);
         fftr<T,M,V,1,true>(Y,xb,tw,t);
+        {
+          const auto rop=bind_row(op,row);
 #pragma unroll
-        for(int k=0; k < V; ++k) {
-          const int l=q*(t+TPS*k);
-          X[k]=mk(op(X[k].x,Y[k].x,row,l+r1),r2 >= 0 ? op(X[k].y,Y[k].y,row,l+r2) : (T) 0);
+          for(int k=0; k < V; ++k) {
+            const int l=q*(t+TPS*k);
+            X[k]=mk(rop(X[k].x,Y[k].x,l+r1),r2 >= 0 ? rop(X[k].y,Y[k].y,l+r2) : (T) 0);
+          }
         }
         fftr<T,M,V,-1,true>(X,xb,tw,t);
         // mirror elements Z(M - sigma) through LDS

@@ -26,7 +26,10 @@
  * l = q s + r is the padded index of the row dimension (0 <= l < N).  A
  * functor derived from fftwpp_amd::RowLayout also gets the outer padded grid
  * point of each row: outer(row, x) fills x[0..nouter-1] (the layout of
- * fftwpp_amd_rows_args).  Multipliers on Hermitian or real plans must keep
+ * fftwpp_amd_rows_args).  Work that depends on the row alone belongs in an
+ * optional member `__device__ R at(int64_t row) const`: the kernels then call
+ * at(row) once per row and R's operator()(F, G, l) (complex and real forms)
+ * for every element.  Multipliers on Hermitian or real plans must keep
  * the conjugate symmetry of the data (a product weighted by a function of
  * |k| in every dimension does).  Row geometries outside the register row
  * kernels (power-of-two m in [16, 4096] covering the row) are refused with
@@ -44,6 +47,7 @@ namespace fftwpp_amd {
 /* The outer layout of the rows of a row stage (fftwpp_amd_rows_args). */
 struct RowLayout {
   int nouter=0;
+  int wide=0;                // rows or strides beyond 32 bits
   int oreal[2]={0,0};
   int64_t om[2]={1,1}, oq[2]={1,1}, ocount[2]={1,1}, ostride[2]={1,1};
 
@@ -57,53 +61,80 @@ struct RowLayout {
       ostride[d]=a.ostride[d] ? (int64_t) a.ostride[d] : 1;
       oreal[d]=a.oreal[d];
     }
+    wide=a.rows >= ((size_t) 1 << 32) || ocount[0]*ostride[0] >= ((int64_t) 1 << 32);
   }
 
-  // x[d] = the padded grid point of row `row` in outer dimension d
+  // j -> padded grid point of outer dimension d
+  template<typename I>
+  __host__ __device__ int64_t point(int d, I j) const
+  {
+    const I m=(I) om[d];
+    if(oreal[d]) {
+      // kept planes of real data: residue 0 at s <= m/2, then whole residues
+      if(j <= m/2) return oq[d]*(int64_t) j;
+      const I t=j-(m/2+1);
+      return oq[d]*(int64_t) (t % m)+1+(int64_t) (t/m);
+    }
+    return oq[d]*(int64_t) (j % m)+(int64_t) (j/m);
+  }
+
+  // x[d] = the padded grid point of row `row` in outer dimension d (32-bit
+  // divisions unless the stage is wider)
   __host__ __device__ void outer(int64_t row, int64_t *x) const
   {
-    for(int d=0; d < nouter; ++d) {
-      const int64_t j=(row/ostride[d]) % ocount[d], m=om[d];
-      if(oreal[d]) {
-        // kept planes of real data: residue 0 at s <= m/2, then whole residues
-        int64_t r=0, s=j;
-        if(j > m/2) {
-          const int64_t t=j-(m/2+1);
-          r=1+t/m;
-          s=t % m;
-        }
-        x[d]=oq[d]*s+r;
-      } else
-        x[d]=oq[d]*(j % m)+j/m;
-    }
+#pragma unroll
+    for(int d=0; d < 2; ++d)
+      if(d < nouter) {
+        if(wide)
+          x[d]=point<int64_t>(d,(row/ostride[d]) % ocount[d]);
+        else
+          x[d]=point<uint32_t>(d,((uint32_t) row/(uint32_t) ostride[d]) % (uint32_t) ocount[d]);
+      }
   }
 };
 
+/* What a translation unit compiles for one functor: row sizes (bit i of
+ * sizes = m of 16 << i, ROWS_ALL_SIZES for 16 ... 4096), precisions (bit 0
+ * FP64, bit 1 FP32) and row kinds (bit 0 complex or centred, bit 1
+ * Hermitian).  Every kernel is instantiated per functor, so narrowing these to
+ * the plans a program runs shortens its build; other geometries are refused. */
+template<unsigned SIZES=ROWS_ALL_SIZES, unsigned PRECISIONS=3u, unsigned KINDS=3u>
+struct FusedSet {};
+
 /* Launch the row stage described by `a` with functor op: 0 after the launch,
- * FFTWPP_AMD_ERR_UNSUPPORTED when no register row kernel takes the geometry,
- * FFTWPP_AMD_ERR_HIP on a launch failure. */
-template<class Op>
-inline int fused_rows(const fftwpp_amd_rows_args *a, Op op)
+ * FFTWPP_AMD_ERR_UNSUPPORTED when no compiled register row kernel takes the
+ * geometry, FFTWPP_AMD_ERR_HIP on a launch failure. */
+template<class Op, unsigned SIZES=ROWS_ALL_SIZES, unsigned PRECISIONS=3u, unsigned KINDS=3u>
+inline int fused_rows(const fftwpp_amd_rows_args *a, Op op,
+                      FusedSet<SIZES,PRECISIONS,KINDS> = FusedSet<SIZES,PRECISIONS,KINDS>())
 {
   if constexpr(std::is_base_of<RowLayout,Op>::value) op.setLayout(*a);
   const DimArgs *d=(const DimArgs *) fftwpp_amd_dim_kernel_args(a->dim,a->dtype);
   if(!d) return FFTWPP_AMD_ERR_ARG;
   hipStream_t st=(hipStream_t) a->stream;
   const bool herm=a->hermitian != 0;
-  const int rc=a->dtype == FFTWPP_AMD_F32 ?
-    rows_fused<float>(*d,herm,a->f,a->g,a->rows,a->rowstride,a->out,a->outstride,a->scale,op,st) :
-    rows_fused<double>(*d,herm,a->f,a->g,a->rows,a->rowstride,a->out,a->outstride,a->scale,op,st);
+  int rc=1;
+  if(a->dtype == FFTWPP_AMD_F32) {
+    if constexpr((PRECISIONS & 2u) != 0)
+      rc=rows_fused<float,Op,SIZES,KINDS>(*d,herm,a->f,a->g,a->rows,a->rowstride,a->out,
+                                           a->outstride,a->scale,op,st);
+  } else {
+    if constexpr((PRECISIONS & 1u) != 0)
+      rc=rows_fused<double,Op,SIZES,KINDS>(*d,herm,a->f,a->g,a->rows,a->rowstride,a->out,
+                                            a->outstride,a->scale,op,st);
+  }
   return rc == 0 ? FFTWPP_AMD_OK : (rc == 1 ? FFTWPP_AMD_ERR_UNSUPPORTED : FFTWPP_AMD_ERR_HIP);
 }
 
 /* out <- a (*) b on device pointers with the product op in the row stage
  * (normalised, like fftwpp_amd_conv_execute_out). */
-template<class Op>
+template<class Op, unsigned SIZES=ROWS_ALL_SIZES, unsigned PRECISIONS=3u, unsigned KINDS=3u>
 inline int conv_execute_fused(fftwpp_amd_conv *h, void *out, const void *a, const void *b,
-                              const Op &op, void *stream=nullptr)
+                              const Op &op, void *stream=nullptr,
+                              FusedSet<SIZES,PRECISIONS,KINDS> = FusedSet<SIZES,PRECISIONS,KINDS>())
 {
   fftwpp_amd_rows_hook hook=[](const fftwpp_amd_rows_args *args, void *user) -> int {
-    return fused_rows(args,*static_cast<const Op *>(user));
+    return fused_rows(args,*static_cast<const Op *>(user),FusedSet<SIZES,PRECISIONS,KINDS>());
   };
   return fftwpp_amd_conv_execute_fused(h,out,a,b,hook,const_cast<Op *>(&op),stream);
 }

@@ -197,6 +197,13 @@ def test_fused_unit_weight_is_library_product(gpu, oracle):
     plan.convolve_out(ref, f, g)
     torch.cuda.synchronize()
     assert torch.equal(out, ref)
+    # the plain product as a caller's functor
+    fb = ctypes.CDLL(TESTLIB).fusedmult_binary
+    fb.argtypes = [ctypes.c_void_p] * 5
+    out.zero_()
+    assert fb(plan._h, out.data_ptr(), f.data_ptr(), g.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
 
 
 def test_fused_refusals(gpu, oracle):
