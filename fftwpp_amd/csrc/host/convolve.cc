@@ -917,10 +917,6 @@ static bool rowsSupported(fftBase *fft, int dtype)
     (fft->kind() == FFTWPP_AMD_KIND_COMPLEX && fftwpp_amd_batch_rows_tps(fft->dev,dtype) != 0);
 }
 
-// The fused row stage of dimension fft: the register row kernels when they
-// cover the row, else the long Hermitian path for Hermitian rows of a
-// power-of-two m (residue passes in parallel, partial sums in work slot
-// `slot`), else the LDS row kernel.
 // The row stage handed to a caller's hook (fftwpp_amd_conv_execute_fused):
 // `outer` carries the outer layout of the rows, the rest is filled here.
 static void hookRows(const RowHook *hook, fftBase *fft, int dtype, const void *f,
@@ -964,6 +960,41 @@ static fftwpp_amd_rows_args outerLayout(fftBase *x, size_t nx, bool realx, fftBa
   return a;
 }
 
+// The inner row stage without the fused kernels: rows no row engine takes
+// (complex or centred rows beyond the register, LDS and batched engines) and
+// inner transforms of C > 1 interleaved columns (element stride C).  Every
+// residue of f and g forward into full padded rows, the multiplier on the
+// device arrays, every residue backward into f -- the residue loop of
+// Convolution::convolveRaw (convolve.cc:7513-7575) with all residues at
+// once.  Row i starts at i rs; work slots `slot` and slot + 1.
+static void unfusedRows(fftBase *fft, DeviceWork &work, size_t slot, int dtype, int code,
+                        void *f, void *g, size_t rows, size_t C, size_t rs,
+                        void *stream, const char *what)
+{
+  if(fft->kind() == FFTWPP_AMD_KIND_HERMITIAN)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,std::string(what)+": Hermitian rows beyond the row kernels");
+  if(code == FFTWPP_AMD_MULT_REAL_BINARY)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,std::string(what)+": realMultBinary takes Hermitian rows");
+  const size_t N=fft->m*fft->q, E=eltsize(dtype), q=fft->q;
+  char *Y[2]={(char *) work.get(slot,rows*N*C*E),(char *) work.get(slot+1,rows*N*C*E)};
+  char *X[2]={(char *) f,(char *) g};
+  // single columns: the rows become the columns of one launch; else batches
+  // of at most maxBatch rows (the grid's y extent)
+  const size_t step=C == 1 ? rows : maxBatch;
+  for(int a=0; a < 2; ++a)
+    for(size_t b0=0; b0 < rows; b0 += step)
+      check(fftwpp_amd_forward_many(fft->dev,dtype,X[a]+b0*rs*E,C,C,std::min(step,rows-b0),rs,
+                                    Y[a]+b0*N*C*E,C,N*C,0,q,stream),what);
+  check(fftwpp_amd_multiply(dtype,code,Y[0],Y[1],rows*N*C,stream),what);
+  for(size_t b0=0; b0 < rows; b0 += step)
+    check(fftwpp_amd_backward_many(fft->dev,dtype,Y[0]+b0*N*C*E,C,N*C,X[0]+b0*rs*E,C,C,
+                                   std::min(step,rows-b0),rs,0,q,0,1.0,stream),what);
+}
+
+// The fused row stage of dimension fft: the register row kernels when they
+// cover the row, else the long Hermitian path for Hermitian rows of a
+// power-of-two m (residue passes in parallel, partial sums in work slot
+// `slot`), else the LDS row kernel.
 static void fusedRows(fftBase *fft, DeviceWork &work, size_t slot, int dtype,
                       int code, const void *f, const void *g, size_t rows,
                       size_t rs, void *out, size_t os, double s, void *stream,
@@ -1182,8 +1213,11 @@ Convolution2::Convolution2(fftBase *fftx, fftBase *ffty, Complex **,
   multCode=multiplierCode(mult);
   indices.copy(NULL,1);
   indices.fft=ffty;
-  if(ly != Ly) throw Error(FFTWPP_AMD_ERR_ARG,"fftx->C must equal the stored length of the y transform");
-  if(ffty->C != 1) throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner transform with C > 1");
+  // C > 1: the y rows hold C interleaved sequences (element stride C)
+  if(ffty->C > 1 && ffty->S != ffty->C)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner transform with C > 1 takes interleaved columns (S = C)");
+  if(ly*ffty->C != Ly)
+    throw Error(FFTWPP_AMD_ERR_ARG,"fftx->C must equal the stored length of the y transform times its C");
 }
 
 Convolution2::~Convolution2() {}
@@ -1218,34 +1252,38 @@ void Convolution2::execDevice(void *out, const void *f0, const void *f1, bool no
 void Convolution2::execComplex(void *out, const void *f0, const void *f1, bool normalize,
                                int dtype, void *stream, int code, bool real)
 {
-  if(!rowsSupported(ffty,dtype))
-    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner sub-FFT size beyond the fused row kernels");
+  const size_t Cy=ffty->C, lyc=ly*Cy;   // words of a stored y row
+  const bool fused=Cy == 1 && rowsSupported(ffty,dtype);
+  if(rowHook && !fused)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"a fused multiplier needs y rows of the fused row kernels");
   double s=normalize ? scale : 1.0;
   size_t E=eltsize(dtype);
   size_t qx=fftx->q, mx=fftx->m;
   // x-transformed rows: every residue, or the kept planes of real data
   size_t rows=real ? fftwpp_amd_real_planes(fftx->dev) : qx*mx;
-  void *F0=work.get(0,rows*ly*E);
-  void *F1=work.get(1,rows*ly*E);
+  void *F0=work.get(0,rows*lyc*E);
+  void *F1=work.get(1,rows*lyc*E);
   if(timeline) timeline->begin(stream);
   const void *in[2]={f0,f1};
   void *F[2]={F0,F1};
   for(int a=0; a < 2; ++a) {
     if(real)
-      check(fftwpp_amd_forward_real_many(fftx->dev,dtype,in[a],Sx,ly,1,0,F[a],ly,0,stream),"x forward");
+      check(fftwpp_amd_forward_real_many(fftx->dev,dtype,in[a],Sx,lyc,1,0,F[a],lyc,0,stream),"x forward");
     else
-      check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,ly,1,0,F[a],ly,0,0,qx,stream),"x forward");
+      check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,lyc,1,0,F[a],lyc,0,0,qx,stream),"x forward");
     if(timeline) timeline->mark(real ? (a == 0 ? "x real forward f" : "x real forward g") : (a == 0 ? "x forward f" : "x forward g"));
   }
   if(rowHook)
     hookRows(rowHook,ffty,dtype,F0,F1,rows,ly,F0,ly,1.0,stream,outerLayout(fftx,rows,real),"y rows");
-  else
+  else if(fused)
     fusedRows(ffty,work,2,dtype,code,F0,F1,rows,ly,F0,ly,1.0,stream,"y rows");
-  if(timeline) timeline->mark("y rows");
-  if(real)
-    check(fftwpp_amd_backward_real_many(fftx->dev,dtype,F0,ly,0,out,Sx,ly,1,0,s,stream),"x backward");
   else
-    check(fftwpp_amd_backward_many(fftx->dev,dtype,F0,ly,0,out,Sx,ly,1,0,0,qx,0,s,stream),"x backward");
+    unfusedRows(ffty,work,8,dtype,code,F0,F1,rows,Cy,lyc,stream,"y rows");
+  if(timeline) timeline->mark(fused ? "y rows" : "y rows (unfused)");
+  if(real)
+    check(fftwpp_amd_backward_real_many(fftx->dev,dtype,F0,lyc,0,out,Sx,lyc,1,0,s,stream),"x backward");
+  else
+    check(fftwpp_amd_backward_many(fftx->dev,dtype,F0,lyc,0,out,Sx,lyc,1,0,0,qx,0,s,stream),"x backward");
   if(timeline) timeline->mark(real ? "x real backward" : "x backward");
 }
 
@@ -1281,13 +1319,16 @@ Convolution3::Convolution3(fftBase *fftx, fftBase *ffty, fftBase *fftz,
   multCode=multiplierCode(mult);
   indices.copy(NULL,2);
   indices.fft=fftz;
-  if(lz != Lz) throw Error(FFTWPP_AMD_ERR_ARG,"ffty->C must equal the stored length of the z transform");
+  // C > 1: the z rows hold C interleaved sequences (element stride C)
+  if(fftz->C > 1 && fftz->S != fftz->C)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner transform with C > 1 takes interleaved columns (S = C)");
+  if(lz*fftz->C != Lz)
+    throw Error(FFTWPP_AMD_ERR_ARG,"ffty->C must equal the stored length of the z transform times its C");
   if(Sx < Ly*Sy) throw Error(FFTWPP_AMD_ERR_ARG,"Sx cannot be less than Ly*Sy");
   if(fftx->C != (contiguous() ? Ly*Sy : Lz)) throw Error(FFTWPP_AMD_ERR_ARG,"fftx->C is invalid");
-  if(fftz->C != 1) throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner transform with C > 1");
   // y and z both m = 256, q = 2, complex, rows of lz words back to back: the
   // y-z stage of every x-plane runs as one plane-kernel launch
-  planes=contiguous() && Sx == Ly*Sy && !fftx->realInput() &&
+  planes=contiguous() && Sx == Ly*Sy && !fftx->realInput() && fftz->C == 1 &&
     fftwpp_amd_plane_supported(ffty->dev,fftz->dev);
 }
 
@@ -1297,7 +1338,7 @@ size_t Convolution3::workBytes(int dtype, bool plane) const
 {
   const size_t E=eltsize(dtype);
   const size_t planesX=realPlanes(fftx) ? fftwpp_amd_real_planes(fftx->dev) : fftx->q*fftx->m;
-  const size_t X=planesX*Ly*lz*E;                     // one x-transformed array
+  const size_t X=planesX*Ly*Lz*E;                     // one x-transformed array
   if(plane) return 2*X+fftwpp_amd_plane_work(dtype);
   return 2*X+2*ffty->q*X;
 }
@@ -1315,7 +1356,7 @@ bool Convolution3::usesPlanes(int dtype) const
   return memoryLimit != 0 && workBytes(dtype,false) > memoryLimit;
 }
 
-size_t Convolution3::arraySize() const { return Sx*(Lx-1)+Sy*(Ly-1)+lz; }
+size_t Convolution3::arraySize() const { return Sx*(Lx-1)+Sy*(Ly-1)+Lz; }
 
 void Convolution3::convolveDevice(Complex *f0, const Complex *f1,
                                   bool normalize, void *stream)
@@ -1345,16 +1386,19 @@ void Convolution3::execDevice(void *out, const void *f0, const void *f1, bool no
 void Convolution3::execComplex(void *out, const void *f0, const void *f1, bool normalize,
                                int dtype, void *stream, int code, bool real)
 {
-  if(!rowsSupported(fftz,dtype))
-    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"inner sub-FFT size beyond the fused row kernels");
+  // z rows of Lz = lz C words (C interleaved sequences when C > 1)
+  const size_t Cz=fftz->C;
+  const bool fused=Cz == 1 && rowsSupported(fftz,dtype);
+  if(rowHook && !fused)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"a fused multiplier needs z rows of the fused row kernels");
   double s=normalize ? scale : 1.0;
   size_t E=eltsize(dtype);
   size_t qx=fftx->q, mx=fftx->m, qy=ffty->q, my=ffty->m;
-  // x-transformed planes of Ly x lz: every residue, or the kept planes of
+  // x-transformed planes of Ly x Lz: every residue, or the kept planes of
   // real data
   size_t nplanes=real ? fftwpp_amd_real_planes(fftx->dev) : qx*mx;
-  size_t plane=Ly*lz;
-  size_t yplane=qy*my*lz;        // y-transformed plane
+  size_t plane=Ly*Lz;
+  size_t yplane=qy*my*Lz;        // y-transformed plane
   void *X0=work.get(0,nplanes*plane*E);
   void *X1=work.get(1,nplanes*plane*E);
   const void *in[2]={f0,f1};
@@ -1370,7 +1414,7 @@ void Convolution3::execComplex(void *out, const void *f0, const void *f1, bool n
       if(contiguous())
         check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,plane,1,0,X[a],plane,0,0,qx,stream),"x forward");
       else
-        check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,lz,Ly,Sy,X[a],plane,lz,0,qx,stream),"x forward");
+        check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,Lz,Ly,Sy,X[a],plane,Lz,0,qx,stream),"x forward");
       if(timeline) timeline->mark(a == 0 ? "x forward f" : "x forward g");
     }
     check(fftwpp_amd_plane_conv(ffty->dev,fftz->dev,dtype,code,X0,X1,X0,nplanes,Ly,lz,P,stream),
@@ -1390,31 +1434,33 @@ void Convolution3::execComplex(void *out, const void *f0, const void *f1, bool n
     if(real && contiguous())
       check(fftwpp_amd_forward_real_many(fftx->dev,dtype,in[a],Sx,plane,1,0,X[a],plane,0,stream),"x forward");
     else if(real)
-      check(fftwpp_amd_forward_real_many(fftx->dev,dtype,in[a],Sx,lz,Ly,Sy,X[a],plane,lz,stream),"x forward");
+      check(fftwpp_amd_forward_real_many(fftx->dev,dtype,in[a],Sx,Lz,Ly,Sy,X[a],plane,Lz,stream),"x forward");
     else if(contiguous())
       check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,plane,1,0,X[a],plane,0,0,qx,stream),"x forward");
     else
-      check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,lz,Ly,Sy,X[a],plane,lz,0,qx,stream),"x forward");
+      check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,Lz,Ly,Sy,X[a],plane,Lz,0,qx,stream),"x forward");
     if(timeline) timeline->mark(real ? (a == 0 ? "x real forward f" : "x real forward g") : (a == 0 ? "x forward f" : "x forward g"));
-    check(fftwpp_amd_forward_many(ffty->dev,dtype,X[a],lz,lz,nplanes,plane,Y[a],lz,yplane,0,qy,stream),"y forward");
+    check(fftwpp_amd_forward_many(ffty->dev,dtype,X[a],Lz,Lz,nplanes,plane,Y[a],Lz,yplane,0,qy,stream),"y forward");
     if(timeline) timeline->mark(a == 0 ? "y forward f" : "y forward g");
   }
   if(rowHook)
     hookRows(rowHook,fftz,dtype,Y0,Y1,nplanes*qy*my,lz,Y0,lz,1.0,stream,
              outerLayout(fftx,nplanes,real,ffty),"z rows");
-  else
+  else if(fused)
     fusedRows(fftz,work,4,dtype,code,Y0,Y1,nplanes*qy*my,lz,Y0,lz,1.0,stream,"z rows");
-  if(timeline) timeline->mark("z rows");
-  check(fftwpp_amd_backward_many(ffty->dev,dtype,Y0,lz,yplane,X0,lz,lz,nplanes,plane,0,qy,0,1.0,stream),"y backward");
+  else
+    unfusedRows(fftz,work,8,dtype,code,Y0,Y1,nplanes*qy*my,Cz,Lz,stream,"z rows");
+  if(timeline) timeline->mark(fused ? "z rows" : "z rows (unfused)");
+  check(fftwpp_amd_backward_many(ffty->dev,dtype,Y0,Lz,yplane,X0,Lz,Lz,nplanes,plane,0,qy,0,1.0,stream),"y backward");
   if(timeline) timeline->mark("y backward");
   if(real && contiguous())
     check(fftwpp_amd_backward_real_many(fftx->dev,dtype,X0,plane,0,out,Sx,plane,1,0,s,stream),"x backward");
   else if(real)
-    check(fftwpp_amd_backward_real_many(fftx->dev,dtype,X0,plane,lz,out,Sx,lz,Ly,Sy,s,stream),"x backward");
+    check(fftwpp_amd_backward_real_many(fftx->dev,dtype,X0,plane,Lz,out,Sx,Lz,Ly,Sy,s,stream),"x backward");
   else if(contiguous())
     check(fftwpp_amd_backward_many(fftx->dev,dtype,X0,plane,0,out,Sx,plane,1,0,0,qx,0,s,stream),"x backward");
   else
-    check(fftwpp_amd_backward_many(fftx->dev,dtype,X0,plane,lz,out,Sx,lz,Ly,Sy,0,qx,0,s,stream),"x backward");
+    check(fftwpp_amd_backward_many(fftx->dev,dtype,X0,plane,Lz,out,Sx,Lz,Ly,Sy,0,qx,0,s,stream),"x backward");
   if(timeline) timeline->mark(real ? "x real backward" : "x backward");
 }
 

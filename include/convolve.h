@@ -489,7 +489,7 @@ public:
     return (double) fftx->normalization()*(double) ffty->normalization()*
       (double) fftz->normalization();
   }
-  bool contiguous() { return Sy == lz; }
+  bool contiguous() const { return Sy == Lz; }
 
   void convolveRaw(Complex **f, size_t offset=0, Indices * =NULL) { run(f,offset,false); }
   void convolveRaw(double **f, size_t offset=0, Indices * =NULL) { run((Complex **) f,offset,false); }

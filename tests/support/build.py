@@ -1,6 +1,7 @@
 """Build tests/support/libtestmult.so: the user device multipliers of
 multipliers.hip, the fused multipliers of fusedmult.hip (compiled into the
-library's row kernels), the C++ FFT face of fftface.cc and the reference transform
+library's row kernels), the C++ FFT face of fftface.cc, the inner-column and
+unfused-row convolutions of innerc.cc and the reference transform
 tests of residues.cc, linked against the in-tree libfftwpp_amd.so (found next to
 the package at run time through $ORIGIN)."""
 import os
@@ -21,6 +22,7 @@ def build(verbose=False):
                                      os.path.join(HERE, "fusedmult.hip"),
                                      os.path.join(HERE, "fftface.cc"),
                                      os.path.join(HERE, "residues.cc"),
+                                     os.path.join(HERE, "innerc.cc"),
                                      "-L", pkg, "-lfftwpp_amd",
                                      "-Wl,-rpath,$ORIGIN/../../fftwpp_amd",
                                      "-o", LIB + ".tmp"]
