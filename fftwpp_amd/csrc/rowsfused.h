@@ -115,6 +115,25 @@ inline int rowreg_launch_rp(const DimArgs &a, const void *f, const void *g, size
   return fused_status(hipGetLastError());
 }
 
+// Hermitian rows of M = 512 ... 2048 on the packed kernel (f + i g in one
+// transform per residue), persistent grid.
+template<typename T, int M, class Op>
+inline int hrowpk_launch(const DimArgs &a, const void *f, const void *g, size_t rows,
+                         size_t rs, void *out, size_t os, double scale, const Op &op,
+                         hipStream_t st)
+{
+  typedef HRowPk<T,M> K;
+  const size_t lds=hrowpk_lds<T,M>((int) a.q);
+  const hipError_t e=fused_allow_lds(hrowpk_kernel<T,M,Op>,lds);
+  if(e != hipSuccess) return fused_status(e);
+  size_t nb=(rows+K::SLOTS-1)/K::SLOTS;
+  nb=std::min(nb,fused_resident(hrowpk_kernel<T,M,Op>,K::NT,lds));
+  hipLaunchKernelGGL((hrowpk_kernel<T,M,Op>),dim3((unsigned) nb),dim3(K::NT),lds,st,a,
+                     (const cplx<T> *) f,(const cplx<T> *) g,(int64_t) rows,(int64_t) rs,
+                     (cplx<T> *) out,(int64_t) os,(T) scale,op);
+  return fused_status(hipGetLastError());
+}
+
 template<typename T, int M, bool HERM, class Op>
 inline int rowreg_launch(const DimArgs &a, const void *f, const void *g, size_t rows,
                          size_t rs, void *out, size_t os, double scale, const Op &op,
@@ -122,6 +141,10 @@ inline int rowreg_launch(const DimArgs &a, const void *f, const void *g, size_t 
 {
   constexpr int TPS=M/RowV::V;
   const int64_t passes=HERM ? (a.q+1)/2 : a.q;
+  // Hermitian rows of 512 ... 2048 points with enough rows to fill the chip:
+  // the packed kernel (measured 12-28% faster than rowreg_kernel there)
+  if constexpr(HERM && M >= 512 && M <= 2048)
+    if(rows >= 512) return hrowpk_launch<T,M>(a,f,g,rows,rs,out,os,scale,op,st);
   if(TPS <= 256 && rows < 512 && passes >= 2)
     return rowreg_launch_rp<T,M,HERM,2>(a,f,g,rows,rs,out,os,scale,op,st);
   return rowreg_launch_rp<T,M,HERM,1>(a,f,g,rows,rs,out,os,scale,op,st);

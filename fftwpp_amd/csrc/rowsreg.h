@@ -237,6 +237,152 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
 }
 
 
+// Residue r of the f + i g pair of a Hermitian row (H <= M), folded for the
+// forward transform, X[k] = U_r(sigma) at sigma = t + TPS k:
+//   U_r(sigma) = w_N^{sigma r} [u(sigma) + w_q^{-r} v(sigma)],
+//   u(sigma) = f(sigma) + i g(sigma)                    (sigma < H),
+//   v(sigma) = conj f(M - sigma) + i conj g(M - sigma)  (M - sigma < H),
+// the wavenumbers k = sigma and sigma - M of the block (f(-k) = conj f(k),
+// w_N^{(sigma - M) r} = w_N^{sigma r} w_q^{-r}).  rf, rg: buffers over the
+// row's H words; tr[k] = w_N^{TPS k r}, tr[V] = w_N^{M r} (LDS); b = w_N^{t r}.
+// Out-of-range words are read at index 0 and then zeroed (no branches).
+template<typename T, int M>
+__device__ __forceinline__ void hpk_fold(cplx<T> *X, __amdgpu_buffer_rsrc_t rf,
+                                         __amdgpu_buffer_rsrc_t rg, const cplx<T> *tr,
+                                         cplx<T> b, int t, int r, int H)
+{
+  constexpr int V=RowV::V, TPS=M/V;
+  constexpr unsigned E=sizeof(cplx<T>);
+  const cplx<T> c=conjg(tr[V]);
+  const cplx<T> zero=mk((T) 0,(T) 0);
+#pragma unroll
+  for(int k=0; k < V; ++k) {
+    const int sigma=t+TPS*k;
+    const bool iu=sigma < H, iv=M-sigma < H;
+    const unsigned ou=(unsigned) (iu ? sigma : 0)*E, ov=(unsigned) (iv ? M-sigma : 0)*E;
+    const cplx<T> fu=ld_buf<T>(rf,ou,0), gu=ld_buf<T>(rg,ou,0);
+    const cplx<T> fv=ld_buf<T>(rf,ov,0), gv=ld_buf<T>(rg,ov,0);
+    const cplx<T> u=iu ? mk(fu.x-gu.y,fu.y+gu.x) : zero;
+    const cplx<T> v=iv ? mk(fv.x+gv.y,gv.x-fv.y) : zero;
+    cplx<T> a=u+c*v;
+    if(r) a=(b*tr[k])*a;
+    X[k]=a;
+  }
+}
+
+// Hermitian rows with H <= M and TPS = M/8 >= 64 (M >= 512), SLOTS = 256/TPS
+// rows per block of 256 threads (M = 1024: two rows of two waves).  The
+// forward transforms of f and g share one complex FFT per residue: the fold
+// U_r of f + i g (hpk_fold) transforms to F_r + i G_r with F_r, G_r real
+// (the folds of Hermitian data are Hermitian), so P_r = F_r G_r is
+// realMultBinary's product (fftPadHermitian::forward2 and the multiplier,
+// convolve.cc:4517-4609, 26-110).  Residues are paired for the inverse,
+// P_{r1} + i P_{r2}, and unpacked with the mirror element Z(M - sigma) as in
+// rowreg_kernel (backward2, convolve.cc:4749-4843): q forward and ceil(q/2)
+// inverse transforms per row instead of 3 ceil(q/2).  Rows are read through
+// buffer resources (a 32-bit offset per element) and the uniform residue
+// twiddles w_N^{TPS k r} come from an LDS table, which keeps addresses and
+// twiddles out of the registers that the residue loop carries.  Slots past
+// the last row run on row rows - 1 without storing (the block barriers of the
+// transforms need every thread).  Dynamic LDS: hrowpk_lds().
+template<typename T, int M>
+struct HRowPk {
+  static constexpr int V=RowV::V;
+  static constexpr int TPS=M/V;
+  static constexpr int SLOTS=256/TPS > 0 ? 256/TPS : 1;
+  static constexpr int NT=SLOTS*TPS;
+};
+
+template<typename T, int M>
+inline size_t hrowpk_lds(int q)
+{
+  typedef HRowPk<T,M> K;
+  return (size_t) (M+q*(K::V+1))*sizeof(cplx<T>)+(size_t) K::SLOTS*2*RPlan<M,K::V>::PAD*sizeof(T);
+}
+
+template<typename T, int M, class Op>
+__global__ void __launch_bounds__((HRowPk<T,M>::NT))
+hrowpk_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows, int64_t rs,
+              cplx<T> *out, int64_t os, T scale, Op op)
+{
+  typedef HRowPk<T,M> K;
+  constexpr int V=K::V, TPS=K::TPS, SLOTS=K::SLOTS;
+  static_assert(TPS >= 64, "a slot spans whole waves");
+  typedef RPlan<M,V> P;
+  constexpr unsigned E=sizeof(cplx<T>);
+  extern __shared__ char smem_raw[];
+  const int q=(int) d.q, H=(int) d.H;
+  cplx<T> *tw=reinterpret_cast<cplx<T> *>(smem_raw);
+  cplx<T> *tq=tw+M;                           // tq[r (V + 1) + k] = w_N^{TPS k r}
+  const cplx<T> *zN=(const cplx<T> *) d.zN;
+  fill_twr<T,M,V>(tw,zN,q,threadIdx.x,blockDim.x);
+  for(int e=threadIdx.x; e < q*(V+1); e += blockDim.x) {
+    const int r=e/(V+1), k=e-r*(V+1);
+    tq[e]=zN[TPS*k*r];                        // TPS k r <= M r < N
+  }
+  __syncthreads();
+  // a slot spans whole waves: its index is uniform over the wave
+  const int slot=__builtin_amdgcn_readfirstlane((int) threadIdx.x)/TPS;
+  const int t=threadIdx.x-slot*TPS;
+  T *xb=reinterpret_cast<T *>(tq+q*(V+1))+slot*2*P::PAD;
+  for(int64_t rb=(int64_t) blockIdx.x*SLOTS; rb < rows; rb += (int64_t) gridDim.x*SLOTS) {
+    const int64_t row=rb+slot;
+    const bool live=row < rows;
+    const int64_t rr=live ? row : rows-1;
+    const __amdgpu_buffer_rsrc_t rf=buf_rsrc(f+rr*rs,H), rg=buf_rsrc(g+rr*rs,H);
+    cplx<T> acc[V];
+#pragma unroll
+    for(int k=0; k < V; ++k) acc[k]=mk((T) 0,(T) 0);
+    const auto rop=bind_row(op,row);
+    for(int r1=0; r1 < q; r1 += 2) {
+      const int r2=r1+1 < q ? r1+1 : -1;
+      const cplx<T> b1=zN[t*r1];
+      cplx<T> X[V];
+      T p1[V];
+      hpk_fold<T,M>(X,rf,rg,tq+r1*(V+1),b1,t,r1,H);
+      fftr<T,M,V,1,true>(X,xb,tw,t);
+#pragma unroll
+      for(int k=0; k < V; ++k) p1[k]=rop(X[k].x,X[k].y,q*(t+TPS*k)+r1);
+      if(r2 >= 0) {
+        hpk_fold<T,M>(X,rf,rg,tq+r2*(V+1),zN[t*r2],t,r2,H);
+        fftr<T,M,V,1,true>(X,xb,tw,t);
+#pragma unroll
+        for(int k=0; k < V; ++k) X[k]=mk(p1[k],rop(X[k].x,X[k].y,q*(t+TPS*k)+r2));
+      } else {
+#pragma unroll
+        for(int k=0; k < V; ++k) X[k]=mk(p1[k],(T) 0);
+      }
+      fftr<T,M,V,-1,true>(X,xb,tw,t);
+      // mirror elements Z(M - sigma) through LDS
+#pragma unroll
+      for(int k=0; k < V; ++k) {
+        const int e=xidx<T,M,V>(t+TPS*k);
+        xb[e]=X[k].x;
+        xb[P::PAD+e]=X[k].y;
+      }
+      rsync<TPS>();
+      const cplx<T> *t1=tq+r1*(V+1), *t2=tq+(r2 >= 0 ? r2 : 0)*(V+1);
+      const cplx<T> b2=r2 >= 0 ? zN[t*r2] : mk((T) 0,(T) 0);
+#pragma unroll
+      for(int k=0; k < V; ++k) {
+        const int e=xidx<T,M,V>((M-(t+TPS*k)) & (M-1));
+        const cplx<T> z=X[k], zc=mk(xb[e],-xb[P::PAD+e]);
+        cplx<T> v=mulc((T) 0.5*(z+zc),b1*t1[k]);
+        if(r2 >= 0) v=v+mulc(muli((T) 0.5*(z-zc),-1),b2*t2[k]);
+        acc[k]=acc[k]+v;
+      }
+      rsync<TPS>();
+    }
+    if(live) {
+      const __amdgpu_buffer_rsrc_t ro=buf_rsrc(out+row*os,H);
+#pragma unroll
+      for(int k=0; k < V; ++k)
+        if(t+TPS*k < H) st_buf<T>(ro,(unsigned) (t+TPS*k)*E,0,scale*acc[k]);
+    }
+  }
+}
+
+
 // Hermitian rows longer than one block (H > M): the residue-pair passes of a
 // row (fftPadHermitian's D = 2 pairs, convolve.cc:4517-5012) run as
 // independent work units (row, pass), each on one slot of TPS threads.  A
