@@ -809,6 +809,23 @@ static size_t resident_blocks(K kernel, int threads, size_t lds)
   return n;
 }
 
+// Resident blocks per CU of `kernel` at `threads` threads and `lds` bytes of
+// dynamic LDS (0 on failure), cached per (kernel, threads, lds).
+template<typename K>
+static int blocks_per_cu(K kernel, int threads, size_t lds)
+{
+  static std::mutex mu;
+  static std::vector<std::pair<std::pair<const void *,size_t>,int>> known;
+  const std::pair<const void *,size_t> key((const void *) kernel,lds*4096+(size_t) threads);
+  std::lock_guard<std::mutex> lock(mu);
+  for(const auto &e : known) if(e.first == key) return e.second;
+  int per=0;
+  if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per,(const void *) kernel,threads,lds) != hipSuccess)
+    per=0;
+  known.push_back({key,per});
+  return per;
+}
+
 // The register-resident m=256 kernels apply to non-centred complex
 // dimensions with p = 1.
 static bool fast256(const fftwpp_amd_dim *d, size_t s1=0, size_t s2=0)
@@ -840,38 +857,47 @@ static int colreg_launch(const DimArgs &a, bool fwd, const void *in,
                          size_t ics, size_t ocs, hipStream_t st)
 {
   typedef ColReg<T,M> CR;
-  const size_t lds=M*sizeof(cplx<T>)+CR::TC*CR::CS*sizeof(T);
+  size_t lds=M*sizeof(cplx<T>)+CR::TC*CR::CS*sizeof(T);
   const dim3 grid((unsigned) ((C+CR::TC-1)/CR::TC),(unsigned) nbatch);
   const bool p2=a.L > M;
-  int rc;
+  // the N residue twiddles in LDS when they fit without lowering occupancy
+  auto zfit=[&](auto kernel) {
+    const size_t z=lds+(size_t) a.N*sizeof(cplx<T>);
+    return z <= lds_max() && blocks_per_cu(kernel,CR::NT,z) == blocks_per_cu(kernel,CR::NT,lds);
+  };
+  int rc, zl=0;
   if(fwd) {
     if(p2) {
-      if((rc=allow_lds(colfwd_reg_kernel<T,M,2>,lds))) return rc;
+      if((rc=allow_lds(colfwd_reg_kernel<T,M,2>,lds_max()))) return rc;
+      if((zl=zfit(colfwd_reg_kernel<T,M,2>))) lds += (size_t) a.N*sizeof(cplx<T>);
       hipLaunchKernelGGL((colfwd_reg_kernel<T,M,2>),grid,dim3(CR::NT),lds,st,a,
                          (const cplx<T> *) in,(int64_t) s_in,(int64_t) C,(int64_t) bsi,
                          (cplx<T> *) out,(int64_t) s_out,(int64_t) bso,(int64_t) r0,(int64_t) nres,
-                         (int64_t) ics,(int64_t) ocs);
+                         (int64_t) ics,(int64_t) ocs,zl);
     } else {
-      if((rc=allow_lds(colfwd_reg_kernel<T,M,1>,lds))) return rc;
+      if((rc=allow_lds(colfwd_reg_kernel<T,M,1>,lds_max()))) return rc;
+      if((zl=zfit(colfwd_reg_kernel<T,M,1>))) lds += (size_t) a.N*sizeof(cplx<T>);
       hipLaunchKernelGGL((colfwd_reg_kernel<T,M,1>),grid,dim3(CR::NT),lds,st,a,
                          (const cplx<T> *) in,(int64_t) s_in,(int64_t) C,(int64_t) bsi,
                          (cplx<T> *) out,(int64_t) s_out,(int64_t) bso,(int64_t) r0,(int64_t) nres,
-                         (int64_t) ics,(int64_t) ocs);
+                         (int64_t) ics,(int64_t) ocs,zl);
     }
     return launch_check("colfwd_reg_kernel");
   }
   if(p2) {
-    if((rc=allow_lds(colbwd_reg_kernel<T,M,2>,lds))) return rc;
+    if((rc=allow_lds(colbwd_reg_kernel<T,M,2>,lds_max()))) return rc;
+    if((zl=zfit(colbwd_reg_kernel<T,M,2>))) lds += (size_t) a.N*sizeof(cplx<T>);
     hipLaunchKernelGGL((colbwd_reg_kernel<T,M,2>),grid,dim3(CR::NT),lds,st,a,
                        (const cplx<T> *) in,(int64_t) s_in,(int64_t) bsi,(cplx<T> *) out,
                        (int64_t) s_out,(int64_t) C,(int64_t) bso,(int64_t) r0,(int64_t) nres,
-                       accumulate,(T) scale,(int64_t) ics,(int64_t) ocs);
+                       accumulate,(T) scale,(int64_t) ics,(int64_t) ocs,zl);
   } else {
-    if((rc=allow_lds(colbwd_reg_kernel<T,M,1>,lds))) return rc;
+    if((rc=allow_lds(colbwd_reg_kernel<T,M,1>,lds_max()))) return rc;
+    if((zl=zfit(colbwd_reg_kernel<T,M,1>))) lds += (size_t) a.N*sizeof(cplx<T>);
     hipLaunchKernelGGL((colbwd_reg_kernel<T,M,1>),grid,dim3(CR::NT),lds,st,a,
                        (const cplx<T> *) in,(int64_t) s_in,(int64_t) bsi,(cplx<T> *) out,
                        (int64_t) s_out,(int64_t) C,(int64_t) bso,(int64_t) r0,(int64_t) nres,
-                       accumulate,(T) scale,(int64_t) ics,(int64_t) ocs);
+                       accumulate,(T) scale,(int64_t) ics,(int64_t) ocs,zl);
   }
   return launch_check("colbwd_reg_kernel");
 }

@@ -547,6 +547,24 @@ template<typename T, int M> struct ColReg {
   static __device__ __forceinline__ int xm(int c) { return (c*BETA) & (M < 32 ? M-1 : 31); }
 };
 
+// The stage twiddle table of the column kernels (tw) and the residue twiddle
+// table they read: zlds = 1 copies the dimension's N twiddles into LDS after
+// the exchange areas (the residue twiddles are per-lane gathers, about 10%
+// faster from LDS; the host asks for it when the LDS fits without lowering
+// occupancy), else the table in global memory.  Ends with a block barrier.
+template<typename T, int M>
+__device__ __forceinline__ const cplx<T> *col_twiddles(const DimArgs &d, cplx<T> *tw, int zlds)
+{
+  typedef ColReg<T,M> CR;
+  const cplx<T> *zN=(const cplx<T> *) d.zN;
+  fill_twr<T,M,CR::V>(tw,zN,(int) d.q,threadIdx.x,CR::NT);
+  cplx<T> *zl=reinterpret_cast<cplx<T> *>(reinterpret_cast<T *>(tw+M)+CR::TC*CR::CS);
+  if(zlds)
+    for(int e=threadIdx.x; e < (int) d.N; e += CR::NT) zl[e]=zN[e];
+  __syncthreads();
+  return zlds ? zl : zN;
+}
+
 // F[b][(r-r0) M + s][c] = sum_j f[b][j][c] w_N^{(j+k0)(q s + r)}, j < L;
 // column c starts at c*ics of f and c*ocs of F (unit strides for
 // interleaved columns, row strides for contiguous rows taken as columns).
@@ -554,15 +572,13 @@ template<typename T, int M, int P2>
 __global__ void __launch_bounds__(512)
 colfwd_reg_kernel(DimArgs d, const cplx<T> *f, int64_t S, int64_t C,
                   int64_t bsi, cplx<T> *F, int64_t SF, int64_t bso, int64_t r0,
-                  int64_t nres, int64_t ics, int64_t ocs)
+                  int64_t nres, int64_t ics, int64_t ocs, int zlds)
 {
   typedef ColReg<T,M> CR;
   constexpr int V=CR::V, TPS=CR::TPS, TC=CR::TC;
   extern __shared__ char smem_raw[];
   cplx<T> *tw=reinterpret_cast<cplx<T> *>(smem_raw);
-  const cplx<T> *zN=(const cplx<T> *) d.zN;
-  fill_twr<T,M,V>(tw,zN,(int) d.q,threadIdx.x,CR::NT);
-  __syncthreads();
+  const cplx<T> *zN=col_twiddles<T,M>(d,tw,zlds);
   // interleaved columns: consecutive lanes on consecutive columns; rows
   // (column stride ics > 1): a row's threads on consecutive lanes
   const bool rowmap=ics != 1;
@@ -613,15 +629,14 @@ template<typename T, int M, int P2>
 __global__ void __launch_bounds__(512)
 colbwd_reg_kernel(DimArgs d, const cplx<T> *F, int64_t SF, int64_t bsi,
                   cplx<T> *f, int64_t S, int64_t C, int64_t bso, int64_t r0,
-                  int64_t nres, int accumulate, T scale, int64_t ics, int64_t ocs)
+                  int64_t nres, int accumulate, T scale, int64_t ics, int64_t ocs,
+                  int zlds)
 {
   typedef ColReg<T,M> CR;
   constexpr int V=CR::V, TPS=CR::TPS, TC=CR::TC;
   extern __shared__ char smem_raw[];
   cplx<T> *tw=reinterpret_cast<cplx<T> *>(smem_raw);
-  const cplx<T> *zN=(const cplx<T> *) d.zN;
-  fill_twr<T,M,V>(tw,zN,(int) d.q,threadIdx.x,CR::NT);
-  __syncthreads();
+  const cplx<T> *zN=col_twiddles<T,M>(d,tw,zlds);
   // interleaved columns: consecutive lanes on consecutive columns; rows
   // (column stride ics > 1): a row's threads on consecutive lanes
   const bool rowmap=ics != 1;
