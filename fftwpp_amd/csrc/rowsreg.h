@@ -237,8 +237,9 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
 }
 
 
-// Residue r of the f + i g pair of a Hermitian row (H <= M), folded for the
-// forward transform, X[k] = U_r(sigma) at sigma = t + TPS k:
+// Residues r1 and r2 of the f + i g pair of a Hermitian row (H <= M), folded
+// for the forward transform from one read of the row: X1[k] = U_r1(sigma),
+// X2[k] = U_r2(sigma) at sigma = t + TPS k, with
 //   U_r(sigma) = w_N^{sigma r} [u(sigma) + w_q^{-r} v(sigma)],
 //   u(sigma) = f(sigma) + i g(sigma)                    (sigma < H),
 //   v(sigma) = conj f(M - sigma) + i conj g(M - sigma)  (M - sigma < H),
@@ -247,13 +248,14 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
 // row's H words; tr[k] = w_N^{TPS k r}, tr[V] = w_N^{M r} (LDS); b = w_N^{t r}.
 // Out-of-range words are read at index 0 and then zeroed (no branches).
 template<typename T, int M>
-__device__ __forceinline__ void hpk_fold(cplx<T> *X, __amdgpu_buffer_rsrc_t rf,
-                                         __amdgpu_buffer_rsrc_t rg, const cplx<T> *tr,
-                                         cplx<T> b, int t, int r, int H)
+__device__ __forceinline__ void hpk_fold2(cplx<T> *X1, cplx<T> *X2, __amdgpu_buffer_rsrc_t rf,
+                                          __amdgpu_buffer_rsrc_t rg, const cplx<T> *tr1,
+                                          const cplx<T> *tr2, cplx<T> b1, cplx<T> b2, int t,
+                                          int r1, int r2, int H)
 {
   constexpr int V=RowV::V, TPS=M/V;
   constexpr unsigned E=sizeof(cplx<T>);
-  const cplx<T> c=conjg(tr[V]);
+  const cplx<T> c1=conjg(tr1[V]), c2=conjg(tr2[V]);
   const cplx<T> zero=mk((T) 0,(T) 0);
 #pragma unroll
   for(int k=0; k < V; ++k) {
@@ -264,16 +266,18 @@ __device__ __forceinline__ void hpk_fold(cplx<T> *X, __amdgpu_buffer_rsrc_t rf,
     const cplx<T> fv=ld_buf<T>(rf,ov,0), gv=ld_buf<T>(rg,ov,0);
     const cplx<T> u=iu ? mk(fu.x-gu.y,fu.y+gu.x) : zero;
     const cplx<T> v=iv ? mk(fv.x+gv.y,gv.x-fv.y) : zero;
-    cplx<T> a=u+c*v;
-    if(r) a=(b*tr[k])*a;
-    X[k]=a;
+    cplx<T> a=u+c1*v, b=u+c2*v;
+    if(r1) a=(b1*tr1[k])*a;
+    if(r2) b=(b2*tr2[k])*b;
+    X1[k]=a;
+    X2[k]=b;
   }
 }
 
 // Hermitian rows with H <= M and TPS = M/8 >= 64 (M >= 512), SLOTS = 256/TPS
 // rows per block of 256 threads (M = 1024: two rows of two waves).  The
 // forward transforms of f and g share one complex FFT per residue: the fold
-// U_r of f + i g (hpk_fold) transforms to F_r + i G_r with F_r, G_r real
+// U_r of f + i g (hpk_fold2) transforms to F_r + i G_r with F_r, G_r real
 // (the folds of Hermitian data are Hermitian), so P_r = F_r G_r is
 // realMultBinary's product (fftPadHermitian::forward2 and the multiplier,
 // convolve.cc:4517-4609, 26-110).  Residues are paired for the inverse,
@@ -337,14 +341,17 @@ hrowpk_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows, int64
     for(int r1=0; r1 < q; r1 += 2) {
       const int r2=r1+1 < q ? r1+1 : -1;
       const cplx<T> b1=zN[t*r1];
-      cplx<T> X[V];
+      // both residues folded from one read of the row
+      cplx<T> X[V], X2[V];
       T p1[V];
-      hpk_fold<T,M>(X,rf,rg,tq+r1*(V+1),b1,t,r1,H);
+      hpk_fold2<T,M>(X,X2,rf,rg,tq+r1*(V+1),tq+(r2 >= 0 ? r2 : r1)*(V+1),b1,
+                     r2 >= 0 ? zN[t*r2] : b1,t,r1,r2 >= 0 ? r2 : r1,H);
       fftr<T,M,V,1,true>(X,xb,tw,t);
 #pragma unroll
       for(int k=0; k < V; ++k) p1[k]=rop(X[k].x,X[k].y,q*(t+TPS*k)+r1);
       if(r2 >= 0) {
-        hpk_fold<T,M>(X,rf,rg,tq+r2*(V+1),zN[t*r2],t,r2,H);
+#pragma unroll
+        for(int k=0; k < V; ++k) X[k]=X2[k];
         fftr<T,M,V,1,true>(X,xb,tw,t);
 #pragma unroll
         for(int k=0; k < V; ++k) X[k]=mk(p1[k],rop(X[k].x,X[k].y,q*(t+TPS*k)+r2));
