@@ -140,7 +140,7 @@ template<typename T> struct BatchV { static constexpr int V=sizeof(T) == 8 ? 8 :
 // When TPS > 64 the host launches one slot per block, so the block barriers
 // inside fftr() see the same calls from every thread.  Dynamic LDS: M
 // complex twiddles + slots * 2 RPlan<M,V>::PAD scalar words (one-round
-// exchanges).
+// exchanges) + q V complex residue twiddles.
 //
 // D = 0 selects the sequence mode: a work unit is one residue r of an item
 // and writes its inverse-transformed, conjugate-twiddled sequence
@@ -163,8 +163,15 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
   cplx<T> *tw=reinterpret_cast<cplx<T> *>(smem_raw);
   const cplx<T> *zN=(const cplx<T> *) d.zN;
   fill_twr<T,M,V>(tw,zN,(int) d.q,threadIdx.x,blockDim.x);
-  __syncthreads();
   const int slots=blockDim.x/TPS;
+  // residue twiddles w_N^{(t + TPS k) r} = w_N^{t r} w_N^{TPS k r}: the
+  // uniform factor from an LDS table tq[r V + k] after the exchange areas
+  cplx<T> *tq=reinterpret_cast<cplx<T> *>(reinterpret_cast<T *>(tw+M)+slots*2*P::PAD);
+  for(int e=threadIdx.x; e < (int) d.q*V; e += blockDim.x) {
+    const int r=e/V, k=e-r*V;
+    tq[e]=zN[TPS*k*r];                   // TPS k r < M r < N
+  }
+  __syncthreads();
   const int slot=threadIdx.x/TPS, t=threadIdx.x-slot*TPS;
   T *xb=reinterpret_cast<T *>(tw+M)+slot*2*P::PAD;
   constexpr bool zm=ZM;                  // the sequence mode (the host passes D = 0)
@@ -217,10 +224,12 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
         eb += r;
         if(eb >= qi) eb -= qi;
       }
+      const cplx<T> bt=zN[t*r];
+      const cplx<T> *tr=tq+r*V;
       if(r) {
 #pragma unroll
         for(int k=0; k < V; ++k) {
-          const cplx<T> w=zN[(t+TPS*k)*r];      // sigma r < M q = N
+          const cplx<T> w=bt*tr[k];             // w_N^{sigma r}
           X[k]=w*X[k];
           Y[k]=w*Y[k];
         }
@@ -233,7 +242,7 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
       fftr<T,M,V,-1,true>(X,xb,tw,t);
       if(r) {
 #pragma unroll
-        for(int k=0; k < V; ++k) X[k]=mulc(X[k],zN[(t+TPS*k)*r]);
+        for(int k=0; k < V; ++k) X[k]=mulc(X[k],bt*tr[k]);
       }
       if constexpr(zm) {
         cplx<T> *z=part+(ii*qi+r)*(int64_t) M;

@@ -1151,8 +1151,8 @@ static size_t batch_tps(size_t m, size_t elt)
   return m/(elt == sizeof(cplx<double>) ? BatchV<double>::V : BatchV<float>::V);
 }
 
-// Threads per block and LDS bytes of the register-FFT kernel.
-static void batch_reg_shape(size_t m, size_t elt, size_t threads,
+// Threads per block and LDS bytes of the register-FFT kernel for q residues.
+static void batch_reg_shape(size_t m, size_t q, size_t elt, size_t threads,
                             size_t *nt, size_t *lds)
 {
   const size_t tps=batch_tps(m,elt);
@@ -1160,7 +1160,7 @@ static void batch_reg_shape(size_t m, size_t elt, size_t threads,
   if(n > 512) n=512/tps*tps;
   const size_t pad=m+m/16;
   *nt=n;
-  *lds=m*elt+(n/tps)*2*pad*(elt/2);
+  *lds=m*elt+(n/tps)*2*pad*(elt/2)+q*(m/tps)*elt;
 }
 
 template<typename T, int M, bool ZM>
@@ -1168,7 +1168,8 @@ static int batch_reg_launch_z(const DimArgs &a, const BatchLaunch &b, int mult,
                               hipStream_t st)
 {
   size_t nt, lds;
-  batch_reg_shape(M,sizeof(cplx<T>),b.threads,&nt,&lds);
+  batch_reg_shape(M,(size_t) a.q,sizeof(cplx<T>),b.threads,&nt,&lds);
+  if(lds > lds_max()) return fail(FFTWPP_AMD_ERR_UNSUPPORTED,"batch tile does not fit in LDS");
   int rc=allow_lds(batch_reg_kernel<T,M,ZM>,lds);
   if(rc) return rc;
   hipLaunchKernelGGL((batch_reg_kernel<T,M,ZM>),dim3((unsigned) b.nblocks),dim3((unsigned) nt),
@@ -1760,8 +1761,8 @@ size_t fftwpp_amd_batch_rows_lds(const fftwpp_amd_dim *d, int dtype,
   if(batch_tps(d->m,elt)) {
     if(TR > 1 || lacc) return 0;
     size_t nt, lds;
-    batch_reg_shape(d->m,elt,64,&nt,&lds);
-    return lds;
+    batch_reg_shape(d->m,d->q,elt,64,&nt,&lds);
+    return lds > lds_max() ? 0 : lds;
   }
   const size_t b=batch_lds(d->m,elt,TR,lacc,NULL);
   return b > lds_max() ? 0 : b;
