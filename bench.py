@@ -55,6 +55,10 @@ def parse():
     ap.add_argument("--no-traffic", dest="traffic", action="store_false",
                     help="skip the counter passes (roofline.traffic null)")
     ap.add_argument("--batched-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--workload", default=None,
+                    choices=["hconv2d", "real3d", "conv1d", "user_multiplier"],
+                    help="run one secondary workload alone and print its JSON (for "
+                         "profiling that workload's kernels)")
     ap.add_argument("--traffic-child", default=None,
                     choices=["f64", "f32", "f64-plane", "f32-plane"],
                     help=argparse.SUPPRESS)
@@ -62,6 +66,17 @@ def parse():
                     help="run a 512 MiB device-to-device copy first (the known-size "
                          "reference for HBM traffic counters)")
     return ap.parse_args()
+
+
+def calibration_copy(torch, dev):
+    """Two 512 MiB device-to-device copies: the known-size reference that scales
+    the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    src = torch.ones(64 << 20, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
 
 
 def inputs(L, seed=SEED):
@@ -623,6 +638,18 @@ def main():
         print(json.dumps(bench_batched(torch, torch.device("cuda:0"), args.steps, args.warmup)),
               flush=True)
         return
+    if args.workload:
+        import torch
+        dev = torch.device("cuda:0")
+        if args.calibrate:
+            calibration_copy(torch, dev)
+        fn = {"hconv2d": lambda: bench_hconv2d(torch, dev, args.steps, args.warmup),
+              "real3d": lambda: bench_real3d(torch, dev, args.steps, args.warmup),
+              "conv1d": lambda: bench_conv1d(torch, dev, args.steps),
+              "user_multiplier": lambda: bench_user_multiplier(torch, dev, min(args.steps, 5),
+                                                               1)}[args.workload]
+        print(json.dumps(fn()), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "one GPU only: --gpus N>1 is not supported (the batched "
                                    "items would shard across GPUs with no exchange, the 3D and "
@@ -663,12 +690,7 @@ def main():
     L3 = L ** 3
     dev = torch.device("cuda:0")
     if args.calibrate:
-        src = torch.ones(64 << 20, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(2):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        calibration_copy(torch, dev)
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     f, g = inputs(L)
 
