@@ -580,25 +580,35 @@ int fftwpp_amd_conv_create(fftwpp_amd_conv **h, int dim, int hermitian,
 int fftwpp_amd_conv_set_memory_limit(fftwpp_amd_conv *c, size_t bytes)
 {
   if(!c) return FFTWPP_AMD_ERR_ARG;
-  if(!c->c3 || c->hermitian) {
-    fftwpp_amd_internal_set_error("fftwpp_amd_conv_set_memory_limit: 3D complex plans only");
+  if(c->c3) c->c3->convolve3->memoryLimit=bytes;
+  else if(c->h3) c->h3->convolve3->memoryLimit=bytes;
+  else if(c->c2) c->c2->convolve2->memoryLimit=bytes;
+  else if(c->h2) c->h2->convolve2->memoryLimit=bytes;
+  else {
+    fftwpp_amd_internal_set_error("fftwpp_amd_conv_set_memory_limit: 2D and 3D plans only");
     return FFTWPP_AMD_ERR_UNSUPPORTED;
   }
-  c->c3->convolve3->memoryLimit=bytes;
   return FFTWPP_AMD_OK;
 }
 
 int fftwpp_amd_conv_work_bytes(const fftwpp_amd_conv *c, size_t *bytes, int *plane)
 {
   if(!c || !bytes) return FFTWPP_AMD_ERR_ARG;
-  if(!c->c3 || c->hermitian) {
-    fftwpp_amd_internal_set_error("fftwpp_amd_conv_work_bytes: 3D complex plans only");
+  const bool real=c->hermitian == 3;
+  if(c->c3 || c->h3) {
+    const Convolution3 *k=c->c3 ? c->c3->convolve3 : c->h3->convolve3;
+    const bool p=k->usesPlanes(c->dtype);
+    const size_t D=p ? 0 : k->residueBlock(c->dtype,real);
+    *bytes=p ? k->workBytes(c->dtype,true) : k->workBytes(c->dtype,false,D);
+    if(plane) *plane=p ? 1 : 0;
+  } else if(c->c2 || c->h2) {
+    const Convolution2 *k=c->c2 ? c->c2->convolve2 : c->h2->convolve2;
+    *bytes=k->workBytes(c->dtype,k->residueBlock(c->dtype,real));
+    if(plane) *plane=0;
+  } else {
+    fftwpp_amd_internal_set_error("fftwpp_amd_conv_work_bytes: 2D and 3D plans only");
     return FFTWPP_AMD_ERR_UNSUPPORTED;
   }
-  const Convolution3 *k=c->c3->convolve3;
-  const bool p=k->usesPlanes(c->dtype);
-  *bytes=k->workBytes(c->dtype,p);
-  if(plane) *plane=p ? 1 : 0;
   return FFTWPP_AMD_OK;
 }
 

@@ -1259,32 +1259,75 @@ void Convolution2::execComplex(void *out, const void *f0, const void *f1, bool n
   double s=normalize ? scale : 1.0;
   size_t E=eltsize(dtype);
   size_t qx=fftx->q, mx=fftx->m;
-  // x-transformed rows: every residue, or the kept planes of real data
-  size_t rows=real ? fftwpp_amd_real_planes(fftx->dev) : qx*mx;
-  void *F0=work.get(0,rows*lyc*E);
-  void *F1=work.get(1,rows*lyc*E);
-  if(timeline) timeline->begin(stream);
+  // x residues per pass (all unless the memory limit asks for blocks)
+  const size_t Dx=residueBlock(dtype,real);
+  if(Dx == 0)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"2D convolution: work buffers exceed the memory limit");
+  // x-transformed rows per pass: Dx residues, or the kept planes of real data
+  const size_t maxrows=real ? fftwpp_amd_real_planes(fftx->dev) : Dx*mx;
+  void *F[2]={work.get(0,maxrows*lyc*E),work.get(1,maxrows*lyc*E)};
   const void *in[2]={f0,f1};
-  void *F[2]={F0,F1};
-  for(int a=0; a < 2; ++a) {
-    if(real)
-      check(fftwpp_amd_forward_real_many(fftx->dev,dtype,in[a],Sx,lyc,1,0,F[a],lyc,0,stream),"x forward");
-    else
-      check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,lyc,1,0,F[a],lyc,0,0,qx,stream),"x forward");
-    if(timeline) timeline->mark(real ? (a == 0 ? "x real forward f" : "x real forward g") : (a == 0 ? "x forward f" : "x forward g"));
+  if(Dx < qx && (out == f0 || out == f1)) {
+    // residue blocks write partial results into out: keep the input it aliases
+    void *keep=work.get(10,arraySize()*E);
+    check(fftwpp_amd_memcpy(keep,out,arraySize()*E,stream),"input copy");
+    in[out == f0 ? 0 : 1]=keep;
   }
-  if(rowHook)
-    hookRows(rowHook,ffty,dtype,F0,F1,rows,ly,F0,ly,1.0,stream,outerLayout(fftx,rows,real),"y rows");
-  else if(fused)
-    fusedRows(ffty,work,2,dtype,code,F0,F1,rows,ly,F0,ly,1.0,stream,"y rows");
-  else
-    unfusedRows(ffty,work,8,dtype,code,F0,F1,rows,Cy,lyc,stream,"y rows");
-  if(timeline) timeline->mark(fused ? "y rows" : "y rows (unfused)");
-  if(real)
-    check(fftwpp_amd_backward_real_many(fftx->dev,dtype,F0,lyc,0,out,Sx,lyc,1,0,s,stream),"x backward");
-  else
-    check(fftwpp_amd_backward_many(fftx->dev,dtype,F0,lyc,0,out,Sx,lyc,1,0,0,qx,0,s,stream),"x backward");
-  if(timeline) timeline->mark(real ? "x real backward" : "x backward");
+  if(timeline) timeline->begin(stream);
+  for(size_t r0=0; r0 < qx; r0 += Dx) {
+    const size_t nres=std::min(Dx,qx-r0);
+    const size_t rows=real ? maxrows : nres*mx;
+    const bool first=r0 == 0, last=r0+nres == qx;
+    for(int a=0; a < 2; ++a) {
+      if(real)
+        check(fftwpp_amd_forward_real_many(fftx->dev,dtype,in[a],Sx,lyc,1,0,F[a],lyc,0,stream),"x forward");
+      else
+        check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,lyc,1,0,F[a],lyc,0,r0,nres,stream),"x forward");
+      if(timeline) timeline->mark(real ? (a == 0 ? "x real forward f" : "x real forward g") : (a == 0 ? "x forward f" : "x forward g"));
+    }
+    if(rowHook)
+      hookRows(rowHook,ffty,dtype,F[0],F[1],rows,ly,F[0],ly,1.0,stream,outerLayout(fftx,rows,real),"y rows");
+    else if(fused)
+      fusedRows(ffty,work,2,dtype,code,F[0],F[1],rows,ly,F[0],ly,1.0,stream,"y rows");
+    else
+      unfusedRows(ffty,work,8,dtype,code,F[0],F[1],rows,Cy,lyc,stream,"y rows");
+    if(timeline) timeline->mark(fused ? "y rows" : "y rows (unfused)");
+    // this block's x backward added to the earlier blocks' sum; the last block
+    // applies the normalisation to the whole sum
+    if(real)
+      check(fftwpp_amd_backward_real_many(fftx->dev,dtype,F[0],lyc,0,out,Sx,lyc,1,0,s,stream),"x backward");
+    else
+      check(fftwpp_amd_backward_many(fftx->dev,dtype,F[0],lyc,0,out,Sx,lyc,1,0,r0,nres,!first,
+                                     last ? s : 1.0,stream),"x backward");
+    if(timeline) timeline->mark(real ? "x real backward" : "x backward");
+  }
+}
+
+// Work bytes of an execution with Dx x residues per pass (0: all): the two
+// x-transformed arrays, the unfused row stage's two expanded arrays when it
+// runs, and with residue blocks a copy of the input an in-place call
+// overwrites.
+size_t Convolution2::workBytes(int dtype, size_t Dx) const
+{
+  const size_t E=eltsize(dtype);
+  const bool real=realPlanes(fftx);
+  if(Dx == 0 || real) Dx=fftx->q;
+  const size_t rows=real ? fftwpp_amd_real_planes(fftx->dev) : Dx*fftx->m;
+  const size_t Cy=ffty->C;
+  size_t w=2*rows*ly*Cy*E;
+  if(Cy != 1 || !rowsSupported(ffty,dtype)) w += 2*rows*ffty->m*ffty->q*Cy*E;
+  if(Dx < fftx->q) w += arraySize()*E;
+  return w;
+}
+
+size_t Convolution2::residueBlock(int dtype, bool real) const
+{
+  const size_t qx=fftx->q;
+  if(!memoryLimit) return qx;
+  if(real || rowHook) return workBytes(dtype,qx) <= memoryLimit ? qx : 0;
+  for(size_t D=qx; D > 0; --D)
+    if(workBytes(dtype,D) <= memoryLimit) return D;
+  return 0;
 }
 
 void Convolution2::run(Complex **f, size_t offset, bool normalize)
@@ -1334,13 +1377,32 @@ Convolution3::Convolution3(fftBase *fftx, fftBase *ffty, fftBase *fftz,
 
 Convolution3::~Convolution3() {}
 
-size_t Convolution3::workBytes(int dtype, bool plane) const
+size_t Convolution3::workBytes(int dtype, bool plane, size_t Dx) const
 {
   const size_t E=eltsize(dtype);
-  const size_t planesX=realPlanes(fftx) ? fftwpp_amd_real_planes(fftx->dev) : fftx->q*fftx->m;
+  const bool real=realPlanes(fftx);
+  if(Dx == 0 || real) Dx=fftx->q;
+  const size_t planesX=real ? fftwpp_amd_real_planes(fftx->dev) : Dx*fftx->m;
   const size_t X=planesX*Ly*Lz*E;                     // one x-transformed array
   if(plane) return 2*X+fftwpp_amd_plane_work(dtype);
-  return 2*X+2*ffty->q*X;
+  // residue blocks: the input that out aliases is kept (in-place calls)
+  const size_t keep=Dx < fftx->q ? arraySize()*E : 0;
+  // the unfused z stage expands every z row to its padded length
+  const size_t zrows=planesX*ffty->q*ffty->m;
+  const size_t unf=fftz->C != 1 || !rowsSupported(fftz,dtype) ? 2*zrows*fftz->m*fftz->q*fftz->C*E : 0;
+  return 2*X+2*ffty->q*X+keep+unf;
+}
+
+// x residues per pass: all of them, or under a memory limit the largest
+// block whose work buffers fit (0: none fits).  Real outer dimensions keep
+// all their planes in one pass.
+size_t Convolution3::residueBlock(int dtype, bool real) const
+{
+  const size_t qx=fftx->q;
+  if(!memoryLimit || real || rowHook) return workBytes(dtype,false,qx) <= memoryLimit || !memoryLimit ? qx : 0;
+  for(size_t D=qx; D > 0; --D)
+    if(workBytes(dtype,false,D) <= memoryLimit) return D;
+  return 0;
 }
 
 // The plane kernel runs when it applies and either FFTWPP_AMD_PLANE=1 asks for
@@ -1353,7 +1415,8 @@ bool Convolution3::usesPlanes(int dtype) const
   if(!planes) return false;
   const char *env=getenv("FFTWPP_AMD_PLANE");
   if(env && env[0] == '1') return true;
-  return memoryLimit != 0 && workBytes(dtype,false) > memoryLimit;
+  return memoryLimit != 0 && workBytes(dtype,false,fftx->q) > memoryLimit &&
+    workBytes(dtype,true) <= memoryLimit;
 }
 
 size_t Convolution3::arraySize() const { return Sx*(Lx-1)+Sy*(Ly-1)+Lz; }
@@ -1394,20 +1457,19 @@ void Convolution3::execComplex(void *out, const void *f0, const void *f1, bool n
   double s=normalize ? scale : 1.0;
   size_t E=eltsize(dtype);
   size_t qx=fftx->q, mx=fftx->m, qy=ffty->q, my=ffty->m;
-  // x-transformed planes of Ly x Lz: every residue, or the kept planes of
-  // real data
-  size_t nplanes=real ? fftwpp_amd_real_planes(fftx->dev) : qx*mx;
   size_t plane=Ly*Lz;
   size_t yplane=qy*my*Lz;        // y-transformed plane
-  void *X0=work.get(0,nplanes*plane*E);
-  void *X1=work.get(1,nplanes*plane*E);
   const void *in[2]={f0,f1};
-  void *X[2]={X0,X1};
   const bool viaPlanes=!real && !rowHook && code != FFTWPP_AMD_MULT_REAL_BINARY && usesPlanes(dtype);
-  if(memoryLimit && workBytes(dtype,viaPlanes) > memoryLimit)
+  // x residues per pass of the separate y, z, y stages (all of them unless
+  // the memory limit asks for residue blocks)
+  const size_t Dx=viaPlanes ? qx : residueBlock(dtype,real);
+  if(Dx == 0)
     throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"3D convolution: work buffers exceed the memory limit");
   if(viaPlanes) {
     // x forward (f, g) -> y-z plane convolutions (one launch) -> x backward
+    const size_t nplanes=qx*mx;
+    void *X[2]={work.get(0,nplanes*plane*E),work.get(1,nplanes*plane*E)};
     void *P=work.get(5,fftwpp_amd_plane_work(dtype));
     if(timeline) timeline->begin(stream);
     for(int a=0; a < 2; ++a) {
@@ -1417,51 +1479,67 @@ void Convolution3::execComplex(void *out, const void *f0, const void *f1, bool n
         check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,Lz,Ly,Sy,X[a],plane,Lz,0,qx,stream),"x forward");
       if(timeline) timeline->mark(a == 0 ? "x forward f" : "x forward g");
     }
-    check(fftwpp_amd_plane_conv(ffty->dev,fftz->dev,dtype,code,X0,X1,X0,nplanes,Ly,lz,P,stream),
+    check(fftwpp_amd_plane_conv(ffty->dev,fftz->dev,dtype,code,X[0],X[1],X[0],nplanes,Ly,lz,P,stream),
           "y-z planes");
     if(timeline) timeline->mark("y-z planes");
-    check(fftwpp_amd_backward_many(fftx->dev,dtype,X0,plane,0,out,Sx,plane,1,0,0,qx,0,s,stream),
+    check(fftwpp_amd_backward_many(fftx->dev,dtype,X[0],plane,0,out,Sx,plane,1,0,0,qx,0,s,stream),
           "x backward");
     if(timeline) timeline->mark("x backward");
     return;
   }
-  void *Y0=work.get(2,nplanes*yplane*E);
-  void *Y1=work.get(3,nplanes*yplane*E);
-  // x forward: columns (y,z); non-contiguous y strides become batches
-  if(timeline) timeline->begin(stream);
-  void *Y[2]={Y0,Y1};
-  for(int a=0; a < 2; ++a) {
-    if(real && contiguous())
-      check(fftwpp_amd_forward_real_many(fftx->dev,dtype,in[a],Sx,plane,1,0,X[a],plane,0,stream),"x forward");
-    else if(real)
-      check(fftwpp_amd_forward_real_many(fftx->dev,dtype,in[a],Sx,Lz,Ly,Sy,X[a],plane,Lz,stream),"x forward");
-    else if(contiguous())
-      check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,plane,1,0,X[a],plane,0,0,qx,stream),"x forward");
-    else
-      check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,Lz,Ly,Sy,X[a],plane,Lz,0,qx,stream),"x forward");
-    if(timeline) timeline->mark(real ? (a == 0 ? "x real forward f" : "x real forward g") : (a == 0 ? "x forward f" : "x forward g"));
-    check(fftwpp_amd_forward_many(ffty->dev,dtype,X[a],Lz,Lz,nplanes,plane,Y[a],Lz,yplane,0,qy,stream),"y forward");
-    if(timeline) timeline->mark(a == 0 ? "y forward f" : "y forward g");
+  // x-transformed planes of Ly x Lz per pass: Dx residues, or the kept planes
+  // of real data
+  const size_t maxplanes=real ? fftwpp_amd_real_planes(fftx->dev) : Dx*mx;
+  void *X[2]={work.get(0,maxplanes*plane*E),work.get(1,maxplanes*plane*E)};
+  void *Y[2]={work.get(2,maxplanes*yplane*E),work.get(3,maxplanes*yplane*E)};
+  if(Dx < qx && (out == f0 || out == f1)) {
+    // residue blocks write partial results into out: keep the input it aliases
+    void *keep=work.get(9,arraySize()*E);
+    check(fftwpp_amd_memcpy(keep,out,arraySize()*E,stream),"input copy");
+    in[out == f0 ? 0 : 1]=keep;
   }
-  if(rowHook)
-    hookRows(rowHook,fftz,dtype,Y0,Y1,nplanes*qy*my,lz,Y0,lz,1.0,stream,
-             outerLayout(fftx,nplanes,real,ffty),"z rows");
-  else if(fused)
-    fusedRows(fftz,work,4,dtype,code,Y0,Y1,nplanes*qy*my,lz,Y0,lz,1.0,stream,"z rows");
-  else
-    unfusedRows(fftz,work,8,dtype,code,Y0,Y1,nplanes*qy*my,Cz,Lz,stream,"z rows");
-  if(timeline) timeline->mark(fused ? "z rows" : "z rows (unfused)");
-  check(fftwpp_amd_backward_many(ffty->dev,dtype,Y0,Lz,yplane,X0,Lz,Lz,nplanes,plane,0,qy,0,1.0,stream),"y backward");
-  if(timeline) timeline->mark("y backward");
-  if(real && contiguous())
-    check(fftwpp_amd_backward_real_many(fftx->dev,dtype,X0,plane,0,out,Sx,plane,1,0,s,stream),"x backward");
-  else if(real)
-    check(fftwpp_amd_backward_real_many(fftx->dev,dtype,X0,plane,Lz,out,Sx,Lz,Ly,Sy,s,stream),"x backward");
-  else if(contiguous())
-    check(fftwpp_amd_backward_many(fftx->dev,dtype,X0,plane,0,out,Sx,plane,1,0,0,qx,0,s,stream),"x backward");
-  else
-    check(fftwpp_amd_backward_many(fftx->dev,dtype,X0,plane,Lz,out,Sx,Lz,Ly,Sy,0,qx,0,s,stream),"x backward");
-  if(timeline) timeline->mark(real ? "x real backward" : "x backward");
+  if(timeline) timeline->begin(stream);
+  for(size_t r0=0; r0 < qx; r0 += Dx) {
+    const size_t nres=std::min(Dx,qx-r0);
+    const size_t nplanes=real ? maxplanes : nres*mx;
+    const bool first=r0 == 0, last=r0+nres == qx;
+    // x forward: columns (y,z); non-contiguous y strides become batches
+    for(int a=0; a < 2; ++a) {
+      if(real && contiguous())
+        check(fftwpp_amd_forward_real_many(fftx->dev,dtype,in[a],Sx,plane,1,0,X[a],plane,0,stream),"x forward");
+      else if(real)
+        check(fftwpp_amd_forward_real_many(fftx->dev,dtype,in[a],Sx,Lz,Ly,Sy,X[a],plane,Lz,stream),"x forward");
+      else if(contiguous())
+        check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,plane,1,0,X[a],plane,0,r0,nres,stream),"x forward");
+      else
+        check(fftwpp_amd_forward_many(fftx->dev,dtype,in[a],Sx,Lz,Ly,Sy,X[a],plane,Lz,r0,nres,stream),"x forward");
+      if(timeline) timeline->mark(real ? (a == 0 ? "x real forward f" : "x real forward g") : (a == 0 ? "x forward f" : "x forward g"));
+      check(fftwpp_amd_forward_many(ffty->dev,dtype,X[a],Lz,Lz,nplanes,plane,Y[a],Lz,yplane,0,qy,stream),"y forward");
+      if(timeline) timeline->mark(a == 0 ? "y forward f" : "y forward g");
+    }
+    if(rowHook)
+      hookRows(rowHook,fftz,dtype,Y[0],Y[1],nplanes*qy*my,lz,Y[0],lz,1.0,stream,
+               outerLayout(fftx,nplanes,real,ffty),"z rows");
+    else if(fused)
+      fusedRows(fftz,work,4,dtype,code,Y[0],Y[1],nplanes*qy*my,lz,Y[0],lz,1.0,stream,"z rows");
+    else
+      unfusedRows(fftz,work,10,dtype,code,Y[0],Y[1],nplanes*qy*my,Cz,Lz,stream,"z rows");
+    if(timeline) timeline->mark(fused ? "z rows" : "z rows (unfused)");
+    check(fftwpp_amd_backward_many(ffty->dev,dtype,Y[0],Lz,yplane,X[0],Lz,Lz,nplanes,plane,0,qy,0,1.0,stream),"y backward");
+    if(timeline) timeline->mark("y backward");
+    // x backward of this block, added to the earlier blocks' sum; the last
+    // block applies the normalisation to the whole sum
+    const double sb=last ? s : 1.0;
+    if(real && contiguous())
+      check(fftwpp_amd_backward_real_many(fftx->dev,dtype,X[0],plane,0,out,Sx,plane,1,0,s,stream),"x backward");
+    else if(real)
+      check(fftwpp_amd_backward_real_many(fftx->dev,dtype,X[0],plane,Lz,out,Sx,Lz,Ly,Sy,s,stream),"x backward");
+    else if(contiguous())
+      check(fftwpp_amd_backward_many(fftx->dev,dtype,X[0],plane,0,out,Sx,plane,1,0,r0,nres,!first,sb,stream),"x backward");
+    else
+      check(fftwpp_amd_backward_many(fftx->dev,dtype,X[0],plane,Lz,out,Sx,Lz,Ly,Sy,r0,nres,!first,sb,stream),"x backward");
+    if(timeline) timeline->mark(real ? "x real backward" : "x backward");
+  }
 }
 
 void Convolution3::run(Complex **f, size_t offset, bool normalize)

@@ -111,9 +111,9 @@ class DeviceConvolution:
         return self.convolve_(out, b)
 
     def set_memory_limit(self, nbytes):
-        """Keep the plan's device work buffers within nbytes (3D complex
-        plans; 0 = no limit): a limit below the separate y, z and y stages'
-        need selects the plane kernel for the y-z stage."""
+        """Keep the plan's device work buffers within nbytes (2D and 3D
+        plans; 0 = no limit): a 3D plan the plane kernel takes runs its y-z
+        stage there, other plans run their outer residues in blocks."""
         check(lib().fftwpp_amd_conv_set_memory_limit(self._h, int(nbytes)))
 
     def work_bytes(self):

@@ -458,6 +458,16 @@ public:
   Timeline *timeline=NULL;  // when set, every launch is timed
   const RowHook *rowHook=NULL;  // when set, the row stage runs through it
 
+  // Device bytes the plan's work buffers may hold (0: no limit); over it the
+  // x residues run in blocks of Dx per pass (the reference's residue loop,
+  // convolve.h:1475-1535) with a copy of the input an in-place call
+  // overwrites.
+  size_t memoryLimit=0;
+  // Work bytes with Dx x residues per pass (0: all of them).
+  size_t workBytes(int dtype, size_t Dx=0) const;
+  // x residues per pass (0: none fits the limit).
+  size_t residueBlock(int dtype, bool real) const;
+
 protected:
   int multCode;
   size_t ly;  // stored length of a y row (Ly or ceil(Ly/2))
@@ -511,12 +521,18 @@ public:
   // Device bytes the plan's work buffers may hold (0: no limit).  The
   // separate y, z and y launches need 2 + 2 q_y x-transformed arrays; the
   // plane kernel (y and z with m = 256, q = 2, L = 256) needs two plus
-  // fftwpp_amd_plane_work(), and is taken when the limit excludes the first.
+  // fftwpp_amd_plane_work(), and is taken when the limit excludes the first;
+  // otherwise the x residues run in blocks of Dx per pass (the reference's
+  // residue loop, convolve.h:1812-1879), which needs 2 + 2 q_y arrays of Dx
+  // residues and a copy of the input an in-place call overwrites.
   size_t memoryLimit=0;
-  // Work bytes of a complex execution in dtype, plane engine or not.
-  size_t workBytes(int dtype, bool plane) const;
+  // Work bytes of a complex execution in dtype: plane engine, or Dx x
+  // residues per pass (0: all).
+  size_t workBytes(int dtype, bool plane, size_t Dx=0) const;
   // Whether an execution in dtype takes the plane kernel.
   bool usesPlanes(int dtype) const;
+  // x residues per pass of the separate stages (0: none fits the limit).
+  size_t residueBlock(int dtype, bool real) const;
 
 protected:
   int multCode;

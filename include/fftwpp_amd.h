@@ -302,16 +302,20 @@ int fftwpp_amd_conv_create(fftwpp_amd_conv **h, int dim, int hermitian,
                            const size_t *m, int dtype);
 int fftwpp_amd_conv_destroy(fftwpp_amd_conv *h);
 
-/* Memory-bounded execution of 3D complex plans (no reference counterpart:
- * the reference bounds its buffers by processing D residues per pass,
- * convolve.h:445-451).  The plan's device work buffers stay within `bytes`
- * (0 = no limit): when the separate y, z and y stages would need more, the
- * y-z stage runs as the plane kernel (two x-transformed arrays plus
- * fftwpp_amd_plane_work() bytes instead of 2 + 2 q_y arrays); an execution
- * that fits neither fails with FFTWPP_AMD_ERR_UNSUPPORTED.  Results are the
- * same either way.  fftwpp_amd_conv_work_bytes reports the bytes the next
- * execution in the plan's precision holds and whether it takes the plane
- * kernel. */
+/* Memory-bounded execution of 2D and 3D plans (the reference bounds its
+ * buffers by processing D residues per pass, convolve.h:445-451, and loops
+ * over the outer residues, convolve.h:1475-1535, 1812-1879).  The plan's
+ * device work buffers stay within `bytes` (0 = no limit): a 3D plan whose
+ * y-z stage the plane kernel takes (y and z complex with m = 256, q = 2,
+ * L = 256) runs it when the separate y, z and y stages would need more (two
+ * x-transformed arrays plus fftwpp_amd_plane_work() bytes instead of 2 + 2 q_y
+ * arrays); otherwise the outer (x) residues run in the largest blocks that
+ * fit, each block's backward transform added to the output, with a copy of
+ * the input an in-place call overwrites.  Real plans run in one pass.  An
+ * execution that fits none of these fails with FFTWPP_AMD_ERR_UNSUPPORTED.
+ * Results are the same in every case.  fftwpp_amd_conv_work_bytes reports the
+ * bytes the next execution in the plan's precision holds and whether it takes
+ * the plane kernel. */
 int fftwpp_amd_conv_set_memory_limit(fftwpp_amd_conv *h, size_t bytes);
 int fftwpp_amd_conv_work_bytes(const fftwpp_amd_conv *h, size_t *bytes, int *plane);
 
