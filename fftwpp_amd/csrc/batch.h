@@ -128,6 +128,37 @@ batch_conv_kernel(DimArgs d, const BatchItem *items, int64_t count,
   }
 }
 
+// a[k] = f[j_k], b[k] = g[j_k] at j_k = min(j0 + TPS k, L - 1), every load
+// issued before any is used.  FP64 reads through buffer resources with 32-bit
+// offsets (one address register per load instead of two: the kernel then
+// stays below 256 VGPRs without spills); FP32 through plain pointers, which
+// measured faster there.  Clamped offsets stay inside the item, so no range
+// check is relied on.  Items of 2^27 words or more (byte offsets beyond 31
+// bits) take the pointer loads in FP64 too.
+template<typename T, int V, int TPS>
+__device__ __forceinline__ void batch_fold_loads(const cplx<T> *f, const cplx<T> *g, int L,
+                                                 int j0, cplx<T> *a, cplx<T> *b)
+{
+  if constexpr(sizeof(T) == 8) {
+    if(L < (1 << 27)) {
+      const __amdgpu_buffer_rsrc_t rf=buf_rsrc<T>(f,(size_t) L), rg=buf_rsrc<T>(g,(size_t) L);
+#pragma unroll
+      for(int k=0; k < V; ++k) {
+        const unsigned o=(unsigned) min(j0+TPS*k,L-1)*(unsigned) sizeof(cplx<T>);
+        a[k]=ld_buf<T>(rf,o,0);
+        b[k]=ld_buf<T>(rg,o,0);
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for(int k=0; k < V; ++k) {
+    const int j=min(j0+TPS*k,L-1);
+    a[k]=f[j];
+    b[k]=g[j];
+  }
+}
+
 // Values per thread of the register FFT: 16 in FP32, 8 in FP64 (two
 // transforms and the FFT's temporaries then fit two waves per SIMD).
 template<typename T> struct BatchV { static constexpr int V=sizeof(T) == 8 ? 8 : 16; };
@@ -213,13 +244,16 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
       for(int k=0; k < V; ++k) X[k]=Y[k]=mk((T) 0,(T) 0);
       for(int jb=0, eb=0; jb < L; jb += M) {
         const cplx<T> wb=zN[eb*M];               // w_q^{b r}, eb = b r mod q
+        // all 2 V loads of a block issued before the first use: indices
+        // clamped into the item, the terms beyond it zeroed by their factor
+        // (per-element branches made the loads wait one by one)
+        cplx<T> a[V], b[V];
+        batch_fold_loads<T,V,TPS>(fi,gi,L,jb+t,a,b);
 #pragma unroll
         for(int k=0; k < V; ++k) {
-          const int j=jb+t+TPS*k;
-          if(j < L) {
-            X[k]=X[k]+wb*fi[j];
-            Y[k]=Y[k]+wb*gi[j];
-          }
+          const cplx<T> w=jb+t+TPS*k < L ? wb : mk((T) 0,(T) 0);
+          X[k]=X[k]+w*a[k];
+          Y[k]=Y[k]+w*b[k];
         }
         eb += r;
         if(eb >= qi) eb -= qi;
@@ -253,12 +287,19 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
       const bool first=r == rbeg, last=r == rend-1;
       for(int jb=0, eb=0; jb < L; jb += M) {
         const cplx<T> wb=zN[eb*M];
+        // earlier residues' sums read unconditionally (clamped), as the fold
+        cplx<T> s[V];
+#pragma unroll
+        for(int k=0; k < V; ++k) s[k]=mk((T) 0,(T) 0);
+        if(!first) {
+#pragma unroll
+          for(int k=0; k < V; ++k) s[k]=dst[min(jb+t+TPS*k,L-1)];
+        }
 #pragma unroll
         for(int k=0; k < V; ++k) {
           const int j=jb+t+TPS*k;
           if(j < L) {
-            cplx<T> v=mulc(X[k],wb);
-            if(!first) v=v+dst[j];
+            const cplx<T> v=mulc(X[k],wb)+s[k];
             dst[j]=last ? sc*v : v;
           }
         }
@@ -271,8 +312,11 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
 
 // out[off + sigma + M b] = scale * sum_{r < q} w_q^{-b r} Z[i q M + r M + sigma]
 // for the blocks b of each item (sequence mode of batch_reg_kernel); one
-// block per item (grid-stride), threads over sigma.
-template<typename T>
+// block per item (grid-stride), threads over sigma.  With QMAX > 0 (q <= QMAX)
+// a thread loads its q values Z_r(sigma) once, before the sums over the
+// blocks (indices clamped to r < q, the extra terms given a zero factor);
+// QMAX = 0 takes any q and re-reads them per block.
+template<typename T, int QMAX>
 __global__ void __launch_bounds__(256)
 batch_unfold_kernel(const BatchItem *items, int64_t count, const cplx<T> *Z,
                     int q, int M, const cplx<T> *zN, cplx<T> *out, T scale)
@@ -282,15 +326,32 @@ batch_unfold_kernel(const BatchItem *items, int64_t count, const cplx<T> *Z,
     const cplx<T> *zi=Z+i*(int64_t) q*M;
     const int L=(int) it.L;
     for(int sg=threadIdx.x; sg < M; sg += blockDim.x) {
-      for(int b=0, jb=0; jb < L; ++b, jb += M) {
-        if(jb+sg >= L) break;
-        cplx<T> v=zi[sg];
-        for(int r=1, e=b; r < q; ++r) {
-          v=v+mulc(zi[r*M+sg],zN[(int64_t) e*M]);    // w_q^{-b r}, e = b r mod q
-          e += b;
-          if(e >= q) e -= q;
+      if constexpr(QMAX > 0) {
+        cplx<T> z[QMAX];
+#pragma unroll
+        for(int r=0; r < QMAX; ++r) z[r]=zi[min(r,q-1)*M+sg];
+        for(int b=0, jb=0; jb+sg < L; ++b, jb += M) {
+          cplx<T> v=z[0];
+#pragma unroll
+          for(int r=1, e=b; r < QMAX; ++r) {
+            e=e >= q ? e-q : e;                    // e = b r mod q (b < q)
+            const cplx<T> w=r < q ? zN[(int64_t) e*M] : mk((T) 0,(T) 0);
+            v=v+mulc(z[r],w);
+            e += b;
+          }
+          out[it.off+jb+sg]=scale*v;
         }
-        out[it.off+jb+sg]=scale*v;
+      } else {
+        for(int b=0, jb=0; jb < L; ++b, jb += M) {
+          if(jb+sg >= L) break;
+          cplx<T> v=zi[sg];
+          for(int r=1, e=b; r < q; ++r) {
+            v=v+mulc(zi[r*M+sg],zN[(int64_t) e*M]);    // w_q^{-b r}, e = b r mod q
+            e += b;
+            if(e >= q) e -= q;
+          }
+          out[it.off+jb+sg]=scale*v;
+        }
       }
     }
   }

@@ -1207,9 +1207,20 @@ static int batch_reg_t(const DimArgs &a, const BatchLaunch &b, int mult,
   if(rc) return rc;
   if(b.D == 0) {
     const size_t nb=b.count < 4096 ? b.count : 4096;
-    hipLaunchKernelGGL(batch_unfold_kernel<T>,dim3((unsigned) nb),dim3(256),0,st,
-                       b.items,(int64_t) b.count,(const cplx<T> *) b.part,(int) a.q,(int) a.m,
-                       (const cplx<T> *) a.zN,(cplx<T> *) b.out,(T) b.scale);
+    const dim3 grid((unsigned) nb), blk(256);
+    const BatchItem *it=b.items;
+    const int64_t n=(int64_t) b.count;
+    const cplx<T> *Z=(const cplx<T> *) b.part, *zN=(const cplx<T> *) a.zN;
+    cplx<T> *out=(cplx<T> *) b.out;
+    const int q=(int) a.q, m=(int) a.m;
+    if(q <= 4)
+      hipLaunchKernelGGL((batch_unfold_kernel<T,4>),grid,blk,0,st,it,n,Z,q,m,zN,out,(T) b.scale);
+    else if(q <= 8)
+      hipLaunchKernelGGL((batch_unfold_kernel<T,8>),grid,blk,0,st,it,n,Z,q,m,zN,out,(T) b.scale);
+    else if(q <= 16)
+      hipLaunchKernelGGL((batch_unfold_kernel<T,16>),grid,blk,0,st,it,n,Z,q,m,zN,out,(T) b.scale);
+    else
+      hipLaunchKernelGGL((batch_unfold_kernel<T,0>),grid,blk,0,st,it,n,Z,q,m,zN,out,(T) b.scale);
     return launch_check("batch_unfold_kernel");
   }
   const size_t G=(a.q+b.D-1)/b.D;

@@ -104,23 +104,27 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
         // w_N^{(j+k0) r}: j + k0 is sigma or sigma - M
         const cplx<T> bt=zN[t*r], zm=conjg(zN[M*r]);
         cplx<T> X[V], Y[V];
+        // every load issued before the first use: indices clamped into the
+        // row, the terms beyond it zeroed by their factor (per-element
+        // branches made the loads wait one by one)
 #pragma unroll
         for(int k=0; k < V; ++k) {
-          const int sigma=t+TPS*k;
-          const int j=(sigma-k0) & (M-1);      // k0 <= 0, M a power of two
-          cplx<T> a=mk((T) 0,(T) 0), b=a;
-          if(j < L) {
-            a=fr[j];
-            b=gr[j];
-            if(r) {
-              cplx<T> w=bt*zN[TPS*k*r];
-              if(j+k0 < 0) w=w*zm;
-              a=w*a;
-              b=w*b;
-            }
+          const int j=min((t+TPS*k-k0) & (M-1),L-1);   // k0 <= 0, M a power of two
+          X[k]=fr[j];
+          Y[k]=gr[j];
+        }
+#pragma unroll
+        for(int k=0; k < V; ++k) {
+          const int j=(t+TPS*k-k0) & (M-1);
+          if(r) {
+            cplx<T> w=bt*zN[TPS*k*r];
+            if(j+k0 < 0) w=w*zm;
+            w=j < L ? w : mk((T) 0,(T) 0);
+            X[k]=w*X[k];
+            Y[k]=w*Y[k];
+          } else if(j >= L) {
+            X[k]=Y[k]=mk((T) 0,(T) 0);
           }
-          X[k]=a;
-          Y[k]=b;
         }
         fftr<T,M,V,1,true>(X,xb,tw,t);
         fftr<T,M,V,1,true>(Y,xb,tw,t);
