@@ -104,27 +104,23 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
         // w_N^{(j+k0) r}: j + k0 is sigma or sigma - M
         const cplx<T> bt=zN[t*r], zm=conjg(zN[M*r]);
         cplx<T> X[V], Y[V];
-        // every load issued before the first use: indices clamped into the
-        // row, the terms beyond it zeroed by their factor (per-element
-        // branches made the loads wait one by one)
 #pragma unroll
         for(int k=0; k < V; ++k) {
-          const int j=min((t+TPS*k-k0) & (M-1),L-1);   // k0 <= 0, M a power of two
-          X[k]=fr[j];
-          Y[k]=gr[j];
-        }
-#pragma unroll
-        for(int k=0; k < V; ++k) {
-          const int j=(t+TPS*k-k0) & (M-1);
-          if(r) {
-            cplx<T> w=bt*zN[TPS*k*r];
-            if(j+k0 < 0) w=w*zm;
-            w=j < L ? w : mk((T) 0,(T) 0);
-            X[k]=w*X[k];
-            Y[k]=w*Y[k];
-          } else if(j >= L) {
-            X[k]=Y[k]=mk((T) 0,(T) 0);
+          const int sigma=t+TPS*k;
+          const int j=(sigma-k0) & (M-1);      // k0 <= 0, M a power of two
+          cplx<T> a=mk((T) 0,(T) 0), b=a;
+          if(j < L) {
+            a=fr[j];
+            b=gr[j];
+            if(r) {
+              cplx<T> w=bt*zN[TPS*k*r];
+              if(j+k0 < 0) w=w*zm;
+              a=w*a;
+              b=w*b;
+            }
           }
+          X[k]=a;
+          Y[k]=b;
         }
         fftr<T,M,V,1,true>(X,xb,tw,t);
         fftr<T,M,V,1,true>(Y,xb,tw,t);
@@ -615,14 +611,21 @@ colfwd_reg_kernel(DimArgs d, const cplx<T> *f, int64_t S, int64_t C,
   }
   for(int64_t rr=0; rr < nres; ++rr) {
     const int r=(int) (r0+rr);
-    cplx<T> X[V];
+    // the residue's twiddles loaded before the first use (r = 0 reads
+    // w_N^0 = 1): a load per branch made each wait for the previous one
+    cplx<T> w[P2][V];
 #pragma unroll
     for(int k=0; k < V; ++k) {
       const int j0=(t+TPS*k-k0) & (M-1);
-      cplx<T> a=r ? raw[0][k]*zN[wrapN((j0+k0)*r,N)] : raw[0][k];
 #pragma unroll
-      for(int i=1; i < P2; ++i)
-        a=a+(r ? raw[i][k]*zN[wrapN((j0+M*i+k0)*r,N)] : raw[i][k]);
+      for(int i=0; i < P2; ++i) w[i][k]=zN[wrapN((j0+M*i+k0)*r,N)];
+    }
+    cplx<T> X[V];
+#pragma unroll
+    for(int k=0; k < V; ++k) {
+      cplx<T> a=raw[0][k]*w[0][k];
+#pragma unroll
+      for(int i=1; i < P2; ++i) a=a+raw[i][k]*w[i][k];
       X[k]=a;
     }
     fftr_blkx<T,M,V,1,CR>(X,xb,tw,t,xm);
@@ -672,14 +675,18 @@ colbwd_reg_kernel(DimArgs d, const cplx<T> *F, int64_t SF, int64_t bsi,
 #pragma unroll
     for(int k=0; k < V; ++k) X[k]=ok ? Fr[(int64_t) (t+TPS*k)*SF] : mk((T) 0,(T) 0);
     fftr_blkx<T,M,V,-1,CR>(X,xb,tw,t,xm);
+    // twiddles loaded before the first use (r = 0 reads 1), as forward
+    cplx<T> w[P2][V];
 #pragma unroll
     for(int k=0; k < V; ++k) {
       const int j0=(t+TPS*k-k0) & (M-1);
 #pragma unroll
-      for(int i=0; i < P2; ++i) {
-        const int j=j0+M*i;
-        acc[i][k]=acc[i][k]+(r ? mulc(X[k],zN[wrapN((j+k0)*r,N)]) : X[k]);
-      }
+      for(int i=0; i < P2; ++i) w[i][k]=zN[wrapN((j0+M*i+k0)*r,N)];
+    }
+#pragma unroll
+    for(int k=0; k < V; ++k) {
+#pragma unroll
+      for(int i=0; i < P2; ++i) acc[i][k]=acc[i][k]+mulc(X[k],w[i][k]);
     }
   }
   if(ok) {
@@ -763,14 +770,18 @@ colfwd_real_kernel(DimArgs d, const T *f, int64_t S, int64_t C, int64_t bsi,
     }
   const int R=(int) RealKept::residues(q);
   for(int r=0; r < R; ++r) {
+    // twiddles loaded before the first use (r = 0 reads 1)
+    cplx<T> w[P2][V];
+#pragma unroll
+    for(int k=0; k < V; ++k)
+#pragma unroll
+      for(int i=0; i < P2; ++i) w[i][k]=zN[wrapN((t+TPS*k+M*i)*r,N)];
     cplx<T> X[V];
 #pragma unroll
     for(int k=0; k < V; ++k) {
-      const int j0=t+TPS*k;
-      cplx<T> a=r ? raw[0][k]*zN[wrapN(j0*r,N)] : mk(raw[0][k],(T) 0);
+      cplx<T> a=raw[0][k]*w[0][k];
 #pragma unroll
-      for(int i=1; i < P2; ++i)
-        a=a+(r ? raw[i][k]*zN[wrapN((j0+M*i)*r,N)] : mk(raw[i][k],(T) 0));
+      for(int i=1; i < P2; ++i) a=a+raw[i][k]*w[i][k];
       X[k]=a;
     }
     fftr_blkx<T,M,V,1,CR>(X,xb,tw,t,xm);
@@ -833,17 +844,21 @@ colbwd_real_kernel(DimArgs d, const cplx<T> *F, int64_t SF, int64_t bsi,
     }
     fftr_blkx<T,M,V,-1,CR>(X,xb,tw,t,xm);
     const T cr=self ? (T) 1 : (T) 2;
+    // twiddles loaded before their use (r = 0 reads 1), half of the values
+    // at a time (all at once costs a wave per SIMD)
+    constexpr int KH=V/2;
 #pragma unroll
-    for(int k=0; k < V; ++k) {
+    for(int h=0; h < V; h += KH) {
+      cplx<T> w[P2][KH];
 #pragma unroll
-      for(int i=0; i < P2; ++i) {
-        const int j=t+TPS*k+M*i;
-        if(r) {
-          const cplx<T> w=zN[wrapN(j*r,N)];
-          acc[i][k] += cr*(X[k].x*w.x+X[k].y*w.y);
-        } else
-          acc[i][k] += X[k].x;
-      }
+      for(int k=0; k < KH; ++k)
+#pragma unroll
+        for(int i=0; i < P2; ++i) w[i][k]=zN[wrapN((t+TPS*(h+k)+M*i)*r,N)];
+#pragma unroll
+      for(int k=0; k < KH; ++k)
+#pragma unroll
+        for(int i=0; i < P2; ++i)
+          acc[i][h+k] += cr*(X[h+k].x*w[i][k].x+X[h+k].y*w[i][k].y);
     }
   }
   if(ok) {
