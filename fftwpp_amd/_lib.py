@@ -61,6 +61,7 @@ SIGNATURES = {
     "fftwpp_amd_event_destroy": (I, [P]),
     # convolution layer
     "fftwpp_amd_conv_create": (I, [ctypes.POINTER(P), I, I, ZP, ZP, ZP, I]),
+    "fftwpp_amd_conv_create_tuned": (I, [ctypes.POINTER(P), I, I, ZP, ZP, I, I, ZP]),
     "fftwpp_amd_conv_destroy": (I, [P]),
     "fftwpp_amd_conv_execute_fused": (I, [P, P, P, P, P, P, P]),
     "fftwpp_amd_conv_set_memory_limit": (I, [P, Z]),

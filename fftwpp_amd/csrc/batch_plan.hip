@@ -656,6 +656,163 @@ void cache_store(const std::string &key, const fftwpp_amd_batch &h)
   if(out) out << os.str();
 }
 
+
+// ------------------------------------------------------------------------
+// Timed sub-FFT sizes of single convolutions (fftwpp_amd_conv_create_tuned).
+// The reference's optimizer times candidate parameters (m, D, I) on the CPU
+// when a plan is made (fftBase::OptBase, convolve.h:164-236, convolve.cc:239-470);
+// on the GPU the kernels fix D and
+// the in-place choice, so the sizes m of the dimensions are what is timed.
+// ------------------------------------------------------------------------
+
+// Smallest 2^a 3^b 5^c 7^d >= n (the sizes the LDS engine transforms).
+size_t next_smooth(size_t n)
+{
+  for(size_t v=std::max<size_t>(n,1);; ++v) {
+    size_t r=v;
+    for(size_t f : {2,3,5,7}) while(r % f == 0) r /= f;
+    if(r == 1) return v;
+  }
+}
+
+// Code of the 1D plan that has dimension d's own rule (fftwpp_amd_conv_create
+// kinds: 0 complex, 1 centred Hermitian, 2 centred complex, 3 real).
+int dim_kind(int hermitian, int dim, int d)
+{
+  if(hermitian == 1) return d == dim-1 ? 1 : 2;
+  if(hermitian == 3) return d == 0 ? 3 : 0;
+  return hermitian;
+}
+
+// Element bytes and stored words of one input of a plan.
+size_t conv_words(int dim, int hermitian, const size_t *L)
+{
+  size_t w=1;
+  for(int d=0; d < dim; ++d)
+    w *= hermitian == 1 && d == dim-1 ? (L[d]+1)/2 : L[d];
+  return w;
+}
+
+size_t conv_elt(int hermitian, int dtype)
+{
+  const size_t c=dtype == FFTWPP_AMD_F64 ? 16 : 8;
+  return hermitian == 3 ? c/2 : c;
+}
+
+std::string conv_key(int dim, int hermitian, const size_t *L, const size_t *M, int dtype)
+{
+  uint64_t k=1469598103934665603ull;
+  int dev=0;
+  hipDeviceProp_t prop;
+  memset(&prop,0,sizeof(prop));
+  if(hipGetDevice(&dev) == hipSuccess) (void) hipGetDeviceProperties(&prop,dev);
+  const int vals[4]={dim,hermitian,dtype,prop.multiProcessorCount};
+  k=fnv1a(k,vals,sizeof(vals));
+  k=fnv1a(k,prop.gcnArchName,strnlen(prop.gcnArchName,sizeof(prop.gcnArchName)));
+  size_t lm[6]={0,0,0,0,0,0};
+  for(int d=0; d < dim; ++d) { lm[d]=L[d]; lm[3+d]=M ? M[d] : 0; }
+  k=fnv1a(k,lm,sizeof(lm));
+  char buf[24];
+  snprintf(buf,sizeof(buf),"c%016llx",(unsigned long long) k);
+  return buf;
+}
+
+// The cached sizes of a convolution key ("m":[...] of its line), if any.
+bool conv_cache_lookup(const std::string &key, std::vector<size_t> &m)
+{
+  const std::string path=cache_path();
+  if(path.empty()) return false;
+  std::lock_guard<std::mutex> lock(cache_mu);
+  std::ifstream in(path);
+  std::string line, want="\"key\":\""+key+"\"";
+  while(std::getline(in,line)) {
+    if(line.find(want) == std::string::npos) continue;
+    const size_t at=line.find("\"m\":[");
+    if(at == std::string::npos) continue;
+    m.clear();
+    const char *c=line.c_str()+at+5;
+    while(*c && *c != ']') {
+      char *end;
+      const unsigned long long v=strtoull(c,&end,10);
+      if(end == c) break;
+      m.push_back((size_t) v);
+      c=*end == ',' ? end+1 : end;
+    }
+    return !m.empty();
+  }
+  return false;
+}
+
+void conv_cache_store(const std::string &key, const size_t *m, int dim, double ms)
+{
+  const std::string path=cache_path();
+  if(path.empty()) return;
+  std::ostringstream os;
+  os << "{\"key\":\"" << key << "\",\"m\":[";
+  for(int d=0; d < dim; ++d) os << (d ? "," : "") << m[d];
+  os << "],\"ms\":" << ms << "}\n";
+  std::lock_guard<std::mutex> lock(cache_mu);
+  const size_t slash=path.rfind('/');
+  if(slash != std::string::npos) {
+    std::string dir=path.substr(0,slash);
+    for(size_t i=1; i <= dir.size(); ++i)
+      if(i == dir.size() || dir[i] == '/') (void) mkdir(dir.substr(0,i).c_str(),0755);
+  }
+  std::ofstream out(path,std::ios::app);
+  if(out) out << os.str();
+}
+
+// The sub-FFT sizes a plan runs with, into chosen (when not null).
+int conv_chosen(const fftwpp_amd_conv *h, int dim, size_t *chosen)
+{
+  if(!chosen) return FFTWPP_AMD_OK;
+  size_t N[3], m[3], q[3];
+  const int rc=fftwpp_amd_conv_geometry(h,N,m,q);
+  if(rc) return rc;
+  for(int d=0; d < dim; ++d) chosen[d]=m[d];
+  return FFTWPP_AMD_OK;
+}
+
+// Candidate sizes of dimension d: the smooth size of every split of L into
+// p <= 8 blocks and the powers of two 16..4096 that cover L in at most 8
+// blocks, kept when a 1D plan of the dimension's rule accepts them, ordered
+// by padded length (then larger m), at most `keep` of them within 1.5 times
+// the shortest padded length.  The library's own choice is always kept.
+std::vector<size_t> dim_candidates(int hermitian, int dim, int d, size_t L, size_t M,
+                                   int dtype, size_t mine, size_t keep)
+{
+  std::vector<size_t> raw;
+  for(size_t p=1; p <= 8; ++p) raw.push_back(next_smooth(ceilquot(L,p)));
+  for(size_t v=16; v <= 4096; v *= 2)
+    if(8*v >= L) raw.push_back(v);
+  raw.push_back(mine);
+  std::sort(raw.begin(),raw.end());
+  raw.erase(std::unique(raw.begin(),raw.end()),raw.end());
+  const size_t lim=std::max<size_t>(fftwpp_amd_max_m(dtype),4096);
+  std::vector<std::pair<size_t,size_t>> ok;           // (N, m)
+  for(size_t m : raw) {
+    if(m > lim) continue;
+    fftwpp_amd_conv *h=nullptr;
+    const size_t Ld=L, Md=M, md=m;
+    if(fftwpp_amd_conv_create(&h,1,dim_kind(hermitian,dim,d),&Ld,&Md,&md,dtype) != FFTWPP_AMD_OK)
+      continue;
+    size_t N[3], mm[3], q[3];
+    if(fftwpp_amd_conv_geometry(h,N,mm,q) == FFTWPP_AMD_OK && mm[0] == m) ok.push_back({N[0],m});
+    fftwpp_amd_conv_destroy(h);
+  }
+  std::sort(ok.begin(),ok.end(),[](const std::pair<size_t,size_t> &a,
+                                   const std::pair<size_t,size_t> &b) {
+    return a.first < b.first || (a.first == b.first && a.second > b.second);
+  });
+  std::vector<size_t> out{mine};
+  for(const auto &c : ok) {
+    if(out.size() >= keep) break;
+    if(c.second == mine || 2*c.first > 3*ok[0].first) continue;
+    out.push_back(c.second);
+  }
+  return out;
+}
+
 } // namespace
 
 extern "C" {
@@ -907,6 +1064,135 @@ int fftwpp_amd_batch_execute(fftwpp_amd_batch *h, void *out, const void *f,
   }
   h->graphs.push_back(gr);
   return FFTWPP_AMD_OK;
+}
+
+
+/* Plan creation with timed sub-FFT sizes: see fftwpp_amd.h. */
+int fftwpp_amd_conv_create_tuned(fftwpp_amd_conv **out, int dim, int hermitian,
+                                 const size_t *L, const size_t *M, int dtype,
+                                 int flags, size_t *chosen)
+{
+  if(!out || !L) return bfail(FFTWPP_AMD_ERR_ARG,"null argument");
+  *out=nullptr;
+  if(dim < 1 || dim > 3) return bfail(FFTWPP_AMD_ERR_ARG,"dim must be 1, 2 or 3");
+  const std::string key=conv_key(dim,hermitian,L,M,dtype);
+  std::vector<size_t> cached;
+  if(!(flags & FFTWPP_AMD_TUNE_RETIME) && conv_cache_lookup(key,cached) &&
+     (int) cached.size() == dim &&
+     fftwpp_amd_conv_create(out,dim,hermitian,L,M,cached.data(),dtype) == FFTWPP_AMD_OK)
+    return conv_chosen(*out,dim,chosen);
+  // the library's own choice: the first candidate and the plan to beat
+  fftwpp_amd_conv *base=nullptr;
+  int rc=fftwpp_amd_conv_create(&base,dim,hermitian,L,M,nullptr,dtype);
+  if(rc) return rc;
+  size_t N0[3], m0[3], q0[3];
+  rc=fftwpp_amd_conv_geometry(base,N0,m0,q0);
+  fftwpp_amd_conv_destroy(base);
+  if(rc) return rc;
+  const size_t keep=dim == 1 ? 6 : (dim == 2 ? 4 : 3);
+  std::vector<std::vector<size_t>> cand(dim);
+  for(int d=0; d < dim; ++d) {
+    cand[d]=dim_candidates(hermitian,dim,d,L[d],M ? M[d] : 0,dtype,m0[d],keep);
+    if(getenv("FFTWPP_AMD_TUNE_VERBOSE")) {
+      fprintf(stderr,"fftwpp_amd tune: dimension %d candidates",d);
+      for(size_t m : cand[d]) fprintf(stderr," %zu",m);
+      fprintf(stderr,"\n");
+    }
+  }
+  // device inputs (finite bit pattern) shared by every candidate
+  const size_t bytes=conv_words(dim,hermitian,L)*conv_elt(hermitian,dtype);
+  void *a=nullptr, *b=nullptr, *o=nullptr;
+  hipStream_t s=nullptr;
+  hipEvent_t e0=nullptr, e1=nullptr;
+  rc=bhip(hipMalloc(&a,bytes),"tune inputs");
+  if(!rc) rc=bhip(hipMalloc(&b,bytes),"tune inputs");
+  if(!rc) rc=bhip(hipMalloc(&o,bytes),"tune output");
+  if(!rc) rc=bhip(hipMemset(a,0x3e,bytes),"tune inputs");
+  if(!rc) rc=bhip(hipMemset(b,0x3d,bytes),"tune inputs");
+  if(!rc) rc=bhip(hipStreamCreateWithFlags(&s,hipStreamNonBlocking),"tune stream");
+  if(!rc) rc=bhip(hipEventCreate(&e0),"tune event");
+  if(!rc) rc=bhip(hipEventCreate(&e1),"tune event");
+  // average ms of one execution of the plan with sizes m (nullptr: the
+  // library's choice), or -1 when it is refused or slower than 3x `best`:
+  // one untimed run (allocates the work buffers), one timed, then
+  // repetitions filling about 20 ms, at least three
+  const bool verbose=getenv("FFTWPP_AMD_TUNE_VERBOSE") != nullptr;
+  auto show=[&](const size_t *m, const char *what, double ms) {
+    if(!verbose) return;
+    fprintf(stderr,"fftwpp_amd tune: m =");
+    for(int d=0; d < dim; ++d) fprintf(stderr," %zu",m ? m[d] : m0[d]);
+    if(ms >= 0) fprintf(stderr,"%s: %.4f ms\n",what,ms);
+    else fprintf(stderr,"%s: %s\n",what,fftwpp_amd_last_error());
+  };
+  auto time_plan=[&](const size_t *m, double best) -> double {
+    fftwpp_amd_conv *h=nullptr;
+    if(fftwpp_amd_conv_create(&h,dim,hermitian,L,M,m,dtype) != FFTWPP_AMD_OK) {
+      (void) hipGetLastError();
+      show(m," refused",-1);
+      return -1;
+    }
+    float ms=0;
+    double avg=-1;
+    bool good=fftwpp_amd_conv_execute_out(h,o,a,b,s) == FFTWPP_AMD_OK &&
+      hipEventRecord(e0,s) == hipSuccess &&
+      fftwpp_amd_conv_execute_out(h,o,a,b,s) == FFTWPP_AMD_OK &&
+      hipEventRecord(e1,s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+      hipEventElapsedTime(&ms,e0,e1) == hipSuccess;
+    if(good && best >= 0 && ms >= 3*best) show(m," skipped, warm-up run",ms);
+    if(good && (best < 0 || ms < 3*best)) {
+      const int reps=std::max(3,std::min(50,(int) (20.0/std::max(ms,0.001f))));
+      good=hipEventRecord(e0,s) == hipSuccess;
+      for(int i=0; good && i < reps; ++i)
+        good=fftwpp_amd_conv_execute_out(h,o,a,b,s) == FFTWPP_AMD_OK;
+      good=good && hipEventRecord(e1,s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+        hipEventElapsedTime(&ms,e0,e1) == hipSuccess;
+      if(good) avg=ms/reps;
+    }
+    (void) hipStreamSynchronize(s);
+    (void) hipGetLastError();                 // a refused candidate leaves no error behind
+    fftwpp_amd_conv_destroy(h);
+    if(avg >= 0) show(m,m ? "" : " (library choice)",avg);
+    return avg;
+  };
+  double best=rc ? -1 : time_plan(nullptr,-1);
+  bool chose=false;                           // a candidate beat the library's choice
+  size_t bestm[3]={m0[0],m0[1],m0[2]};
+  size_t idx[3]={0,0,0};
+  for(bool more=!rc; more; ) {
+    size_t m[3]={0,0,0};
+    bool same=true;
+    for(int d=0; d < dim; ++d) {
+      m[d]=cand[d][idx[d]];
+      same=same && m[d] == m0[d];
+    }
+    if(!same) {
+      const double t=time_plan(m,best);
+      if(t >= 0 && (best < 0 || t < best)) {
+        best=t;
+        chose=true;
+        for(int d=0; d < dim; ++d) bestm[d]=m[d];
+      }
+    }
+    more=false;
+    for(int d=dim-1; d >= 0; --d) {
+      if(++idx[d] < cand[d].size()) { more=true; break; }
+      idx[d]=0;
+    }
+  }
+  if(e0) (void) hipEventDestroy(e0);
+  if(e1) (void) hipEventDestroy(e1);
+  if(s) (void) hipStreamDestroy(s);
+  if(a) (void) hipFree(a);
+  if(b) (void) hipFree(b);
+  if(o) (void) hipFree(o);
+  if(rc) return rc;
+  if(best < 0) return bfail(FFTWPP_AMD_ERR_HIP,"convolution tuning found no working plan");
+  rc=fftwpp_amd_conv_create(out,dim,hermitian,L,M,chose ? bestm : nullptr,dtype);
+  if(rc) return rc;
+  // zeros record that the library's own choice was fastest
+  const size_t none[3]={0,0,0};
+  conv_cache_store(key,chose ? bestm : none,dim,best);
+  return conv_chosen(*out,dim,chosen);
 }
 
 } // extern "C"

@@ -44,10 +44,16 @@ class DeviceConvolution:
     (Lx,Ly,Lz) real data (fftPadReal outer dimension, tests/hybridconvr*.cc;
     dtype torch.float64 or torch.float32).  M and m optionally give the
     minimum padded sizes and forced sub-FFT sizes per dimension.
+
+    tune=True times candidate sub-FFT sizes on the current device when the
+    plan is made (fftwpp_amd_conv_create_tuned: the reference's optimizer,
+    fftBase::OptBase, convolve.h:164-236, on the GPU) and keeps the fastest;
+    the choice is cached, so a later plan of the same problem takes no
+    timing (retune=True times again).  It excludes m.
     """
 
     def __init__(self, shape, kind="complex", dtype=torch.complex128,
-                 M=None, m=None, lengths=None):
+                 M=None, m=None, lengths=None, tune=False, retune=False):
         if isinstance(shape, int):
             shape = (shape,)
         self.shape = tuple(int(s) for s in shape)
@@ -60,10 +66,18 @@ class DeviceConvolution:
                 lengths[-1] = 2 * self.shape[-1] - 1
         self.lengths = tuple(int(v) for v in lengths)
         h = ctypes.c_void_p()
-        check(lib().fftwpp_amd_conv_create(
-            ctypes.byref(h), self.dim, KINDS[kind], sizes(self.lengths),
-            sizes(M) if M is not None else None,
-            sizes(m) if m is not None else None, _dtype_code(dtype, kind)))
+        if tune:
+            if m is not None:
+                raise ValueError("tune=True chooses the sub-FFT sizes: give no m")
+            check(lib().fftwpp_amd_conv_create_tuned(
+                ctypes.byref(h), self.dim, KINDS[kind], sizes(self.lengths),
+                sizes(M) if M is not None else None, _dtype_code(dtype, kind),
+                1 if retune else 0, None))
+        else:
+            check(lib().fftwpp_amd_conv_create(
+                ctypes.byref(h), self.dim, KINDS[kind], sizes(self.lengths),
+                sizes(M) if M is not None else None,
+                sizes(m) if m is not None else None, _dtype_code(dtype, kind)))
         self._h = h
 
     def __del__(self):

@@ -302,6 +302,26 @@ int fftwpp_amd_conv_create(fftwpp_amd_conv **h, int dim, int hermitian,
                            const size_t *m, int dtype);
 int fftwpp_amd_conv_destroy(fftwpp_amd_conv *h);
 
+/* fftwpp_amd_conv_create with timed sub-FFT sizes: the GPU counterpart of the
+ * reference's optimizer, which times candidate parameters on the CPU when a
+ * plan is made without forced ones (fftBase::OptBase, convolve.h:164-236, convolve.cc:239-470).
+ * The library's own choice is timed first; then, per dimension, the sizes of
+ * every split of L into p <= 8 blocks (smooth sizes) and the covering powers
+ * of two that a plan of that dimension accepts, the shortest padded lengths
+ * first (within 1.5 times the shortest; 6, 4 or 3 per dimension for 1D, 2D,
+ * 3D); every combination runs on device data and the fastest average wins.
+ * The result is kept in the tuning file of the batch planner (see
+ * fftwpp_amd_batch_create) under a key of (dim, kind, L, M, precision,
+ * device architecture and CU count), so a later creation of the same plan
+ * takes no timing unless flags has FFTWPP_AMD_TUNE_RETIME.  chosen (may be
+ * NULL) receives the sub-FFT size of each dimension.  Tuning allocates and
+ * runs on the current device, synchronously; with FFTWPP_AMD_TUNE_VERBOSE set
+ * in the environment every candidate and its time go to stderr. */
+#define FFTWPP_AMD_TUNE_RETIME 1
+int fftwpp_amd_conv_create_tuned(fftwpp_amd_conv **h, int dim, int hermitian,
+                                 const size_t *L, const size_t *M, int dtype,
+                                 int flags, size_t *chosen);
+
 /* Memory-bounded execution of 2D and 3D plans (the reference bounds its
  * buffers by processing D residues per pass, convolve.h:445-451, and loops
  * over the outer residues, convolve.h:1475-1535, 1812-1879).  The plan's
