@@ -56,7 +56,7 @@ def parse():
                     help="skip the counter passes (roofline.traffic null)")
     ap.add_argument("--batched-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--workload", default=None,
-                    choices=["hconv2d", "real3d", "conv1d", "user_multiplier"],
+                    choices=["hconv2d", "real3d", "conv1d", "user_multiplier", "batched"],
                     help="run one secondary workload alone and print its JSON (for "
                          "profiling that workload's kernels)")
     ap.add_argument("--traffic-child", default=None,
@@ -416,6 +416,35 @@ def bench_batched(torch, dev, steps, warmup, per_call=True):
     return res
 
 
+def bench_batched_replay(torch, dev, steps, warmup):
+    """The batched workload's tuned graph plans alone (FP64 and FP32), built
+    through the tuning cache of the environment (FFTWPP_AMD_TUNE_CACHE or the
+    default file): a second run finds its entries and times nothing at plan
+    creation, so a kernel trace of it holds only the replayed batches."""
+    import numpy as np
+    from fftwpp_amd.batch import BatchedConvolution
+    Ls = batched_lengths()
+    total = int(Ls.sum())
+    rng = np.random.Generator(np.random.PCG64(SEED + 4))
+    fh = rng.uniform(-1, 1, total) + 1j * rng.uniform(-1, 1, total)
+    gh = rng.uniform(-1, 1, total) + 1j * rng.uniform(-1, 1, total)
+    res = {"workload": "4096 1D complex convolutions, L log-uniform in [64,16384], M = 2L "
+                       "(tuned graph replay only)"}
+    for dt, key in [(torch.complex128, "f64"), (torch.complex64, "f32")]:
+        f = torch.tensor(fh, dtype=dt, device=dev)
+        g = torch.tensor(gh, dtype=dt, device=dev)
+        out = torch.empty_like(f)
+        t0 = time.perf_counter()
+        plan = BatchedConvolution(Ls, M=2 * Ls, dtype=dt, tune=True, graph=True)
+        create_s = time.perf_counter() - t0
+        ms = _time_calls(torch, lambda: plan(f, g, out), steps, warmup)
+        res[key] = {"ms_per_batch": round(ms, 4), "create_s": round(create_s, 3),
+                    "classes": plan.describe().strip().splitlines()}
+        del plan, f, g, out
+        torch.cuda.empty_cache()
+    return res
+
+
 def bench_conv1d(torch, dev, steps):
     """One 1D convolution L=1024, M=2048, FP64: latency per call."""
     import numpy as np
@@ -658,6 +687,7 @@ def main():
         fn = {"hconv2d": lambda: bench_hconv2d(torch, dev, args.steps, args.warmup),
               "real3d": lambda: bench_real3d(torch, dev, args.steps, args.warmup),
               "conv1d": lambda: bench_conv1d(torch, dev, args.steps),
+              "batched": lambda: bench_batched_replay(torch, dev, args.steps, args.warmup),
               "user_multiplier": lambda: bench_user_multiplier(torch, dev, min(args.steps, 5),
                                                                1)}[args.workload]
         print(json.dumps(fn()), flush=True)
