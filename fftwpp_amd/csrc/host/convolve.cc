@@ -850,10 +850,11 @@ void generalConvolution(fftBase *const *dims, size_t nd, size_t A, size_t B,
 // of `words` complex words from `offset`; host arrays are staged through
 // device buffers.  The multiplier sees the padded device arrays and finds
 // their layout and stream in `indices`.
-static void userConvolution(fftBase *const *dims, size_t nd, size_t A, size_t B,
-                            multiplier *mult, Indices &indices, size_t threads,
-                            DeviceWork &work, Complex **f, size_t offset,
-                            size_t words, double scale)
+// generalConvolution on f[0..max(A,B)-1], host or device arrays of `words`
+// complex words from `offset`; host arrays are staged through device buffers.
+static void stagedGeneral(fftBase *const *dims, size_t nd, size_t A, size_t B,
+                          DeviceWork &work, Complex **f, size_t offset, size_t words,
+                          double scale, const DeviceMultiplier &mult)
 {
   const size_t na=std::max(A,B);
   const size_t bytes=words*sizeof(Complex);
@@ -866,19 +867,47 @@ static void userConvolution(fftBase *const *dims, size_t nd, size_t A, size_t B,
     if(host[a] && a < A)
       check(fftwpp_amd_memcpy(d[a],p,bytes,NULL),"copy in");
   }
-  generalConvolution(dims,nd,A,B,work,FFTWPP_AMD_F64,d.data(),scale,NULL,
-                     [&](void **F, size_t n, const fftwpp_amd_layout &layout) {
-                       indices.layout=&layout;
-                       indices.stream=NULL;
-                       indices.r=0;
-                       indices.offset=0;
-                       mult((Complex **) F,n,&indices,threads);
-                       indices.layout=NULL;
-                     });
+  generalConvolution(dims,nd,A,B,work,FFTWPP_AMD_F64,d.data(),scale,NULL,mult);
   for(size_t b=0; b < B; ++b)
     if(host[b])
       check(fftwpp_amd_memcpy(f[b]+offset,d[b],bytes,NULL),"copy out");
   check(fftwpp_amd_synchronize(NULL),"synchronize");
+}
+
+static void userConvolution(fftBase *const *dims, size_t nd, size_t A, size_t B,
+                            multiplier *mult, Indices &indices, size_t threads,
+                            DeviceWork &work, Complex **f, size_t offset,
+                            size_t words, double scale)
+{
+  stagedGeneral(dims,nd,A,B,work,f,offset,words,scale,
+                [&](void **F, size_t n, const fftwpp_amd_layout &layout) {
+                  indices.layout=&layout;
+                  indices.stream=NULL;
+                  indices.r=0;
+                  indices.offset=0;
+                  mult((Complex **) F,n,&indices,threads);
+                  indices.layout=NULL;
+                });
+}
+
+// A library multiplier with A != 2 or B != 1 (the reference runs any A and B
+// through its residue loop, convolve.cc:7513-7575): all A inputs are
+// transformed, multBinary / multcorrelation act on F[0] and F[1] as in
+// convolve.cc:33-110 (multNone on none), and the first B arrays are
+// transformed back.  Complex and centred dimensions, as user multipliers.
+static void libraryGeneral(fftBase *const *dims, size_t nd, size_t A, size_t B, int code,
+                           DeviceWork &work, Complex **f, size_t offset, size_t words,
+                           double scale)
+{
+  if(code == FFTWPP_AMD_MULT_REAL_BINARY)
+    throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"realMultBinary with A != 2 or B != 1 (Hermitian data)");
+  if(code != FFTWPP_AMD_MULT_NONE && A < 2)
+    throw Error(FFTWPP_AMD_ERR_ARG,"a binary multiplier needs A >= 2 inputs");
+  stagedGeneral(dims,nd,A,B,work,f,offset,words,scale,
+                [&](void **F, size_t n, const fftwpp_amd_layout &) {
+                  if(code != FFTWPP_AMD_MULT_NONE)
+                    check(fftwpp_amd_multiply(FFTWPP_AMD_F64,code,F[0],F[1],n,NULL),"multiply");
+                });
 }
 
 // ------------------------------------------------------------------------
@@ -1124,12 +1153,15 @@ void Convolution::execComplex(void *out, const void *f0, const void *f1, bool no
 
 void Convolution::run(Complex **f, size_t offset, bool normalize)
 {
-  if(multCode < 0) {
+  if(multCode < 0 || A != 2 || B != 1) {
     fftBase *dims[1]={fft};
     if(fft->C != 1)
       throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"1D Convolution with C > 1");
-    userConvolution(dims,1,A,B,mult,indices,threads,work,f,offset,arraySize(),
-                    normalize ? scale : 1.0);
+    if(multCode < 0)
+      userConvolution(dims,1,A,B,mult,indices,threads,work,f,offset,arraySize(),
+                      normalize ? scale : 1.0);
+    else
+      libraryGeneral(dims,1,A,B,multCode,work,f,offset,arraySize(),normalize ? scale : 1.0);
     return;
   }
   staged(work,f,offset,arraySize(),2,fft->realInput() ? sizeof(double) : sizeof(Complex),
@@ -1332,12 +1364,15 @@ size_t Convolution2::residueBlock(int dtype, bool real) const
 
 void Convolution2::run(Complex **f, size_t offset, bool normalize)
 {
-  if(multCode < 0) {
+  if(multCode < 0 || A != 2 || B != 1) {
     fftBase *dims[2]={fftx,ffty};
-    if(Sx != Ly)
-      throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"user multipliers take contiguous arrays");
-    userConvolution(dims,2,A,B,mult,indices,threads,work,f,offset,arraySize(),
-                    normalize ? scale : 1.0);
+    if(Sx != Ly || ffty->C != 1)
+      throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"user multipliers, and A != 2 or B != 1, take contiguous arrays of one column");
+    if(multCode < 0)
+      userConvolution(dims,2,A,B,mult,indices,threads,work,f,offset,arraySize(),
+                      normalize ? scale : 1.0);
+    else
+      libraryGeneral(dims,2,A,B,multCode,work,f,offset,arraySize(),normalize ? scale : 1.0);
     return;
   }
   staged(work,f,offset,arraySize(),2,fftx->realInput() ? sizeof(double) : sizeof(Complex),
@@ -1544,12 +1579,15 @@ void Convolution3::execComplex(void *out, const void *f0, const void *f1, bool n
 
 void Convolution3::run(Complex **f, size_t offset, bool normalize)
 {
-  if(multCode < 0) {
+  if(multCode < 0 || A != 2 || B != 1) {
     fftBase *dims[3]={fftx,ffty,fftz};
-    if(Sy != Lz || Sx != Ly*Lz)
-      throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"user multipliers take contiguous arrays");
-    userConvolution(dims,3,A,B,mult,indices,threads,work,f,offset,arraySize(),
-                    normalize ? scale : 1.0);
+    if(Sy != Lz || Sx != Ly*Lz || fftz->C != 1)
+      throw Error(FFTWPP_AMD_ERR_UNSUPPORTED,"user multipliers, and A != 2 or B != 1, take contiguous arrays of one column");
+    if(multCode < 0)
+      userConvolution(dims,3,A,B,mult,indices,threads,work,f,offset,arraySize(),
+                      normalize ? scale : 1.0);
+    else
+      libraryGeneral(dims,3,A,B,multCode,work,f,offset,arraySize(),normalize ? scale : 1.0);
     return;
   }
   staged(work,f,offset,arraySize(),2,fftx->realInput() ? sizeof(double) : sizeof(Complex),

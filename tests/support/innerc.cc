@@ -77,4 +77,53 @@ int testface_conv3_inner(size_t Lx, size_t Ly, size_t Lz, size_t C, size_t mz,
   return 0;
 }
 
+// A library multiplier (code 0 multNone, 1 multBinary, 3 multcorrelation)
+// with A inputs and B outputs through the C++ classes: dim = 1..3, lengths
+// L[0..dim-1] (contiguous, non-centred, M = 2L - 1 per dimension); arrays
+// holds max(A, B) arrays of prod(L) complex words back to back, the first A
+// the inputs; the first B are overwritten with the outputs.
+int testface_conv_ab(int dim, const size_t *L, size_t A, size_t B, int code, double *arrays)
+{
+  try {
+    multiplier *mult=code == 0 ? multNone : (code == 1 ? multBinary : multcorrelation);
+    size_t n=1;
+    for(int d=0; d < dim; ++d) n *= L[d];
+    const size_t na=A > B ? A : B;
+    std::vector<Complex *> F(na);
+    for(size_t a=0; a < na; ++a) {
+      F[a]=utils::ComplexAlign(n);
+      memcpy(F[a],arrays+2*a*n,n*sizeof(Complex));
+    }
+    if(dim == 1) {
+      Application app(A,B,mult,1);
+      fftPad fft(L[0],2*L[0]-1,app);
+      Convolution conv(&fft);
+      conv.convolve(F.data());
+    } else if(dim == 2) {
+      Application appx(A,B,multNone,1);
+      fftPad fftx(L[0],2*L[0]-1,appx,L[1],L[1]);
+      Application appy(A,B,mult,appx);
+      fftPad ffty(L[1],2*L[1]-1,appy);
+      Convolution2 conv(&fftx,&ffty);
+      conv.convolve(F.data());
+    } else {
+      Application appx(A,B,multNone,1);
+      fftPad fftx(L[0],2*L[0]-1,appx,L[1]*L[2],L[1]*L[2]);
+      Application appy(A,B,multNone,appx);
+      fftPad ffty(L[1],2*L[1]-1,appy,L[2],L[2]);
+      Application appz(A,B,mult,appx);
+      fftPad fftz(L[2],2*L[2]-1,appz);
+      Convolution3 conv(&fftx,&ffty,&fftz);
+      conv.convolve(F.data());
+    }
+    for(size_t a=0; a < na; ++a) {
+      if(a < B) memcpy(arrays+2*a*n,F[a],n*sizeof(Complex));
+      utils::deleteAlign(F[a]);
+    }
+  } catch(const Error &e) {
+    return e.code;
+  }
+  return 0;
+}
+
 } // extern "C"

@@ -59,3 +59,38 @@ def test_conv3_inner(gpu, oracle, Lx, Ly, Lz, C, mz):
     for c in range(C):
         want = oracle.explicit_conv(f[..., c], g[..., c], (2 * Lx, 2 * Ly, 2 * Lz))
         assert oracle.rel_l2(h[..., c], want) < 1e-12, (Lx, Ly, Lz, C, mz, c)
+
+
+@pytest.mark.parametrize("shape,A,B,code", [((40,), 3, 2, 1), ((33,), 2, 2, 0), ((30,), 4, 1, 3),
+                                            ((8, 7), 3, 1, 3), ((6, 9), 3, 3, 1),
+                                            ((4, 5, 6), 4, 3, 1)])
+def test_library_multiplier_any_AB(gpu, oracle, shape, A, B, code):
+    """multNone / multBinary / multcorrelation with A != 2 or B != 1 through the
+    C++ classes (the reference's residue loop takes any A and B,
+    convolve.cc:7513-7575): all A inputs transformed, the multiplier on F[0]
+    and F[1] (convolve.cc:26-110), the first B arrays transformed back; against
+    explicit padding with the same products, FP64 <= 1e-12."""
+    rng = np.random.default_rng(sum(shape) + 10 * A + B)
+    na = max(A, B)
+    arrays = [_rnd(rng, shape) for _ in range(na)]
+    buf = np.ascontiguousarray(np.stack(arrays))
+    fn = getattr(ctypes.CDLL(TESTLIB), "testface_conv_ab")
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_size_t), ctypes.c_size_t,
+                   ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    L = (ctypes.c_size_t * 3)(*(list(shape) + [0] * (3 - len(shape))))
+    assert fn(len(shape), L, A, B, code, buf.ctypes.data) == 0
+
+    def op(Fs, x):
+        P = list(Fs)
+        if code == 1:
+            P[0] = Fs[0] * Fs[1]
+        elif code == 3:
+            P[0] = Fs[0] * np.conj(Fs[1])
+        return P[:B]
+    want = oracle.explicit_general(arrays[:A] + [np.zeros(shape)] * (B - A if B > A else 0),
+                                   tuple(2 * n for n in shape), op)
+    for b in range(B):
+        assert oracle.rel_l2(buf[b], want[b]) < 1e-12, (shape, A, B, code, b)
+    for a in range(B, na):
+        assert np.array_equal(buf[a], arrays[a])
