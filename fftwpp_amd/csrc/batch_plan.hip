@@ -791,7 +791,7 @@ std::vector<size_t> dim_candidates(int hermitian, int dim, int d, size_t L, size
   const size_t lim=std::max<size_t>(fftwpp_amd_max_m(dtype),4096);
   std::vector<std::pair<size_t,size_t>> ok;           // (N, m)
   for(size_t m : raw) {
-    if(m > lim) continue;
+    if(m < 2 || m > lim) continue;              // m = 1 means "choose" to Application
     fftwpp_amd_conv *h=nullptr;
     const size_t Ld=L, Md=M, md=m;
     if(fftwpp_amd_conv_create(&h,1,dim_kind(hermitian,dim,d),&Ld,&Md,&md,dtype) != FFTWPP_AMD_OK)

@@ -24,6 +24,8 @@ CASES = [
     ((63, 32), "hermitian", [63, 63], "c128"),
     ((12, 10, 24), "complex", None, "c128"),
     ((600,), "real", None, "f64"),
+    ((5,), "complex", None, "c128"),
+    ((24, 40), "real", None, "f64"),
 ]
 
 
