@@ -40,8 +40,8 @@ def progress(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp32", action="store_true")
@@ -106,19 +106,30 @@ CONV_UNITS = 15  # (A+B)(1 + 2 rho_x), A=2, B=1, rho_x=2
 
 
 def time_plan(torch, plan, out, a, b, steps, warmup, stream):
+    """Milliseconds per step of `steps` back-to-back executions (after
+    `warmup` untimed ones), bracketed by synchronisations; then, untimed,
+    the same number of executions with a HIP event after every kernel (on
+    the plan's stream) for the per-stage durations.  The event pass waits
+    for each execution, so it is kept out of the timed steps."""
     from fftwpp_amd import _lib
     L = _lib.lib()
-    ms = (ctypes.c_float * 32)()
-    n = ctypes.c_int()
     for _ in range(warmup):
-        _lib.check(L.fftwpp_amd_conv_execute_timed(plan._h, out.data_ptr(), a.data_ptr(),
-                                                   b.data_ptr(), stream, ms, 32, ctypes.byref(n)))
+        _lib.check(L.fftwpp_amd_conv_execute_out(plan._h, out.data_ptr(), a.data_ptr(),
+                                                 b.data_ptr(), stream))
     torch.cuda.synchronize()
-    sums = None
-    names = None
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     start.record()
+    for _ in range(steps):
+        _lib.check(L.fftwpp_amd_conv_execute_out(plan._h, out.data_ptr(), a.data_ptr(),
+                                                 b.data_ptr(), stream))
+    stop.record()
+    torch.cuda.synchronize()
+    total_ms = start.elapsed_time(stop)
+    ms = (ctypes.c_float * 32)()
+    n = ctypes.c_int()
+    sums = None
+    names = None
     for _ in range(steps):
         _lib.check(L.fftwpp_amd_conv_execute_timed(plan._h, out.data_ptr(), a.data_ptr(),
                                                    b.data_ptr(), stream, ms, 32, ctypes.byref(n)))
@@ -128,9 +139,7 @@ def time_plan(torch, plan, out, a, b, steps, warmup, stream):
             names = [L.fftwpp_amd_conv_stage_name(plan._h, i).decode() for i in range(k)]
         for i in range(k):
             sums[i] += ms[i]
-    stop.record()
     torch.cuda.synchronize()
-    total_ms = start.elapsed_time(stop)
     stages = {names[i]: sums[i] / steps for i in range(len(names))}
     return total_ms / steps, stages
 
