@@ -163,6 +163,77 @@ __device__ __forceinline__ void batch_fold_loads(const cplx<T> *f, const cplx<T>
 // transforms and the FFT's temporaries then fit two waves per SIMD).
 template<typename T> struct BatchV { static constexpr int V=sizeof(T) == 8 ? 8 : 16; };
 
+// One work unit of the FP64 sequence mode: residue r of the item at fi, gi
+// of length L, its sequence Z_r written to z (see batch_reg_kernel).  With a
+// slot of whole waves the fold reads through buffer resources over each block
+// of M words (the base advanced per block, so offsets stay below M words for
+// any L; fi must be wave-uniform); otherwise through pointers.  The forward
+// transforms of f and g run with their butterflies interleaved.
+template<typename T, int M>
+__device__ __forceinline__ void batch_zm_unit(const cplx<T> *zN, const cplx<T> *fi,
+                                              const cplx<T> *gi, int L, int r, int q,
+                                              const cplx<T> *tw, const cplx<T> *tq, T *xb,
+                                              int t, int mult, cplx<T> *z)
+{
+  constexpr int V=BatchV<T>::V;
+  constexpr int TPS=RPlan<M,V>::TPS;
+  constexpr bool BUF=sizeof(T) == 8 && TPS >= 64;
+  cplx<T> X[V], Y[V];
+#pragma unroll
+  for(int k=0; k < V; ++k) X[k]=Y[k]=mk((T) 0,(T) 0);
+  for(int jb=0, eb=0; jb < L; jb += M) {
+    const cplx<T> wb=zN[eb*M];                 // w_q^{b r}, eb = b r mod q
+    const int rem=min(L-jb,M);                 // words of this block
+    cplx<T> a[V], b[V];
+    if constexpr(BUF) {
+      const __amdgpu_buffer_rsrc_t rf=buf_rsrc<T>(fi+jb,(size_t) rem);
+      const __amdgpu_buffer_rsrc_t rg=buf_rsrc<T>(gi+jb,(size_t) rem);
+#pragma unroll
+      for(int k=0; k < V; ++k) {
+        const unsigned o=(unsigned) min(t+TPS*k,rem-1)*(unsigned) sizeof(cplx<T>);
+        a[k]=ld_buf<T>(rf,o,0);
+        b[k]=ld_buf<T>(rg,o,0);
+      }
+    } else {
+#pragma unroll
+      for(int k=0; k < V; ++k) {
+        const int j=jb+min(t+TPS*k,rem-1);
+        a[k]=fi[j];
+        b[k]=gi[j];
+      }
+    }
+#pragma unroll
+    for(int k=0; k < V; ++k) {
+      const cplx<T> w=t+TPS*k < rem ? wb : mk((T) 0,(T) 0);
+      X[k]=X[k]+w*a[k];
+      Y[k]=Y[k]+w*b[k];
+    }
+    eb += r;
+    if(eb >= q) eb -= q;
+  }
+  const cplx<T> bt=zN[t*r];
+  const cplx<T> *tr=tq+r*V;
+  if(r) {
+#pragma unroll
+    for(int k=0; k < V; ++k) {
+      const cplx<T> w=bt*tr[k];               // w_N^{sigma r}
+      X[k]=w*X[k];
+      Y[k]=w*Y[k];
+    }
+  }
+  fftr_pair<T,M,V,1>(X,Y,xb,tw,t);
+#pragma unroll
+  for(int k=0; k < V; ++k)
+    X[k]=mult == 1 ? X[k]*Y[k] : (mult == 3 ? mulc(X[k],Y[k]) : X[k]);
+  fftr<T,M,V,-1,true>(X,xb,tw,t);
+  if(r) {
+#pragma unroll
+    for(int k=0; k < V; ++k) X[k]=mulc(X[k],bt*tr[k]);
+  }
+#pragma unroll
+  for(int k=0; k < V; ++k) z[t+TPS*k]=X[k];
+}
+
 // The same work units with the register-resident FFT (fftr.h) of a
 // power-of-two sub-FFT size M: a block has blockDim.x/TPS sequence slots of
 // TPS = M/V threads and each slot convolves one work unit (item, residue
@@ -205,7 +276,39 @@ batch_reg_kernel(DimArgs d, const BatchItem *items, int64_t count,
   __syncthreads();
   const int slot=threadIdx.x/TPS, t=threadIdx.x-slot*TPS;
   T *xb=reinterpret_cast<T *>(tw+M)+slot*2*P::PAD;
-  constexpr bool zm=ZM;                  // the sequence mode (the host passes D = 0)
+  if constexpr(ZM && sizeof(T) == 8) {
+    // the FP64 sequence mode (the host passes D = 0): units (item,
+    // residue), dealt to XCDs as the residue groups below
+    const int qi=(int) d.q;
+    const int64_t units=count*qi;
+    for(int64_t ub=(int64_t) blockIdx.x*slots; ub < units; ub += (int64_t) gridDim.x*slots) {
+      const int64_t unit=ub+slot;
+      if(unit >= units) break;          // only when slots > 1 (TPS <= 64)
+      int64_t ii;
+      int r;
+      if(qi > 1 && slots == 1) {
+        const int64_t chunk=8*(int64_t) qi, c=unit/chunk, full=count/8;
+        const int64_t u=unit-c*chunk;
+        const int64_t W=c < full ? 8 : count-8*full;
+        ii=8*c+u % W;
+        r=(int) (u/W);
+      } else {
+        ii=unit/qi;
+        r=(int) (unit-ii*qi);
+      }
+      if constexpr(TPS >= 64) {
+        // a slot spans whole waves: its unit, and the item's base, are
+        // uniform over the wave
+        ii=__builtin_amdgcn_readfirstlane((int) ii);
+        r=__builtin_amdgcn_readfirstlane(r);
+      }
+      const BatchItem it=items[ii];
+      cplx<T> *z=part+(ii*qi+r)*(int64_t) M;
+      batch_zm_unit<T,M>(zN,f+it.off,g+it.off,(int) it.L,r,qi,tw,tq,xb,t,mult,z);
+    }
+    return;
+  }
+  constexpr bool zm=ZM;                  // the FP32 sequence mode (the host passes D = 0)
   if(zm) D=1;
   const int64_t G=(d.q+D-1)/D;
   const int64_t units=count*G;

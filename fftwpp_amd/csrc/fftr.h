@@ -200,6 +200,82 @@ __device__ __forceinline__ void fftr_stage(cplx<T> *v, T *xb, const cplx<T> *tw,
   }
 }
 
+// Stage s of two transforms of the same size and sign at once (one-round
+// exchanges, the default index map): the butterflies of both sequences are
+// independent work for the same waves, then each is exchanged in turn through
+// the one area (the same barriers as two fftr calls).
+template<typename T, int M, int V, int S, int s>
+__device__ __forceinline__ void fftr_pair_stage(cplx<T> *X, cplx<T> *Y, T *xb,
+                                                const cplx<T> *tw, int t)
+{
+  typedef RPlan<M,V> P;
+  typedef RowXI<T,M,V> XI;
+  constexpr int TPS=P::TPS;
+  constexpr int R=P::radix(s), NB=V/R, Ns=P::ns(s), OFF=P::off(s);
+#pragma unroll
+  for(int b=0; b < NB; ++b) {
+    const int j=t+TPS*b;
+    const int kp=j & (Ns-1);
+    cplx<T> u[R], v[R];
+#pragma unroll
+    for(int tp=0; tp < R; ++tp) {
+      u[tp]=X[b+tp*NB];
+      v[tp]=Y[b+tp*NB];
+    }
+    if constexpr(s > 0) {
+      const cplx<T> *w=tw+OFF+kp;
+#pragma unroll
+      for(int tp=1; tp < R; ++tp) {
+        const cplx<T> z=w[(tp-1)*Ns];
+        u[tp]=S > 0 ? u[tp]*z : mulc(u[tp],z);
+        v[tp]=S > 0 ? v[tp]*z : mulc(v[tp],z);
+      }
+    }
+    bfly<T,R,S>(u);
+    bfly<T,R,S>(v);
+#pragma unroll
+    for(int tp=0; tp < R; ++tp) {
+      X[b+tp*NB]=u[tp];
+      Y[b+tp*NB]=v[tp];
+    }
+  }
+  if constexpr(s+1 < P::NST) {
+#pragma unroll
+    for(int pass=0; pass < 2; ++pass) {
+      cplx<T> *Z=pass == 0 ? X : Y;
+#pragma unroll
+      for(int b=0; b < NB; ++b) {
+        const int j=t+TPS*b;
+        const int kp=j & (Ns-1);
+        const int base=(j-kp)*R+kp;
+#pragma unroll
+        for(int u=0; u < R; ++u) {
+          const int e=XI::at(base+u*Ns);
+          xb[e]=Z[b+u*NB].x;
+          xb[P::PAD+e]=Z[b+u*NB].y;
+        }
+      }
+      rsync<TPS>();
+#pragma unroll
+      for(int k=0; k < V; ++k) {
+        const int e=XI::at(t+TPS*k);
+        Z[k]=mk(xb[e],xb[P::PAD+e]);
+      }
+      rsync<TPS>();
+    }
+    fftr_pair_stage<T,M,V,S,s+1>(X,Y,xb,tw,t);
+  }
+}
+
+// fftr<T,M,V,S,true> of X and of Y, their butterflies interleaved stage by
+// stage; same results, same exchange area and barriers.
+template<typename T, int M, int V, int S>
+__device__ __forceinline__ void fftr_pair(cplx<T> *X, cplx<T> *Y, T *xb, const cplx<T> *tw,
+                                          int t)
+{
+  fftr_pair_stage<T,M,V,S,0>(X,Y,xb,tw,t);
+}
+
 // In-place transform of the sequence held by threads t < TPS (v[V] each)
 // with kernel e^{S 2 pi i jk/M}; xb is the sequence's exchange area
 // (RPlan<M,V>::PAD scalar words), tw the twiddle table of RPlan<M,V>.  When
