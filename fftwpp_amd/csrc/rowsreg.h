@@ -122,8 +122,7 @@ rowreg_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows,
           X[k]=a;
           Y[k]=b;
         }
-        fftr<T,M,V,1,true>(X,xb,tw,t);
-        fftr<T,M,V,1,true>(Y,xb,tw,t);
+        fftr_pair<T,M,V,1>(X,Y,xb,tw,t);
         {
           const auto rop=bind_row(op,row);
 #pragma unroll
