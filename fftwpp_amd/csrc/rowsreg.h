@@ -345,18 +345,18 @@ hrowpk_kernel(DimArgs d, const cplx<T> *f, const cplx<T> *g, int64_t rows, int64
       T p1[V];
       hpk_fold2<T,M>(X,X2,rf,rg,tq+r1*(V+1),tq+(r2 >= 0 ? r2 : r1)*(V+1),b1,
                      r2 >= 0 ? zN[t*r2] : b1,t,r1,r2 >= 0 ? r2 : r1,H);
-      fftr<T,M,V,1,true>(X,xb,tw,t);
-#pragma unroll
-      for(int k=0; k < V; ++k) p1[k]=rop(X[k].x,X[k].y,q*(t+TPS*k)+r1);
       if(r2 >= 0) {
+        // the two residues' transforms with their butterflies interleaved
+        fftr_pair<T,M,V,1>(X,X2,xb,tw,t);
 #pragma unroll
-        for(int k=0; k < V; ++k) X[k]=X2[k];
+        for(int k=0; k < V; ++k) {
+          p1[k]=rop(X[k].x,X[k].y,q*(t+TPS*k)+r1);
+          X[k]=mk(p1[k],rop(X2[k].x,X2[k].y,q*(t+TPS*k)+r2));
+        }
+      } else {
         fftr<T,M,V,1,true>(X,xb,tw,t);
 #pragma unroll
-        for(int k=0; k < V; ++k) X[k]=mk(p1[k],rop(X[k].x,X[k].y,q*(t+TPS*k)+r2));
-      } else {
-#pragma unroll
-        for(int k=0; k < V; ++k) X[k]=mk(p1[k],(T) 0);
+        for(int k=0; k < V; ++k) X[k]=mk(rop(X[k].x,X[k].y,q*(t+TPS*k)+r1),(T) 0);
       }
       fftr<T,M,V,-1,true>(X,xb,tw,t);
       // mirror elements Z(M - sigma) through LDS
