@@ -117,6 +117,41 @@ __device__ __forceinline__ void fft256s(cplx<T> *v, T *row, int j, int h,
   dft16<T,S>(v);
 }
 
+// fft256s of two sequences v and u sharing the exchange row: the 16-point
+// DFTs and twiddles of both are independent work for the same lanes, the
+// exchanges run one after the other (the same synchronisations as two calls).
+template<typename T, int S, int WAVE, int RS>
+__device__ __forceinline__ void fft256s_pair(cplx<T> *v, cplx<T> *u, T *row, int j, int h,
+                                             const cplx<T> *tw16)
+{
+  dft16<T,S>(v);
+  dft16<T,S>(u);
+  T *wa=row+RS*j+h, *wb=row+RS*j-h;
+  const T *ra=row+(j ^ h);
+#pragma unroll
+  for(int part=0; part < 4; ++part) {
+    cplx<T> *z=part < 2 ? v : u;
+    const bool re=(part & 1) == 0;
+#pragma unroll
+    for(int e=0; e < 16; ++e) ((e & 2) ? wb : wa)[e]=re ? z[e].x : z[e].y;
+    lanes_sync<WAVE>();
+#pragma unroll
+    for(int t=0; t < 16; ++t) {
+      if(re) z[t].x=ra[RS*t];
+      else z[t].y=ra[RS*t];
+    }
+    lanes_sync<WAVE>();
+  }
+#pragma unroll
+  for(int t=1; t < 16; ++t) {
+    const cplx<T> w=(tw16+16*t)[j];
+    v[t]=S > 0 ? v[t]*w : mulc(v[t],w);
+    u[t]=S > 0 ? u[t]*w : mulc(u[t],w);
+  }
+  dft16<T,S>(v);
+  dft16<T,S>(u);
+}
+
 // The 16 x 16 stage-twiddle table tw16[16 t + j] = w_256^{t j} from the
 // dimension's table zN (N = 256 q, zN[n] = w_N^n): w_256^n = zN[q n].
 // Cooperative fill by `nthreads` threads of the block.

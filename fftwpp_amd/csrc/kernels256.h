@@ -330,8 +330,7 @@ rowconv256p_kernel(const cplx<T> *f, const cplx<T> *g, int64_t rows,
       a[k]=x*w;
       b[k]=y*w;
     }
-    fft256s<T,1,1,RS>(a,xrow,j,0,tw16);
-    fft256s<T,1,1,RS>(b,xrow,j,0,tw16);
+    fft256s_pair<T,1,1,RS>(a,b,xrow,j,0,tw16);
     {
       const auto rop=bind_row(op,row);
 #pragma unroll
